@@ -1,0 +1,162 @@
+/*
+ * qnd.h -- C ABI of libqnd, the MI355X (gfx950) split-operator TDSE hot path
+ * of QuantumND.
+ *
+ * The reference has no FFI: its hot path is a set of C functions called from
+ * the two drivers (src/QuantumND_imag.c, src/QuantumND_real.c) on
+ * MPI-distributed host arrays.  libqnd replaces that in-process seam with
+ * plain-pointer entry points over one GPU: the whole N-D grid lives in HBM,
+ * tables are analytic on the device, and many steps run per call.  Each
+ * declaration names the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every entry returns 0 on success and a non-zero QND_E* code on error;
+ *    qnd_last_error() gives the message.  The library never exits.
+ *  - One host thread per context; each context owns one HIP stream.  Calls
+ *    are stream-ordered; host-visible outputs are complete on return.
+ *  - Complex host arrays are interleaved FP64 (re, im), row-major, axis 0
+ *    slowest: the reference layout (src/meshgrid.c:205-211).
+ *  - Field tables use the reference layout [step][stage][dim] in FP64
+ *    (src/integrators.c:74-129).
+ *  - A context holds n_slots state vectors ("slots").  States are in
+ *    position representation at the interface; the internal representation
+ *    between calls is the library's business.
+ */
+#ifndef QND_H
+#define QND_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qnd_ctx qnd_ctx;
+
+/* precision of the propagated state */
+enum { QND_C128 = 0,   /* complex128 state, FP64 everywhere */
+       QND_C64 = 1 };  /* complex64 state, FP32 phases from FP64-reduced
+                          angles, FP64 reductions */
+
+enum { QND_GAUGE_LENGTH = 0, QND_GAUGE_VELOCITY = 1 };
+
+/* Observable codes.  1-4 are the reference's names
+ * (src/calculate_observables.c:48-56); 5-6 are extensions. */
+enum { QND_OBS_NONE = 0, QND_OBS_LZ = 1, QND_OBS_ENERGY = 2,
+       QND_OBS_DIPOLE_R = 3, QND_OBS_DIPOLE_V = 4,
+       QND_OBS_NORM = 5,      /* <psi|psi> */
+       QND_OBS_DIPOLE_A = 6   /* -<grad V> - E(t)<psi|psi> (Ehrenfest) */ };
+
+/* error codes */
+enum { QND_OK = 0, QND_EINVAL = 1, QND_EHIP = 2, QND_EUNKNOWN_NAME = 3,
+       QND_EGS_COLLAPSE = 4, QND_EUNSUPPORTED = 5, QND_ESTATE = 6 };
+
+/* Create a context for a dim-D grid of nr[j] points spaced dr[j] with
+ * n_slots state vectors.  Replaces the grid part of Initialize_meshgrid
+ * (src/meshgrid.c:17-179) and the fftw_mpi plan creation
+ * (src/meshgrid.c:159-167): axis grids r, p (FFT order), dNr, dNp and
+ * fft_scaling are defined exactly as there; FFT plans are this library's
+ * own Stockham passes.  No device is touched before this call. */
+int qnd_create(qnd_ctx **ctx, int dim, const int64_t *nr, const double *dr,
+               int precision, int n_slots);
+
+/* Finalize_meshgrid (src/meshgrid.c:184-199) + Finalize_integrator. */
+void qnd_destroy(qnd_ctx *ctx);
+
+/* Message of the last error of this context (or of a failed qnd_create when
+ * ctx is NULL). */
+const char *qnd_last_error(const qnd_ctx *ctx);
+
+/* Potential and complex absorbing potential: the Vpot/Vcap tables of
+ * src/meshgrid.c:123-154, evaluated analytically on the device.  softcore < 0
+ * keeps the reference constant; softcore >= 0 uses V = -(..)/sqrt(r^2 +
+ * softcore^2) (extension).  cap = {strength, width} as CAP in params.real
+ * (already |.|-ed, src/parameters.c:526-534); NULL means no CAP.
+ * Unknown name -> QND_EUNKNOWN_NAME ("Unknown potential: <name>"). */
+int qnd_set_potential(qnd_ctx *ctx, const char *name, double softcore,
+                      const double *cap);
+
+/* Initialize_integrator (src/integrators.c:13-71): Verlet, BM4, BM6, RKN4. */
+int qnd_set_integrator(qnd_ctx *ctx, const char *name, double dt);
+
+/* Extension: cos^(1/8) absorbing mask on the outer `frac` of each half-width,
+ * applied after every real-time step.  frac <= 0 disables it. */
+int qnd_set_mask(qnd_ctx *ctx, double frac);
+
+/* State transfer.  psi: host interleaved FP64 complex, position space. */
+int qnd_upload_state(qnd_ctx *ctx, int slot, const double *psi);
+int qnd_download_state(qnd_ctx *ctx, int slot, double *psi);
+
+/* Unnormalised N-D DFT of slot in place, sign -1 (forward) or +1
+ * (backward): the fftw_execute(plan_forward / plan_backward) of
+ * src/meshgrid.c:166-167 on this library's own Stockham passes.  The slot is
+ * taken in position space and left holding the transform. */
+int qnd_transform(qnd_ctx *ctx, int slot, int sign);
+
+/* nsteps imaginary-time steps of slot, each followed (gram_schmidt != 0) by
+ * modified Gram-Schmidt against slots 0..slot-1.  Replaces the loop body
+ * src/QuantumND_imag.c:182-185: Integration_step_imag
+ * (src/integrators.c:135-163) + Gram_Schmidt_orthonormalization
+ * (src/QuantumND_imag.c:265-311).  QND_EGS_COLLAPSE when the norm falls
+ * below 1e-12. */
+int qnd_imag_steps(qnd_ctx *ctx, int slot, int64_t nsteps, int gram_schmidt);
+
+/* Standalone Gram_Schmidt_orthonormalization of slot against 0..slot-1. */
+int qnd_gram_schmidt(qnd_ctx *ctx, int slot);
+
+/* Calculation_global_energies (src/calculate_observables.c:89-116):
+ * E[m-1] = Re <psi|H^m|psi>, m = 1..M. */
+int qnd_energy_moments(qnd_ctx *ctx, int slot, int M, double *E);
+
+/* Calculation_global_observables (src/calculate_observables.c:119-140) on
+ * slot: values of the requested codes, in order, real parts.  alpha = A(t)
+ * for the velocity gauge (NULL = 0); Efield = E(t) for dipole_a (NULL = 0). */
+int qnd_observables(qnd_ctx *ctx, int slot, const int *codes, int ncodes,
+                    const double *alpha, const double *Efield, double *out);
+
+/* Field tables of Initialize_field_vectors (src/integrators.c:74-129):
+ * Et, At [ntime][s][dim] at the stage times; Eobs, Aobs [ntime][dim] at the
+ * step times t_n (Field_vectors of src/QuantumND_real.c:173).  Copied. */
+int qnd_set_field(qnd_ctx *ctx, int64_t ntime, const double *Et,
+                  const double *At, const double *Eobs, const double *Aobs);
+
+/* Real-time steps step0 .. step0+nsteps-1 of slot in the given gauge.
+ * Replaces the loop body of src/QuantumND_real.c:169-186 (velocity) and
+ * :195-223 (length): Calculation_global_observables every step (the
+ * reference's dt_cum is never reset) followed by Integration_step_velocity
+ * (src/integrators.c:201-232) or Integration_step_length
+ * (src/integrators.c:170-194), then the mask extension.
+ * obs_out: [nsteps][width] host array (NULL when ncodes == 0), width = sum of
+ * widths of the codes (1 for Lz/energy/norm, dim for dipoles). */
+int qnd_real_steps(qnd_ctx *ctx, int slot, int gauge, int64_t step0,
+                   int64_t nsteps, const int *codes, int ncodes,
+                   double *obs_out);
+
+/* Write_final_observables (src/io_functions.c:136-287) numerics: radial
+ * sin^2 mask (R_mask), FFT to unitary p-space; PED[Ne] (sigma = de) and the
+ * masked p-space state psi_p (host, interleaved, native FFT order) from
+ * which the PMD records are selected.  Either output may be NULL. */
+int qnd_final_observables(qnd_ctx *ctx, int slot, const double *R_mask,
+                          const double *pr_max, int Ne, double de,
+                          double *ped, double *psi_p);
+
+/* Host-side timing on the context stream: record event idx (0..15), and the
+ * elapsed milliseconds between two recorded events (synchronizes). */
+int qnd_event_record(qnd_ctx *ctx, int idx);
+int qnd_event_elapsed(qnd_ctx *ctx, int i0, int i1, float *ms);
+int qnd_synchronize(qnd_ctx *ctx);
+
+/* Per-kernel-class profiling: when enabled, every launch of a pass class is
+ * bracketed by events; qnd_profile_read returns the count and total ms per
+ * class (classes: 0 potential/axis-0 pass, 1 kinetic/last-axis pass,
+ * 2 plain FFT pass, 3 small reduction kernels). */
+int qnd_profile_enable(qnd_ctx *ctx, int enable);
+int qnd_profile_read(qnd_ctx *ctx, int64_t *count, double *ms, int nclass);
+
+/* Library version string. */
+const char *qnd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QND_H */

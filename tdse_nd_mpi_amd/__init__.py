@@ -1,0 +1,246 @@
+"""tdse_nd_mpi_amd -- MI355X split-operator TDSE hot path of QuantumND.
+
+Python view of the C ABI in include/qnd.h (libqnd.so, built in-tree from
+tdse_nd_mpi_amd/csrc).  There is no CPU fallback: importing works anywhere,
+but creating a Grid needs the HIP library and a GPU, and fails loudly
+otherwise.
+
+The methods mirror the reference's hot-path functions (src/integrators.c,
+src/calculate_observables.c, src/QuantumND_imag.c, src/QuantumND_real.c);
+each docstring names the one it replaces.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqnd.so")
+BIN_DIR = os.path.join(HERE, "bin")
+
+C128, C64 = 0, 1
+GAUGES = {"length": 0, "velocity": 1}
+OBS = {"none": 0, "Lz": 1, "energy": 2, "dipole_r": 3, "dipole_v": 4, "norm": 5, "dipole_a": 6}
+PRECISIONS = {"c128": C128, "complex128": C128, "fp64": C128, "c64": C64, "complex64": C64, "mixed": C64}
+
+EXPORTS = ("qnd_create", "qnd_destroy", "qnd_last_error", "qnd_set_potential", "qnd_set_integrator",
+           "qnd_set_mask", "qnd_upload_state", "qnd_download_state", "qnd_imag_steps",
+           "qnd_gram_schmidt", "qnd_transform", "qnd_energy_moments", "qnd_observables", "qnd_set_field",
+           "qnd_real_steps", "qnd_final_observables", "qnd_event_record", "qnd_event_elapsed",
+           "qnd_synchronize", "qnd_profile_enable", "qnd_profile_read", "qnd_version")
+
+_lib = None
+
+
+class QNDError(RuntimeError):
+    pass
+
+
+def library():
+    """Load libqnd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise QNDError(f"{LIB_PATH} is not built: run `make -C tdse_nd_mpi_amd` "
+                       "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    i64p = C.POINTER(C.c_int64)
+    vp = C.c_void_p
+    L.qnd_create.argtypes = [C.POINTER(vp), C.c_int, i64p, dp, C.c_int, C.c_int]
+    L.qnd_destroy.argtypes = [vp]
+    L.qnd_destroy.restype = None
+    L.qnd_last_error.argtypes = [vp]
+    L.qnd_last_error.restype = C.c_char_p
+    L.qnd_set_potential.argtypes = [vp, C.c_char_p, C.c_double, dp]
+    L.qnd_set_integrator.argtypes = [vp, C.c_char_p, C.c_double]
+    L.qnd_set_mask.argtypes = [vp, C.c_double]
+    L.qnd_upload_state.argtypes = [vp, C.c_int, dp]
+    L.qnd_download_state.argtypes = [vp, C.c_int, dp]
+    L.qnd_imag_steps.argtypes = [vp, C.c_int, C.c_int64, C.c_int]
+    L.qnd_gram_schmidt.argtypes = [vp, C.c_int]
+    L.qnd_transform.argtypes = [vp, C.c_int, C.c_int]
+    L.qnd_energy_moments.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.qnd_observables.argtypes = [vp, C.c_int, ip, C.c_int, dp, dp, dp]
+    L.qnd_set_field.argtypes = [vp, C.c_int64, dp, dp, dp, dp]
+    L.qnd_real_steps.argtypes = [vp, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
+    L.qnd_final_observables.argtypes = [vp, C.c_int, dp, dp, C.c_int, C.c_double, dp, dp]
+    L.qnd_event_record.argtypes = [vp, C.c_int]
+    L.qnd_event_elapsed.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
+    L.qnd_synchronize.argtypes = [vp]
+    L.qnd_profile_enable.argtypes = [vp, C.c_int]
+    L.qnd_profile_read.argtypes = [vp, i64p, dp, C.c_int]
+    L.qnd_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _opt_d(a):
+    return None if a is None else _d(a)
+
+
+class Grid:
+    """One N-D grid on one GPU with `n_slots` state vectors.
+
+    Replaces Initialize_meshgrid (src/meshgrid.c:17-179) and the FFTW plans.
+    """
+
+    def __init__(self, nr, dr, precision="c128", n_slots=1):
+        L = library()
+        self.nr = tuple(int(n) for n in nr)
+        self.dim = len(self.nr)
+        self.shape = self.nr
+        drs = dr if hasattr(dr, "__len__") else [dr] * self.dim
+        self.dr = np.array(drs, dtype=np.float64)
+        self.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self.n_slots = n_slots
+        self.N = int(np.prod(self.nr))
+        h = C.c_void_p()
+        nr_a = np.array(self.nr, dtype=np.int64)
+        rc = L.qnd_create(C.byref(h), self.dim, nr_a.ctypes.data_as(C.POINTER(C.c_int64)),
+                          _d(self.dr), self.precision, n_slots)
+        if rc:
+            raise QNDError(L.qnd_last_error(None).decode())
+        self.h = h
+        self.s = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().qnd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc):
+        if rc:
+            raise QNDError(library().qnd_last_error(self.h).decode())
+
+    # -- set-up -------------------------------------------------------
+    def set_potential(self, name, softcore=-1.0, cap=None):
+        """Vpot/Vcap of src/meshgrid.c:123-154 (analytic on the device)."""
+        c = None if cap is None else np.array(cap, dtype=np.float64)
+        self._ck(library().qnd_set_potential(self.h, name.encode(), float(softcore), _opt_d(c)))
+
+    def set_integrator(self, name, dt):
+        """Initialize_integrator (src/integrators.c:13-71)."""
+        self._ck(library().qnd_set_integrator(self.h, name.encode(), float(dt)))
+        self.s = {"Verlet": 1, "BM4": 6, "BM6": 10, "RKN4": 6}[name]
+        self.dt = dt
+
+    def set_mask(self, frac):
+        self._ck(library().qnd_set_mask(self.h, float(frac)))
+
+    # -- states -------------------------------------------------------
+    def upload(self, slot, psi):
+        x = np.ascontiguousarray(psi, dtype=np.complex128).reshape(self.shape)
+        self._ck(library().qnd_upload_state(self.h, slot, _d(x.view(np.float64))))
+
+    def download(self, slot):
+        x = np.zeros(self.shape, dtype=np.complex128)
+        self._ck(library().qnd_download_state(self.h, slot, _d(x.view(np.float64))))
+        return x
+
+    # -- imaginary time -----------------------------------------------
+    def imag_steps(self, slot, nsteps, gram_schmidt=True):
+        """Integration_step_imag + Gram_Schmidt_orthonormalization, nsteps times
+        (src/integrators.c:135-163, src/QuantumND_imag.c:182-185,265-311)."""
+        self._ck(library().qnd_imag_steps(self.h, slot, int(nsteps), 1 if gram_schmidt else 0))
+
+    def transform(self, slot, sign):
+        """fftw_execute of the forward (-1) / backward (+1) plan on a slot."""
+        self._ck(library().qnd_transform(self.h, slot, int(sign)))
+
+    def gram_schmidt(self, slot):
+        self._ck(library().qnd_gram_schmidt(self.h, slot))
+
+    def energy_moments(self, slot, M=2):
+        """Calculation_global_energies (src/calculate_observables.c:89-116)."""
+        E = np.zeros(M)
+        self._ck(library().qnd_energy_moments(self.h, slot, M, _d(E)))
+        return E
+
+    def observables(self, slot, names, alpha=None, Efield=None):
+        """Calculation_global_observables (src/calculate_observables.c:119-140)."""
+        codes = np.array([OBS[n] for n in names], dtype=np.int32)
+        width = sum({1: 1, 2: 1, 5: 1, 3: self.dim, 4: self.dim, 6: self.dim}.get(int(c), 0)
+                    for c in codes)
+        out = np.zeros(max(width, 1))
+        al = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
+        Ef = None if Efield is None else np.ascontiguousarray(Efield, dtype=np.float64)
+        self._ck(library().qnd_observables(self.h, slot, codes.ctypes.data_as(C.POINTER(C.c_int)),
+                                           len(codes), _opt_d(al), _opt_d(Ef), _d(out)))
+        return out[:width]
+
+    # -- real time ----------------------------------------------------
+    def set_field(self, Et, At, Eobs=None, Aobs=None):
+        """Field tables [ntime][s][dim] (src/integrators.c:74-129)."""
+        Et = np.ascontiguousarray(Et, dtype=np.float64)
+        At = np.ascontiguousarray(At, dtype=np.float64)
+        nt = Et.shape[0]
+        Eo = np.zeros((nt, self.dim)) if Eobs is None else np.ascontiguousarray(Eobs, dtype=np.float64)
+        Ao = np.zeros((nt, self.dim)) if Aobs is None else np.ascontiguousarray(Aobs, dtype=np.float64)
+        self._ck(library().qnd_set_field(self.h, nt, _d(Et), _d(At), _d(Eo), _d(Ao)))
+        self.ntime = nt
+
+    def real_steps(self, slot, gauge, step0, nsteps, names=()):
+        """Observables + Integration_step_length/velocity, nsteps times
+        (src/QuantumND_real.c:169-186,195-223)."""
+        codes = np.array([OBS[n] for n in names], dtype=np.int32)
+        width = sum({1: 1, 2: 1, 5: 1, 3: self.dim, 4: self.dim, 6: self.dim}.get(int(c), 0)
+                    for c in codes)
+        out = np.zeros((max(nsteps, 1), max(width, 1)))
+        self._ck(library().qnd_real_steps(
+            self.h, slot, GAUGES[gauge], int(step0), int(nsteps),
+            codes.ctypes.data_as(C.POINTER(C.c_int)) if len(codes) else None, len(codes),
+            _d(out) if width else None))
+        return out[:nsteps, :width]
+
+    def final_observables(self, slot, R_mask, pr_max, Ne, de):
+        """Write_final_observables numerics (src/io_functions.c:136-287)."""
+        Rm = np.ascontiguousarray(R_mask, dtype=np.float64)
+        pm = np.ascontiguousarray(pr_max, dtype=np.float64)
+        ped = np.zeros(max(Ne, 1))
+        psi_p = np.zeros(self.shape, dtype=np.complex128)
+        self._ck(library().qnd_final_observables(self.h, slot, _d(Rm), _d(pm), int(Ne), float(de),
+                                                 _d(ped), _d(psi_p.view(np.float64))))
+        return ped[:Ne], psi_p
+
+    # -- timing ---------------------------------------------------------
+    def event_record(self, idx):
+        self._ck(library().qnd_event_record(self.h, idx))
+
+    def event_elapsed(self, i0, i1):
+        ms = C.c_float()
+        self._ck(library().qnd_event_elapsed(self.h, i0, i1, C.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        self._ck(library().qnd_synchronize(self.h))
+
+    def profile(self, enable):
+        self._ck(library().qnd_profile_enable(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        cnt = np.zeros(4, dtype=np.int64)
+        ms = np.zeros(4)
+        self._ck(library().qnd_profile_read(self.h, cnt.ctypes.data_as(C.POINTER(C.c_int64)), _d(ms), 4))
+        return cnt, ms
+
+
+def imag_step_count(T, dt):
+    """Steps of the reference loop for(time=0;time<T;time+=dt) (src/QuantumND_imag.c:182)."""
+    n, t = 0, 0.0
+    while t < T:
+        n += 1
+        t += dt
+    return n

@@ -1,0 +1,1294 @@
+// qnd_api.hip -- libqnd: C ABI over the gfx950 split-operator passes.
+//
+// Internal representation.  For dim >= 2 a state between passes is held as
+// M = FFT_0(psi): axis 0 in Fourier space, the other axes in position space.
+// Then one split-operator stage is exactly two HBM round trips:
+//   axis-0 pass : IFFT_0 -> (all axes in r) potential/CAP/field/mask/obs -> FFT_0
+//   kinetic block: FFT over axes 1..d-1 -> (all axes in k) kinetic -> inverse
+// and the kinetic factors of consecutive stages share one block.  For dim = 1
+// the axis-0 pass also carries the kinetic factors (KPRE/KPOST), so a stage
+// is one pass.  Slots are converted to position space lazily (download,
+// observables that need full transforms).
+#include "qnd_device.hpp"
+#include "../../include/qnd.h"
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace qnd;
+
+#define QND_VERSION "libqnd 0.1 (gfx950)"
+
+namespace {
+
+enum Rep { REP_R = 0, REP_M = 1 };
+
+struct SlotState {
+    int rep = REP_R;
+    bool pend_mask = false;  // mask owed after the last real-time step
+    bool pend_vend = false;  // velocity gauge: last V half-step owed
+};
+
+struct AxisPlan {
+    int n = 0, log2n = 0, T = 0;
+    void *tw = nullptr;  // device twiddle table, C<R>[n]
+};
+
+enum ProfClass { PC_POT = 0, PC_KIN = 1, PC_PLAIN = 2, PC_SMALL = 3, PC_N = 4 };
+
+}  // namespace
+
+struct qnd_ctx {
+    int dim = 0;
+    int64_t nr[QND_MAXD] = {1, 1, 1, 1};
+    double dr[QND_MAXD] = {0, 0, 0, 0};
+    int64_t N = 0;
+    int prec = QND_C128;
+    size_t esz = 16;
+    int n_slots = 0;
+    int device = 0;
+    size_t lds_max = 65536;
+    hipStream_t stream = nullptr;
+    Phys P{};
+    bool have_pot = false, have_int = false;
+    int s = 0;
+    double as[20] = {0}, as_scaled[20] = {0};
+    double dt = 0;
+    double mask_frac = 0;
+    void *d_states = nullptr;
+    void *d_scr[3] = {nullptr, nullptr, nullptr};
+    double *d_part = nullptr;
+    size_t part_cap = 0;
+    double *d_small = nullptr;  // coef [0,4096), gram [4096, 4096+8192), sums [12288, 16384)
+    double *d_rec = nullptr;
+    size_t rec_cap = 0;
+    double *d_prmax = nullptr;
+    std::vector<SlotState> slots;
+    AxisPlan plan[QND_MAXD];
+    std::vector<double> Et, At, Eobs, Aobs;
+    int64_t ntime = 0;
+    hipEvent_t ev[16] = {};
+    bool prof = false;
+    int64_t prof_count[PC_N] = {0};
+    double prof_ms[PC_N] = {0};
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
+    std::string err;
+};
+
+static std::string g_create_err;
+
+namespace {
+
+const int SMALL_COEF = 0, SMALL_GRAM = 4096, SMALL_SUMS = 12288, SMALL_TOTAL = 16384;
+
+int fail(qnd_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    else g_create_err = buf;
+    return code;
+}
+
+#define HIPCK(c, call)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((c), QND_EHIP, "HIP error %s at %s:%d (%s)", hipGetErrorString(e_),  \
+                        __FILE__, __LINE__, #call);                                           \
+    } while (0)
+
+int ilog2(int64_t n)
+{
+    int b = 0;
+    while (((int64_t)1 << b) < n) ++b;
+    return b;
+}
+
+template <typename R> C<R> *state_ptr(qnd_ctx *c, int slot)
+{
+    return reinterpret_cast<C<R> *>(c->d_states) + (int64_t)slot * c->N;
+}
+
+int ensure_part(qnd_ctx *c, size_t ndoubles)
+{
+    if (ndoubles <= c->part_cap) return 0;
+    if (c->d_part) HIPCK(c, hipFree(c->d_part));
+    size_t cap = ndoubles < 4096 ? 4096 : ndoubles;
+    HIPCK(c, hipMalloc(&c->d_part, cap * sizeof(double)));
+    c->part_cap = cap;
+    return 0;
+}
+
+// ---- profiling wrapper ----
+struct ProfScope {
+    qnd_ctx *c;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(qnd_ctx *c_, int cls_) : c(c_), cls(cls_)
+    {
+        if (c->prof) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~ProfScope()
+    {
+        if (c->prof) {
+            hipEventRecord(b, c->stream);
+            c->prof_pending.push_back({cls, {a, b}});
+        }
+    }
+};
+
+void prof_flush(qnd_ctx *c)
+{
+    if (c->prof_pending.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto &p : c->prof_pending) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, p.second.first, p.second.second);
+        c->prof_count[p.first] += 1;
+        c->prof_ms[p.first] += ms;
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
+    c->prof_pending.clear();
+}
+
+// ---- pass geometry ----
+struct Geo {
+    int axis, n, log2n, T, TW, threads, grid;
+    int64_t S, nlines;
+    size_t lds;
+};
+
+int pow2floor(int64_t x)
+{
+    int p = 1;
+    while ((int64_t)p * 2 <= x) p *= 2;
+    return p;
+}
+
+Geo geometry(qnd_ctx *c, int axis)
+{
+    Geo g{};
+    g.axis = axis;
+    g.n = (int)c->nr[axis];
+    g.log2n = ilog2(g.n);
+    g.T = g.n / 8;
+    g.S = 1;
+    for (int j = axis + 1; j < c->dim; ++j) g.S *= c->nr[j];
+    g.nlines = c->N / g.n;
+    const size_t line_bytes = (size_t)g.n * c->esz;
+    int twmax_lds = (int)(c->lds_max / line_bytes);
+    if (twmax_lds < 1) twmax_lds = 1;
+    int tw;
+    if (g.S == 1) {
+        tw = pow2floor(g.nlines / 1024 > 0 ? g.nlines / 1024 : 1);
+        int cap = 256 / g.T > 0 ? 256 / g.T : 1;
+        if (tw > cap) tw = cap;
+    } else {
+        const int twmin = c->esz == 8 ? 4 : 2, twmaxseg = c->esz == 8 ? 16 : 8;
+        tw = pow2floor(g.nlines / 256 > 0 ? g.nlines / 256 : 1);
+        if (tw < twmin) tw = twmin;
+        if (tw > twmaxseg) tw = twmaxseg;
+        while (tw > 1 && (g.S % tw) != 0) tw /= 2;
+    }
+    int thrcap = 1024 / g.T;
+    if (tw > thrcap) tw = thrcap;
+    if (tw > twmax_lds) tw = pow2floor(twmax_lds);
+    if ((int64_t)tw > g.nlines) tw = (int)g.nlines;
+    if (tw < 1) tw = 1;
+    g.TW = tw;
+    g.threads = g.T * tw;
+    g.grid = (int)((g.nlines + tw - 1) / tw);
+    g.lds = (size_t)tw * g.n * c->esz;
+    return g;
+}
+
+template <typename R> PassArgs<R> base_args(qnd_ctx *c, C<R> *data, const Geo &g)
+{
+    PassArgs<R> a;
+    memset(&a, 0, sizeof a);
+    a.data = data;
+    a.tw = reinterpret_cast<const C<R> *>(c->plan[g.axis].tw);
+    a.axis = g.axis;
+    a.n = g.n;
+    a.log2n = g.log2n;
+    a.T = g.T;
+    a.TW = g.TW;
+    a.ldsStride = g.n;
+    a.S = g.S;
+    a.nlines = g.nlines;
+    a.scale1 = 1.0;
+    a.scale2 = 1.0;
+    a.P = c->P;
+    a.slot_stride = c->N;
+    return a;
+}
+
+template <typename R> int launch(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, int cls)
+{
+    ProfScope ps(c, cls);
+    hipLaunchKernelGGL(k_pass<R>, dim3(g.grid), dim3(g.threads), g.lds, c->stream, a);
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+// plain transform of `data` along axis (dir -1 forward, +1 inverse), times scale
+template <typename R> int plain_pass(qnd_ctx *c, C<R> *data, int axis, int dir, double scale)
+{
+    Geo g = geometry(c, axis);
+    PassArgs<R> a = base_args<R>(c, data, g);
+    a.dir_first = dir;
+    a.scale1 = scale;
+    return launch<R>(c, g, a, PC_PLAIN);
+}
+
+// full N-D transform (unnormalised)
+template <typename R> int full_fft(qnd_ctx *c, C<R> *data, int dir)
+{
+    for (int ax = 0; ax < c->dim; ++ax) {
+        int rc = plain_pass<R>(c, data, ax, dir, 1.0);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// kinetic block for dim >= 2: FFT axes 1..d-2, kinetic pass on axis d-1,
+// inverse on axes d-2..1.
+template <typename R> int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A)
+{
+    const int d = c->dim;
+    for (int ax = 1; ax <= d - 2; ++ax) {
+        int rc = plain_pass<R>(c, data, ax, -1, 1.0);
+        if (rc) return rc;
+    }
+    Geo g = geometry(c, d - 1);
+    PassArgs<R> a = base_args<R>(c, data, g);
+    a.dir_first = -1;
+    a.flags = PF_KMID | PF_SECOND;
+    a.kmode = kmode;
+    a.ck_mid = ck;
+    a.scale2 = 1.0 / (double)c->nr[d - 1];
+    if (A)
+        for (int j = 0; j < d; ++j) a.Ak[j] = A[j];
+    int rc = launch<R>(c, g, a, PC_KIN);
+    if (rc) return rc;
+    for (int ax = d - 2; ax >= 1; --ax) {
+        rc = plain_pass<R>(c, data, ax, +1, 1.0 / (double)c->nr[ax]);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// Bring a slot to representation `want`, paying owed velocity/mask ops.
+template <typename R> int ensure_rep(qnd_ctx *c, int slot, int want)
+{
+    SlotState &st = c->slots[slot];
+    C<R> *psi = state_ptr<R>(c, slot);
+    if (st.rep == want && !st.pend_mask && !st.pend_vend) return 0;
+    if (st.rep == REP_M) {
+        // M -> R with the owed operators
+        Geo g = geometry(c, 0);
+        PassArgs<R> a = base_args<R>(c, psi, g);
+        a.dir_first = +1;
+        a.scale1 = 1.0 / (double)c->nr[0];
+        if (st.pend_vend) {
+            a.flags |= PF_V1;
+            a.vmode = VM_VELOCITY;
+            a.cv1 = c->as_scaled[2 * c->s - 1];
+        }
+        if (st.pend_mask) a.flags |= PF_MASK;
+        int rc = launch<R>(c, g, a, PC_POT);
+        if (rc) return rc;
+        st.rep = REP_R;
+        st.pend_mask = st.pend_vend = false;
+    }
+    if (want == REP_M && st.rep == REP_R) {
+        int rc = plain_pass<R>(c, psi, 0, -1, 1.0);
+        if (rc) return rc;
+        st.rep = REP_M;
+    }
+    return 0;
+}
+
+// Reduction helper: launch k_red and sum partials into d_small[SMALL_SUMS..]
+template <typename R>
+int reduce(qnd_ctx *c, int mode, const C<R> *a, C<R> *b, const C<R> *cc, int axis, double fs,
+           const double *alpha_dev, int nq, double scale, double *host_out)
+{
+    const int grid = 512, npart = QND_NOBS;
+    int rc = ensure_part(c, (size_t)grid * npart);
+    if (rc) return rc;
+    {
+        ProfScope ps(c, PC_SMALL);
+        hipLaunchKernelGGL(k_red<R>, dim3(grid), dim3(256), 0, c->stream, mode, a, b, cc, axis, fs,
+                           c->N, c->P, alpha_dev, c->d_part, npart);
+        HIPCK(c, hipGetLastError());
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, c->stream, c->d_part, grid, npart,
+                           nq, scale, c->d_small + SMALL_SUMS);
+        HIPCK(c, hipGetLastError());
+    }
+    if (host_out) {
+        HIPCK(c, hipMemcpyAsync(host_out, c->d_small + SMALL_SUMS, nq * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+template <typename R> int elem(qnd_ctx *c, int mode, C<R> *dst, const C<R> *src, int axis, double shift)
+{
+    ProfScope ps(c, PC_SMALL);
+    hipLaunchKernelGGL(k_elem<R>, dim3(1024), dim3(256), 0, c->stream, mode, dst, src, axis, shift,
+                       c->N, c->P, (const C<R> *)nullptr, (int64_t)0, 0, (const double *)nullptr);
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+template <typename R> int copy_state(qnd_ctx *c, C<R> *dst, const C<R> *src)
+{
+    HIPCK(c, hipMemcpyAsync(dst, src, c->N * sizeof(C<R>), hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// energies (src/calculate_observables.c:146-175)
+// ---------------------------------------------------------------------
+template <typename R> int energies(qnd_ctx *c, int slot, int M, double *E)
+{
+    int rc = ensure_rep<R>(c, slot, REP_R);
+    if (rc) return rc;
+    C<R> *psi = state_ptr<R>(c, slot);
+    C<R> *H = reinterpret_cast<C<R> *>(c->d_scr[0]);
+    C<R> *A = reinterpret_cast<C<R> *>(c->d_scr[1]);
+    double fs = 1.0;
+    for (int j = 0; j < c->dim; ++j) fs /= (double)c->nr[j];
+    if ((rc = copy_state<R>(c, H, psi))) return rc;
+    for (int m = 0; m < M; ++m) {
+        if ((rc = copy_state<R>(c, A, H))) return rc;
+        if ((rc = full_fft<R>(c, A, -1))) return rc;
+        if ((rc = elem<R>(c, EM_MUL_KIN, A, A, 0, 0.0))) return rc;
+        if ((rc = full_fft<R>(c, A, +1))) return rc;
+        double out[2];
+        if ((rc = reduce<R>(c, RM_HCOMB, psi, H, A, 0, fs, nullptr, 2, c->P.dNr, out))) return rc;
+        E[m] = out[0];
+    }
+    return 0;
+}
+
+// Lz (src/calculate_observables.c:178-230), dim >= 2
+template <typename R> int lz(qnd_ctx *c, int slot, const double *alpha, double *out)
+{
+    int rc = ensure_rep<R>(c, slot, REP_R);
+    if (rc) return rc;
+    C<R> *psi = state_ptr<R>(c, slot);
+    C<R> *Pp = reinterpret_cast<C<R> *>(c->d_scr[0]);
+    C<R> *B = reinterpret_cast<C<R> *>(c->d_scr[1]);
+    double fs = 1.0;
+    for (int j = 0; j < c->dim; ++j) fs /= (double)c->nr[j];
+    if ((rc = copy_state<R>(c, Pp, psi))) return rc;
+    if ((rc = full_fft<R>(c, Pp, -1))) return rc;
+    double r1[2], r2[2];
+    if ((rc = elem<R>(c, EM_MUL_PSHIFT, B, Pp, 1, alpha ? alpha[1] : 0.0))) return rc;
+    if ((rc = full_fft<R>(c, B, +1))) return rc;
+    if ((rc = reduce<R>(c, RM_DOTX, psi, B, nullptr, 0, 0, nullptr, 2, 1.0, r1))) return rc;
+    if ((rc = elem<R>(c, EM_MUL_PSHIFT, B, Pp, 0, alpha ? alpha[0] : 0.0))) return rc;
+    if ((rc = full_fft<R>(c, B, +1))) return rc;
+    if ((rc = reduce<R>(c, RM_DOTX, psi, B, nullptr, 1, 0, nullptr, 2, 1.0, r2))) return rc;
+    *out = (r1[0] - r2[0]) * fs * c->P.dNr;
+    return 0;
+}
+
+int obs_width(int code, int dim)
+{
+    switch (code) {
+    case QND_OBS_LZ: case QND_OBS_ENERGY: case QND_OBS_NORM: return 1;
+    case QND_OBS_DIPOLE_R: case QND_OBS_DIPOLE_V: case QND_OBS_DIPOLE_A: return dim;
+    default: return 0;
+    }
+}
+
+// Calculation_global_observables (src/calculate_observables.c:119-140)
+template <typename R>
+int observables(qnd_ctx *c, int slot, const int *codes, int ncodes, const double *alpha,
+                const double *Efield, double *out)
+{
+    int rc = 0;
+    int off = 0;
+    const int d = c->dim;
+    double zero[QND_MAXD] = {0, 0, 0, 0};
+    if (!alpha) alpha = zero;
+    if (!Efield) Efield = zero;
+    for (int k = 0; k < ncodes; ++k) {
+        const int code = codes[k];
+        if (code == QND_OBS_LZ) {
+            if (d <= 1) out[off] = 0.0;
+            else if ((rc = lz<R>(c, slot, alpha, &out[off]))) return rc;
+        } else if (code == QND_OBS_ENERGY) {
+            if ((rc = energies<R>(c, slot, 1, &out[off]))) return rc;
+        } else if (code == QND_OBS_DIPOLE_R || code == QND_OBS_NORM || code == QND_OBS_DIPOLE_A) {
+            if ((rc = ensure_rep<R>(c, slot, REP_R))) return rc;
+            C<R> *psi = state_ptr<R>(c, slot);
+            double v[QND_NOBS];
+            if ((rc = reduce<R>(c, RM_OBS, psi, nullptr, nullptr, 0, 0, nullptr, QND_NOBS, c->P.dNr, v)))
+                return rc;
+            if (code == QND_OBS_NORM) out[off] = v[0];
+            else if (code == QND_OBS_DIPOLE_R)
+                for (int j = 0; j < d; ++j) out[off + j] = v[1 + j];
+            else
+                for (int j = 0; j < d; ++j) out[off + j] = -v[1 + QND_MAXD + j] - Efield[j] * v[0];
+        } else if (code == QND_OBS_DIPOLE_V) {
+            if ((rc = ensure_rep<R>(c, slot, REP_R))) return rc;
+            C<R> *psi = state_ptr<R>(c, slot);
+            C<R> *A = reinterpret_cast<C<R> *>(c->d_scr[0]);
+            if ((rc = copy_state<R>(c, A, psi))) return rc;
+            if ((rc = full_fft<R>(c, A, -1))) return rc;
+            double *dal = c->d_small + SMALL_SUMS + 64;
+            HIPCK(c, hipMemcpyAsync(dal, alpha, d * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            double fs = 1.0, dNp = 1.0;
+            for (int j = 0; j < d; ++j) {
+                fs /= (double)c->nr[j];
+                dNp *= 2. * M_PI / (double)(c->dr[j] * c->nr[j]);
+            }
+            double v[QND_MAXD];
+            if ((rc = reduce<R>(c, RM_PMOM, A, nullptr, nullptr, 0, 0, dal, d, dNp * fs, v))) return rc;
+            for (int j = 0; j < d; ++j) out[off + j] = v[j];
+        }
+        off += obs_width(code, d);
+    }
+    return 0;
+}
+
+// Gram matrix of slots 0..nprev-1 in the M representation, into d_small.
+template <typename R> int gram(qnd_ctx *c, int nprev)
+{
+    std::vector<double> G(2 * (size_t)nprev * nprev, 0.0);
+    const double scale = c->P.dNr / (double)c->nr[0];
+    for (int i = 0; i < nprev; ++i)
+        for (int j = 0; j < nprev; ++j) {
+            double v[2];
+            int rc = reduce<R>(c, RM_DOT, state_ptr<R>(c, i), state_ptr<R>(c, j), nullptr, 0, 0, nullptr,
+                               2, scale, v);
+            if (rc) return rc;
+            G[2 * (i * nprev + j)] = v[0];
+            G[2 * (i * nprev + j) + 1] = v[1];
+        }
+    HIPCK(c, hipMemcpyAsync(c->d_small + SMALL_GRAM, G.data(), G.size() * sizeof(double),
+                            hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// imaginary time (src/integrators.c:135-163 + src/QuantumND_imag.c:265-311)
+// ---------------------------------------------------------------------
+template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int gs)
+{
+    const int d = c->dim, s = c->s;
+    const int nprev = gs ? slot : 0;
+    int rc;
+    for (int j = 0; j < nprev; ++j)
+        if ((rc = ensure_rep<R>(c, j, REP_M))) return rc;
+    if ((rc = ensure_rep<R>(c, slot, REP_M))) return rc;
+    if (gs && nprev > 0 && (rc = gram<R>(c, nprev))) return rc;
+    C<R> *psi = state_ptr<R>(c, slot);
+    double *coef = c->d_small + SMALL_COEF;
+    HIPCK(c, hipMemsetAsync(coef, 0, (2 * nprev + 3) * sizeof(double), c->stream));
+    const Geo g0 = geometry(c, 0);
+    const int npart = 2 * (nprev + 1);
+    if (gs && (rc = ensure_part(c, (size_t)g0.grid * npart))) return rc;
+    const double inv_n0 = 1.0 / (double)c->nr[0];
+    bool pending = false;
+    for (int64_t it = 0; it < nsteps; ++it) {
+        for (int ia = 0; ia < s; ++ia) {
+            PassArgs<R> a = base_args<R>(c, psi, g0);
+            a.dir_first = +1;
+            a.scale1 = inv_n0;
+            a.flags = PF_V2 | PF_SECOND;
+            a.vmode = VM_IMAG;
+            a.cv2 = c->as_scaled[2 * ia];
+            if (ia > 0) {
+                a.flags |= PF_V1;
+                a.cv1 = c->as_scaled[2 * ia - 1];
+            }
+            if (ia == 0 && pending) {
+                a.flags |= PF_GS_UPDATE;
+                a.prev = state_ptr<R>(c, 0);
+                a.nprev = nprev;
+                a.gs_coef = coef;
+                pending = false;
+            }
+            const double ck = c->as_scaled[2 * ia] + c->as_scaled[2 * ia + 1];
+            if (d == 1) {
+                a.flags |= PF_KPOST;
+                a.kmode = KM_IMAG;
+                a.ck_post = ck;
+            }
+            if ((rc = launch<R>(c, g0, a, PC_POT))) return rc;
+            if (d > 1 && (rc = kinetic_block<R>(c, psi, KM_IMAG, ck, nullptr))) return rc;
+        }
+        // last potential half-step, then the Gram-Schmidt partial dots
+        PassArgs<R> a = base_args<R>(c, psi, g0);
+        a.dir_first = +1;
+        a.scale1 = inv_n0;
+        a.flags = PF_V1 | PF_SECOND;
+        a.vmode = VM_IMAG;
+        a.cv1 = c->as_scaled[2 * s - 1];
+        if (gs) {
+            a.flags |= PF_GS_DOTS;
+            a.prev = state_ptr<R>(c, 0);
+            a.nprev = nprev;
+            a.partials = c->d_part;
+            a.npart = npart;
+        }
+        if ((rc = launch<R>(c, g0, a, PC_POT))) return rc;
+        if (gs) {
+            ProfScope ps(c, PC_SMALL);
+            hipLaunchKernelGGL(k_gs_coeffs, dim3(1), dim3(128), 0, c->stream, c->d_part, g0.grid, npart,
+                               nprev, c->P.dNr * inv_n0, c->d_small + SMALL_GRAM, coef);
+            HIPCK(c, hipGetLastError());
+            pending = true;
+        }
+    }
+    if (pending) {
+        ProfScope ps(c, PC_SMALL);
+        hipLaunchKernelGGL(k_elem<R>, dim3(1024), dim3(256), 0, c->stream, (int)EM_GS_UPDATE, psi,
+                           (const C<R> *)psi, 0, 0.0, c->N, c->P, (const C<R> *)state_ptr<R>(c, 0),
+                           c->N, nprev, (const double *)coef);
+        HIPCK(c, hipGetLastError());
+    }
+    if (gs) {
+        double flag = 0;
+        HIPCK(c, hipMemcpyAsync(&flag, coef + 2 * nprev + 1, sizeof(double), hipMemcpyDeviceToHost,
+                                c->stream));
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        if (flag != 0.0)
+            return fail(c, QND_EGS_COLLAPSE,
+                        "ERROR: Wavefunction norm decayed to zero during Gram-Schmidt for state %d.", slot);
+    }
+    prof_flush(c);
+    return 0;
+}
+
+// standalone Gram-Schmidt in position space (no step)
+template <typename R> int gram_schmidt(qnd_ctx *c, int slot)
+{
+    int rc;
+    for (int j = 0; j <= slot; ++j)
+        if ((rc = ensure_rep<R>(c, j, REP_R))) return rc;
+    C<R> *v = state_ptr<R>(c, slot);
+    // sequential modified Gram-Schmidt, as the reference
+    std::vector<double> coef(2 * slot + 3, 0.0);
+    for (int j = 0; j < slot; ++j) {
+        double b[2];
+        if ((rc = reduce<R>(c, RM_DOT, state_ptr<R>(c, j), v, nullptr, 0, 0, nullptr, 2, c->P.dNr, b)))
+            return rc;
+        std::vector<double> cj(3, 0.0);
+        cj[0] = b[0];
+        cj[1] = b[1];
+        cj[2] = 1.0;
+        HIPCK(c, hipMemcpyAsync(c->d_small + SMALL_COEF, cj.data(), 3 * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream));
+        ProfScope ps(c, PC_SMALL);
+        hipLaunchKernelGGL(k_elem<R>, dim3(1024), dim3(256), 0, c->stream, (int)EM_GS_UPDATE, v,
+                           (const C<R> *)v, 0, 0.0, c->N, c->P, (const C<R> *)state_ptr<R>(c, j),
+                           c->N, 1, (const double *)(c->d_small + SMALL_COEF));
+        HIPCK(c, hipGetLastError());
+    }
+    double nv[2];
+    if ((rc = reduce<R>(c, RM_DOT, v, v, nullptr, 0, 0, nullptr, 2, c->P.dNr, nv))) return rc;
+    if (nv[0] < 1.e-12)
+        return fail(c, QND_EGS_COLLAPSE,
+                    "ERROR: Wavefunction norm decayed to zero during Gram-Schmidt for state %d.", slot);
+    if ((rc = elem<R>(c, EM_SCALE, v, v, 0, 1.0 / std::sqrt(nv[0])))) return rc;
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// real time (src/QuantumND_real.c:166-228)
+// ---------------------------------------------------------------------
+bool fused_codes(const int *codes, int ncodes)
+{
+    for (int k = 0; k < ncodes; ++k)
+        if (!(codes[k] == QND_OBS_NORM || codes[k] == QND_OBS_DIPOLE_R || codes[k] == QND_OBS_DIPOLE_A ||
+              codes[k] == QND_OBS_NONE))
+            return false;
+    return true;
+}
+
+template <typename R>
+int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, const int *codes,
+               int ncodes, double *obs_out)
+{
+    const int d = c->dim, s = c->s;
+    int rc;
+    if (step0 < 0 || step0 + nsteps > c->ntime)
+        return fail(c, QND_EINVAL, "steps [%lld, %lld) outside the field table of %lld steps",
+                    (long long)step0, (long long)(step0 + nsteps), (long long)c->ntime);
+    int width = 0;
+    for (int k = 0; k < ncodes; ++k) width += obs_width(codes[k], d);
+    const bool fused = fused_codes(codes, ncodes);
+    const bool want_obs = width > 0;
+    if (want_obs && fused) {
+        size_t need = (size_t)nsteps * width;
+        if (need > c->rec_cap) {
+            if (c->d_rec) HIPCK(c, hipFree(c->d_rec));
+            HIPCK(c, hipMalloc(&c->d_rec, need * sizeof(double)));
+            c->rec_cap = need;
+        }
+    }
+    SlotState &st = c->slots[slot];
+    C<R> *psi = state_ptr<R>(c, slot);
+    const Geo g0 = geometry(c, 0);
+    if ((rc = ensure_part(c, (size_t)g0.grid * QND_NOBS))) return rc;
+    const double inv_n0 = 1.0 / (double)c->nr[0];
+    ObsSpec spec;
+    memset(&spec, 0, sizeof spec);
+    spec.ncodes = ncodes < 8 ? ncodes : 8;
+    for (int k = 0; k < spec.ncodes; ++k) spec.codes[k] = codes[k];
+    spec.dNr = c->P.dNr;
+    if (ncodes > 8) return fail(c, QND_EINVAL, "at most 8 observables per step");
+
+    for (int64_t it = step0; it < step0 + nsteps; ++it) {
+        const double *Et = &c->Et[(size_t)it * s * d];
+        const double *At = &c->At[(size_t)it * s * d];
+        const double *Eo = &c->Eobs[(size_t)it * d];
+        const double *Ao = &c->Aobs[(size_t)it * d];
+        // --- observables at the step boundary (unfused codes: position-space evaluation) ---
+        if (want_obs && !fused) {
+            if ((rc = observables<R>(c, slot, codes, ncodes, gauge == QND_GAUGE_VELOCITY ? Ao : nullptr, Eo,
+                                     obs_out + (it - step0) * width)))
+                return rc;
+        }
+        if ((rc = ensure_rep<R>(c, slot, REP_M)) && st.rep != REP_M) return rc;
+        if (gauge == QND_GAUGE_LENGTH) {
+            // boundary pass: owed mask, observables
+            PassArgs<R> a = base_args<R>(c, psi, g0);
+            a.dir_first = +1;
+            a.scale1 = inv_n0;
+            a.flags = PF_SECOND;
+            if (st.pend_mask) a.flags |= PF_MASK;
+            if (want_obs && fused) {
+                a.flags |= PF_OBS;
+                a.partials = c->d_part;
+                a.npart = QND_NOBS;
+            }
+            if (a.flags != PF_SECOND) {
+                if ((rc = launch<R>(c, g0, a, PC_POT))) return rc;
+            }
+            st.pend_mask = false;
+            if (want_obs && fused) {
+                for (int j = 0; j < d; ++j) spec.E[j] = Eo[j];
+                ProfScope ps(c, PC_SMALL);
+                hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(64), 0, c->stream, c->d_part, g0.grid,
+                                   QND_NOBS, d, spec, c->d_rec + (it - step0) * width);
+                HIPCK(c, hipGetLastError());
+            }
+            for (int ia = 0; ia < s; ++ia) {
+                const double a0 = c->as_scaled[2 * ia], a1 = c->as_scaled[2 * ia + 1];
+                if (d > 1 && ia == 0 && (rc = kinetic_block<R>(c, psi, KM_PHASE, a0, nullptr))) return rc;
+                PassArgs<R> b = base_args<R>(c, psi, g0);
+                b.dir_first = +1;
+                b.scale1 = inv_n0;
+                b.flags = PF_V1 | PF_SECOND;
+                b.vmode = VM_LENGTH;
+                b.cv1 = a0 + a1;
+                for (int j = 0; j < d; ++j) b.Ev[j] = Et[ia * d + j];
+                if (d == 1) {
+                    b.flags |= PF_KPRE | PF_KPOST;
+                    b.kmode = KM_PHASE;
+                    b.ck_pre = a0;
+                    b.ck_post = a1;
+                }
+                if ((rc = launch<R>(c, g0, b, PC_POT))) return rc;
+                if (d > 1) {
+                    double ck = a1 + (ia < s - 1 ? c->as_scaled[2 * ia + 2] : 0.0);
+                    if ((rc = kinetic_block<R>(c, psi, KM_PHASE, ck, nullptr))) return rc;
+                }
+            }
+            st.pend_mask = c->P.mask_on != 0;
+        } else {
+            // velocity gauge: boundary pass carries the owed last half-step,
+            // the owed mask, the observables and the first half-step
+            for (int ia = 0; ia < s; ++ia) {
+                PassArgs<R> a = base_args<R>(c, psi, g0);
+                a.dir_first = +1;
+                a.scale1 = inv_n0;
+                a.flags = PF_V2 | PF_SECOND;
+                a.vmode = VM_VELOCITY;
+                a.cv2 = c->as_scaled[2 * ia];
+                if (ia == 0) {
+                    if (st.pend_vend) {
+                        a.flags |= PF_V1;
+                        a.cv1 = c->as_scaled[2 * s - 1];
+                    }
+                    if (st.pend_mask) a.flags |= PF_MASK;
+                    if (want_obs && fused) {
+                        a.flags |= PF_OBS;
+                        a.partials = c->d_part;
+                        a.npart = QND_NOBS;
+                    }
+                } else {
+                    a.flags |= PF_V1;
+                    a.cv1 = c->as_scaled[2 * ia - 1];
+                }
+                const double ck = c->as_scaled[2 * ia] + c->as_scaled[2 * ia + 1];
+                if (d == 1) {
+                    a.flags |= PF_KPOST;
+                    a.kmode = KM_VELOCITY;
+                    a.ck_post = ck;
+                    a.Ak[0] = At[ia * d];
+                }
+                if ((rc = launch<R>(c, g0, a, PC_POT))) return rc;
+                if (ia == 0) {
+                    st.pend_vend = st.pend_mask = false;
+                    if (want_obs && fused) {
+                        for (int j = 0; j < d; ++j) spec.E[j] = Eo[j];
+                        ProfScope ps(c, PC_SMALL);
+                        hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(64), 0, c->stream, c->d_part,
+                                           g0.grid, QND_NOBS, d, spec, c->d_rec + (it - step0) * width);
+                        HIPCK(c, hipGetLastError());
+                    }
+                }
+                if (d > 1 && (rc = kinetic_block<R>(c, psi, KM_VELOCITY, ck, &At[ia * d]))) return rc;
+            }
+            st.pend_vend = true;
+            st.pend_mask = c->P.mask_on != 0;
+        }
+    }
+    if (want_obs && fused) {
+        HIPCK(c, hipMemcpyAsync(obs_out, c->d_rec, (size_t)nsteps * width * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    prof_flush(c);
+    return 0;
+}
+
+// Write_final_observables numerics (src/io_functions.c:159-199)
+template <typename R>
+int final_observables(qnd_ctx *c, int slot, const double *R_mask, const double *pr_max, int Ne, double de,
+                      double *ped, double *psi_p)
+{
+    int rc = ensure_rep<R>(c, slot, REP_R);
+    if (rc) return rc;
+    C<R> *psi = state_ptr<R>(c, slot);
+    C<R> *A = reinterpret_cast<C<R> *>(c->d_scr[0]);
+    // radial sin^2 mask on the host-visible grid, applied on the device
+    // through a host-built per-point table would cost N doubles; instead the
+    // mask is computed here on the host and the state is masked in place on
+    // a copy.
+    const int64_t N = c->N;
+    std::vector<C<R>> h(N);
+    HIPCK(c, hipMemcpyAsync(h.data(), psi, N * sizeof(C<R>), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    for (int64_t f = 0; f < N; ++f) {
+        int64_t rem = f;
+        double rr = 0.0;
+        for (int j = c->dim - 1; j >= 0; --j) {
+            int64_t i = rem % c->nr[j];
+            rem /= c->nr[j];
+            double x = -.5 * c->dr[j] * c->nr[j] + i * c->dr[j];
+            rr += x * x;
+        }
+        double rad = std::sqrt(rr);
+        double fm = 0.0;
+        if (rad >= R_mask[0] && rad <= R_mask[1])
+            fm = std::pow(std::sin(.5 * M_PI * (rad - R_mask[0]) / (R_mask[1] - R_mask[0])), 2.);
+        else if (rad > R_mask[1])
+            fm = 1.0;
+        h[f].x = (R)(fm * (double)h[f].x);
+        h[f].y = (R)(fm * (double)h[f].y);
+    }
+    HIPCK(c, hipMemcpyAsync(A, h.data(), N * sizeof(C<R>), hipMemcpyHostToDevice, c->stream));
+    if ((rc = full_fft<R>(c, A, -1))) return rc;
+    double fs = 1.0;
+    for (int j = 0; j < c->dim; ++j) fs /= (double)c->nr[j];
+    if ((rc = elem<R>(c, EM_SCALE, A, A, 0, std::sqrt(fs)))) return rc;
+    if (ped && Ne > 0) {
+        const int chunk = 1024;
+        const int grid = (int)((N + chunk - 1) / chunk);
+        if ((rc = ensure_part(c, (size_t)grid * Ne))) return rc;
+        double pm[QND_MAXD] = {0, 0, 0, 0};
+        for (int j = 0; j < c->dim; ++j) pm[j] = pr_max[j];
+        if (!c->d_prmax) HIPCK(c, hipMalloc(&c->d_prmax, QND_MAXD * sizeof(double)));
+        HIPCK(c, hipMemcpyAsync(c->d_prmax, pm, sizeof pm, hipMemcpyHostToDevice, c->stream));
+        const double sigma = de;
+        const double nf = 1. / std::pow(2. * M_PI * sigma * sigma, .5 * c->dim);
+        hipLaunchKernelGGL(k_ped<R>, dim3(grid), dim3(256), 0, c->stream, (const C<R> *)A, N, c->P,
+                           (const double *)c->d_prmax, Ne, de, sigma, nf, chunk, c->d_part);
+        HIPCK(c, hipGetLastError());
+        double *sums = nullptr;
+        HIPCK(c, hipMalloc(&sums, Ne * sizeof(double)));
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->d_part, grid, Ne, Ne, 1.0,
+                           sums);
+        HIPCK(c, hipGetLastError());
+        HIPCK(c, hipMemcpyAsync(ped, sums, Ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        HIPCK(c, hipFree(sums));
+    }
+    if (psi_p) {
+        HIPCK(c, hipMemcpyAsync(h.data(), A, N * sizeof(C<R>), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        for (int64_t f = 0; f < N; ++f) {
+            psi_p[2 * f] = (double)h[f].x;
+            psi_p[2 * f + 1] = (double)h[f].y;
+        }
+    }
+    return 0;
+}
+
+template <typename R> int upload(qnd_ctx *c, int slot, const double *psi)
+{
+    const int64_t N = c->N;
+    C<R> *dst = state_ptr<R>(c, slot);
+    if (sizeof(R) == 8) {
+        HIPCK(c, hipMemcpyAsync(dst, psi, N * 16, hipMemcpyHostToDevice, c->stream));
+    } else {
+        std::vector<C<R>> h(N);
+        for (int64_t f = 0; f < N; ++f) {
+            h[f].x = (R)psi[2 * f];
+            h[f].y = (R)psi[2 * f + 1];
+        }
+        HIPCK(c, hipMemcpyAsync(dst, h.data(), N * sizeof(C<R>), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->slots[slot] = SlotState();
+    return 0;
+}
+
+template <typename R> int download(qnd_ctx *c, int slot, double *psi)
+{
+    int rc = ensure_rep<R>(c, slot, REP_R);
+    if (rc) return rc;
+    const int64_t N = c->N;
+    C<R> *src = state_ptr<R>(c, slot);
+    if (sizeof(R) == 8) {
+        HIPCK(c, hipMemcpyAsync(psi, src, N * 16, hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<C<R>> h(N);
+        HIPCK(c, hipMemcpyAsync(h.data(), src, N * sizeof(C<R>), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        for (int64_t f = 0; f < N; ++f) {
+            psi[2 * f] = (double)h[f].x;
+            psi[2 * f + 1] = (double)h[f].y;
+        }
+    }
+    return 0;
+}
+
+template <typename R> int make_twiddles(qnd_ctx *c, int axis)
+{
+    AxisPlan &p = c->plan[axis];
+    for (int j = 0; j < axis; ++j)
+        if (c->plan[j].n == p.n && c->plan[j].tw) {  // share
+            p.tw = nullptr;
+            p.tw = c->plan[j].tw;
+            return 0;
+        }
+    std::vector<C<R>> t(p.n);
+    for (int k = 0; k < p.n; ++k) {
+        long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)p.n;
+        t[k].x = (R)cosl(ang);
+        t[k].y = (R)sinl(ang);
+    }
+    HIPCK(c, hipMalloc(&p.tw, p.n * sizeof(C<R>)));
+    HIPCK(c, hipMemcpy(p.tw, t.data(), p.n * sizeof(C<R>), hipMemcpyHostToDevice));
+    return 0;
+}
+
+bool potential_params(const char *name, int *kind, double *b, double *c2)
+{
+    *kind = 0;
+    *b = 0.0;
+    if (!strcmp(name, "H_1D")) *c2 = 2.;
+    else if (!strcmp(name, "H_3D")) *c2 = 1e-4;
+    else if (!strcmp(name, "H_2D")) *c2 = .7396;
+    else if (!strcmp(name, "Ne_2D")) { *b = 9.; *c2 = 2.88172; }
+    else if (!strcmp(name, "Ar_2D")) { *b = 17.; *c2 = .91205; }
+    else if (!strcmp(name, "Kr_2D")) { *b = 35.; *c2 = .7875; }
+    else if (!strcmp(name, "Xe_2D")) { *b = 53.; *c2 = .52592; }
+    else if (!strcmp(name, "Ne_short_2D")) { *kind = 1; *c2 = 0.; }
+    else return false;
+    return true;
+}
+
+int check_ready(qnd_ctx *c)
+{
+    if (!c) return QND_EINVAL;
+    if (!c->have_pot) return fail(c, QND_ESTATE, "potential not set (qnd_set_potential)");
+    if (!c->have_int) return fail(c, QND_ESTATE, "integrator not set (qnd_set_integrator)");
+    return 0;
+}
+
+int check_slot(qnd_ctx *c, int slot)
+{
+    if (slot < 0 || slot >= c->n_slots) return fail(c, QND_EINVAL, "slot %d out of range [0,%d)", slot, c->n_slots);
+    return 0;
+}
+
+}  // namespace
+
+// =====================================================================
+// C ABI
+// =====================================================================
+#define DISPATCH(c, fn, ...) ((c)->prec == QND_C64 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
+
+extern "C" {
+
+const char *qnd_version(void) { return QND_VERSION; }
+
+const char *qnd_last_error(const qnd_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int precision, int n_slots)
+{
+    if (!out) return fail(nullptr, QND_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (dim < 1 || dim > QND_MAXD) return fail(nullptr, QND_EINVAL, "dimension %d not in [1,%d]", dim, QND_MAXD);
+    if (precision != QND_C128 && precision != QND_C64)
+        return fail(nullptr, QND_EINVAL, "unknown precision %d", precision);
+    if (n_slots < 1) return fail(nullptr, QND_EINVAL, "n_slots must be >= 1");
+    for (int j = 0; j < dim; ++j) {
+        if (nr[j] < 8 || (nr[j] & (nr[j] - 1)) != 0)
+            return fail(nullptr, QND_EUNSUPPORTED,
+                        "NR[%d] = %lld: the GPU path needs a power of two >= 8", j, (long long)nr[j]);
+        if (!(dr[j] > 0)) return fail(nullptr, QND_EINVAL, "DR[%d] must be positive", j);
+    }
+    qnd_ctx *c = new qnd_ctx();
+    c->dim = dim;
+    c->prec = precision;
+    c->esz = precision == QND_C64 ? 8 : 16;
+    c->n_slots = n_slots;
+    c->N = 1;
+    for (int j = 0; j < dim; ++j) {
+        c->nr[j] = nr[j];
+        c->dr[j] = dr[j];
+        c->N *= nr[j];
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        delete c;
+        return fail(nullptr, QND_EHIP, "no HIP device available");
+    }
+    hipGetDevice(&c->device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
+        size_t m = prop.sharedMemPerBlock;
+        c->lds_max = m > 65536 ? 65536 : m;
+    }
+    for (int j = 0; j < dim; ++j) {
+        if ((size_t)nr[j] * c->esz > c->lds_max || nr[j] / 8 > 1024) {
+            delete c;
+            return fail(nullptr, QND_EUNSUPPORTED, "NR[%d] = %lld exceeds the single-pass line limit", j,
+                        (long long)nr[j]);
+        }
+    }
+    // physics defaults
+    Phys &P = c->P;
+    memset(&P, 0, sizeof P);
+    P.dim = dim;
+    P.dNr = 1.0;
+    for (int j = 0; j < QND_MAXD; ++j) {
+        P.n[j] = 1;
+        P.capL[j] = 1e300;
+        P.mask_x0[j] = 1e300;
+        P.mask_w[j] = 1.0;
+    }
+    for (int j = 0; j < dim; ++j) {
+        P.n[j] = (int)nr[j];
+        P.dr[j] = dr[j];
+        P.L[j] = dr[j] * nr[j];
+        P.dp[j] = 2. * M_PI / P.L[j];
+        P.dNr *= dr[j];
+    }
+    c->slots.resize(n_slots);
+    int rc = 0;
+    auto bail = [&](int code) {
+        qnd_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(nullptr, QND_EHIP, "hipStreamCreate failed"));
+    size_t sbytes = (size_t)c->N * c->esz;
+    if (hipMalloc(&c->d_states, sbytes * n_slots) != hipSuccess)
+        return bail(fail(nullptr, QND_EHIP, "hipMalloc of %zu bytes for states failed", sbytes * n_slots));
+    hipMemset(c->d_states, 0, sbytes * n_slots);
+    for (int k = 0; k < 3; ++k)
+        if (hipMalloc(&c->d_scr[k], sbytes) != hipSuccess)
+            return bail(fail(nullptr, QND_EHIP, "hipMalloc of scratch failed"));
+    if (hipMalloc(&c->d_small, SMALL_TOTAL * sizeof(double)) != hipSuccess)
+        return bail(fail(nullptr, QND_EHIP, "hipMalloc failed"));
+    hipMemset(c->d_small, 0, SMALL_TOTAL * sizeof(double));
+    for (int j = 0; j < dim; ++j) {
+        c->plan[j].n = (int)nr[j];
+        c->plan[j].log2n = ilog2(nr[j]);
+        c->plan[j].T = (int)nr[j] / 8;
+        rc = c->prec == QND_C64 ? make_twiddles<float>(c, j) : make_twiddles<double>(c, j);
+        if (rc) {
+            g_create_err = c->err;
+            return bail(rc);
+        }
+    }
+    for (int k = 0; k < 16; ++k) hipEventCreate(&c->ev[k]);
+    *out = c;
+    return 0;
+}
+
+void qnd_destroy(qnd_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    prof_flush(c);
+    for (int j = 0; j < QND_MAXD; ++j) {
+        if (!c->plan[j].tw) continue;
+        bool shared = false;
+        for (int k = 0; k < j; ++k)
+            if (c->plan[k].tw == c->plan[j].tw) shared = true;
+        if (!shared) hipFree(c->plan[j].tw);
+    }
+    if (c->d_states) hipFree(c->d_states);
+    for (int k = 0; k < 3; ++k)
+        if (c->d_scr[k]) hipFree(c->d_scr[k]);
+    if (c->d_part) hipFree(c->d_part);
+    if (c->d_small) hipFree(c->d_small);
+    if (c->d_rec) hipFree(c->d_rec);
+    if (c->d_prmax) hipFree(c->d_prmax);
+    for (int k = 0; k < 16; ++k)
+        if (c->ev[k]) hipEventDestroy(c->ev[k]);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int qnd_set_potential(qnd_ctx *c, const char *name, double softcore, const double *cap)
+{
+    if (!c || !name) return QND_EINVAL;
+    int kind;
+    double b, c2;
+    if (!potential_params(name, &kind, &b, &c2))
+        return fail(c, QND_EUNKNOWN_NAME, "Unknown potential: %s", name);
+    if (softcore >= 0) c2 = softcore * softcore;
+    c->P.pot_kind = kind;
+    c->P.pot_b = b;
+    c->P.pot_c2 = c2;
+    double cap0 = cap ? cap[0] : 0.0, cap1 = cap ? cap[1] : 0.0;
+    c->P.cap0 = cap0;
+    for (int j = 0; j < c->dim; ++j) c->P.capL[j] = .5 * c->nr[j] * c->dr[j] - cap1;
+    if (cap0 == 0.0)
+        for (int j = 0; j < c->dim; ++j) c->P.capL[j] = 1e300;  // no CAP: skip the exp
+    c->have_pot = true;
+    return 0;
+}
+
+int qnd_set_integrator(qnd_ctx *c, const char *name, double dt)
+{
+    if (!c || !name) return QND_EINVAL;
+    static const double bm4[6] = {.0792036964311957, .1303114101821663, .2228614958676077,
+                                  -.3667132690474257, .3246481886897062, .1096884778767498};
+    static const double bm6[10] = {.050262764400392, .098553683500650, .314960616927694, -.447346482695478,
+                                   .492426372489876, -.425118767797691, .237063913978122, .195602488600053,
+                                   .346358189850727, -.362762779254345};
+    static const double rkn4[6] = {.082984406417405, .162314550766866, .233995250731502,
+                                   .370877414979578, -.409933719901926, .059762097006575};
+    double as[20] = {0};
+    int s;
+    if (!strcmp(name, "Verlet")) { s = 1; as[0] = .5; }
+    else if (!strcmp(name, "BM4")) { s = 6; memcpy(as, bm4, sizeof bm4); }
+    else if (!strcmp(name, "BM6")) { s = 10; memcpy(as, bm6, sizeof bm6); }
+    else if (!strcmp(name, "RKN4")) { s = 6; memcpy(as, rkn4, sizeof rkn4); }
+    else return fail(c, QND_EUNKNOWN_NAME, "Unknown integrator: %s", name);
+    for (int j = 0; j < s; ++j) as[s + j] = as[s - 1 - j];  // mirrored second half
+    c->s = s;
+    c->dt = dt;
+    for (int j = 0; j < 20; ++j) {
+        c->as[j] = as[j];
+        c->as_scaled[j] = as[j] * dt;
+    }
+    c->have_int = true;
+    return 0;
+}
+
+int qnd_set_mask(qnd_ctx *c, double frac)
+{
+    if (!c) return QND_EINVAL;
+    c->mask_frac = frac;
+    c->P.mask_on = frac > 0 ? 1 : 0;
+    for (int j = 0; j < c->dim; ++j) {
+        double L = .5 * c->nr[j] * c->dr[j];
+        c->P.mask_x0[j] = frac > 0 ? (1. - frac) * L : 1e300;
+        c->P.mask_w[j] = frac > 0 ? L - c->P.mask_x0[j] : 1.0;
+    }
+    return 0;
+}
+
+int qnd_upload_state(qnd_ctx *c, int slot, const double *psi)
+{
+    if (!c || !psi) return QND_EINVAL;
+    int rc = check_slot(c, slot);
+    if (rc) return rc;
+    return DISPATCH(c, upload, c, slot, psi);
+}
+
+int qnd_download_state(qnd_ctx *c, int slot, double *psi)
+{
+    if (!c || !psi) return QND_EINVAL;
+    int rc = check_slot(c, slot);
+    if (rc) return rc;
+    if (c->slots[slot].pend_vend && !c->have_int) return fail(c, QND_ESTATE, "integrator not set");
+    return DISPATCH(c, download, c, slot, psi);
+}
+
+int qnd_transform(qnd_ctx *c, int slot, int sign)
+{
+    if (!c) return QND_EINVAL;
+    int rc = check_slot(c, slot);
+    if (rc) return rc;
+    if (sign != -1 && sign != 1) return fail(c, QND_EINVAL, "sign must be -1 or +1");
+    if (c->slots[slot].pend_vend && !c->have_int) return fail(c, QND_ESTATE, "integrator not set");
+    rc = c->prec == QND_C64 ? ensure_rep<float>(c, slot, REP_R) : ensure_rep<double>(c, slot, REP_R);
+    if (rc) return rc;
+    rc = c->prec == QND_C64 ? full_fft<float>(c, state_ptr<float>(c, slot), sign)
+                            : full_fft<double>(c, state_ptr<double>(c, slot), sign);
+    if (rc) return rc;
+    c->slots[slot] = SlotState();  // the content is now whatever the caller asked for
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int qnd_imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int gram_schmidt)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = check_slot(c, slot))) return rc;
+    if (nsteps < 0) return fail(c, QND_EINVAL, "nsteps < 0");
+    if (gram_schmidt && slot > 64) return fail(c, QND_EUNSUPPORTED, "at most 65 states for Gram-Schmidt");
+    return DISPATCH(c, imag_steps, c, slot, nsteps, gram_schmidt);
+}
+
+int qnd_gram_schmidt(qnd_ctx *c, int slot)
+{
+    if (!c) return QND_EINVAL;
+    int rc = check_slot(c, slot);
+    if (rc) return rc;
+    return DISPATCH(c, gram_schmidt, c, slot);
+}
+
+int qnd_energy_moments(qnd_ctx *c, int slot, int M, double *E)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = check_slot(c, slot))) return rc;
+    if (M <= 0) return 0;
+    rc = DISPATCH(c, energies, c, slot, M, E);
+    prof_flush(c);
+    return rc;
+}
+
+int qnd_observables(qnd_ctx *c, int slot, const int *codes, int ncodes, const double *alpha,
+                    const double *Efield, double *out)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = check_slot(c, slot))) return rc;
+    rc = DISPATCH(c, observables, c, slot, codes, ncodes, alpha, Efield, out);
+    prof_flush(c);
+    return rc;
+}
+
+int qnd_set_field(qnd_ctx *c, int64_t ntime, const double *Et, const double *At, const double *Eobs,
+                  const double *Aobs)
+{
+    if (!c || ntime < 0) return QND_EINVAL;
+    if (!c->have_int) return fail(c, QND_ESTATE, "integrator not set (qnd_set_integrator)");
+    const size_t ns = (size_t)ntime * c->s * c->dim, no = (size_t)ntime * c->dim;
+    c->Et.assign(ns, 0.0);
+    c->At.assign(ns, 0.0);
+    c->Eobs.assign(no, 0.0);
+    c->Aobs.assign(no, 0.0);
+    if (Et) memcpy(c->Et.data(), Et, ns * sizeof(double));
+    if (At) memcpy(c->At.data(), At, ns * sizeof(double));
+    if (Eobs) memcpy(c->Eobs.data(), Eobs, no * sizeof(double));
+    if (Aobs) memcpy(c->Aobs.data(), Aobs, no * sizeof(double));
+    c->ntime = ntime;
+    return 0;
+}
+
+int qnd_real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, const int *codes,
+                   int ncodes, double *obs_out)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = check_slot(c, slot))) return rc;
+    if (gauge != QND_GAUGE_LENGTH && gauge != QND_GAUGE_VELOCITY)
+        return fail(c, QND_EINVAL, "unknown gauge %d", gauge);
+    if (ncodes > 0 && (!codes || !obs_out)) return fail(c, QND_EINVAL, "observable codes need an output array");
+    return DISPATCH(c, real_steps, c, slot, gauge, step0, nsteps, codes, ncodes, obs_out);
+}
+
+int qnd_final_observables(qnd_ctx *c, int slot, const double *R_mask, const double *pr_max, int Ne, double de,
+                          double *ped, double *psi_p)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = check_slot(c, slot))) return rc;
+    if (!R_mask || !pr_max) return fail(c, QND_EINVAL, "R_mask and pr_max are required");
+    return DISPATCH(c, final_observables, c, slot, R_mask, pr_max, Ne, de, ped, psi_p);
+}
+
+int qnd_event_record(qnd_ctx *c, int idx)
+{
+    if (!c || idx < 0 || idx >= 16) return QND_EINVAL;
+    HIPCK(c, hipEventRecord(c->ev[idx], c->stream));
+    return 0;
+}
+
+int qnd_event_elapsed(qnd_ctx *c, int i0, int i1, float *ms)
+{
+    if (!c || i0 < 0 || i0 >= 16 || i1 < 0 || i1 >= 16 || !ms) return QND_EINVAL;
+    HIPCK(c, hipEventSynchronize(c->ev[i1]));
+    HIPCK(c, hipEventElapsedTime(ms, c->ev[i0], c->ev[i1]));
+    return 0;
+}
+
+int qnd_synchronize(qnd_ctx *c)
+{
+    if (!c) return QND_EINVAL;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int qnd_profile_enable(qnd_ctx *c, int enable)
+{
+    if (!c) return QND_EINVAL;
+    prof_flush(c);
+    c->prof = enable != 0;
+    for (int k = 0; k < PC_N; ++k) {
+        c->prof_count[k] = 0;
+        c->prof_ms[k] = 0;
+    }
+    return 0;
+}
+
+int qnd_profile_read(qnd_ctx *c, int64_t *count, double *ms, int nclass)
+{
+    if (!c) return QND_EINVAL;
+    prof_flush(c);
+    for (int k = 0; k < nclass && k < PC_N; ++k) {
+        count[k] = c->prof_count[k];
+        ms[k] = c->prof_ms[k];
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,248 @@
+"""GPU parity: libqnd (HIP, gfx950) against the FP64 CPU oracle on identical
+seeded inputs, through the C ABI.
+
+Tolerances: complex128 path rel-L2 1e-12 on states and 1e-11 absolute on
+energies/observables (reference parity bar); complex64 path 2e-5 relative
+(FP32 storage and phases, FP64 reductions).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import tdse_nd_mpi_amd as q
+
+pytestmark = pytest.mark.gpu
+
+TOL128 = 1e-12
+TOL64 = 2e-5
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def rand_state(shape, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    return x
+
+
+def smooth_state(o, seed, n_states=1):
+    return o.initial_guess(n_states, guess_mode=1, seed=seed)
+
+
+@pytest.mark.parametrize("prec,tol", [("c128", 1e-14), ("c64", 1e-6)])
+@pytest.mark.parametrize("shape", [(8,), (16,), (32,), (64,), (128,), (256,), (512,), (1024,), (2048,),
+                                   (4096,), (16, 32), (512, 512), (8, 16, 32), (32, 8, 64)])
+def test_transform_matches_numpy(gpu, shape, prec, tol):
+    if prec == "c128" and shape[-1] > 4096:
+        pytest.skip("line too long for c128")
+    print(f"case transform shape={shape} prec={prec}", flush=True)
+    x = rand_state(shape, 1)
+    g = q.Grid(shape, 0.1, prec, n_slots=1)
+    for sign in (-1, +1):
+        g.upload(0, x)
+        g.transform(0, sign)
+        got = g.download(0)
+        ref = np.fft.fftn(x) if sign < 0 else np.fft.ifftn(x) * x.size
+        assert rel(got, ref) < tol, (sign, rel(got, ref))
+    g.close()
+
+
+IMAG_CASES = [
+    ((256,), 0.2, "H_1D", "BM4"),
+    ((128,), 0.3, "H_1D", "Verlet"),
+    ((128,), 0.3, "H_1D", "RKN4"),
+    ((64, 32), 0.4, "H_2D", "BM4"),
+    ((32, 64), 0.4, "Ne_2D", "BM6"),
+    ((16, 16, 16), 0.5, "H_3D", "BM4"),
+    ((8, 16, 8, 8), 0.8, "H_3D", "Verlet"),
+]
+
+
+@pytest.mark.parametrize("nr,dr,pot,integ", IMAG_CASES)
+def test_imag_steps_gram_schmidt(gpu, nr, dr, pot, integ):
+    print(f"case imag nr={nr} pot={pot} integ={integ}", flush=True)
+    dt, nst, nsteps = 0.05, 3, 12
+    o = O.Oracle(nr, dr, pot, integ, dt)
+    g = q.Grid(nr, dr, "c128", n_slots=nst)
+    g.set_potential(pot)
+    g.set_integrator(integ, dt)
+    states = smooth_state(o, 11, nst)
+    ref = states.copy()
+    for j in range(nst):
+        g.upload(j, states[j])
+        for _ in range(nsteps):
+            ref[j] = o.step_imag(ref[j])
+            rc, ref = o.gram_schmidt(ref, j)
+            assert rc == 0
+        g.imag_steps(j, nsteps, gram_schmidt=True)
+    for j in range(nst):
+        got = g.download(j)
+        assert rel(got, ref[j]) < TOL128, (j, rel(got, ref[j]))
+    E = g.energy_moments(nst - 1, 2)
+    Eo = o.energies(ref[nst - 1], 2)
+    assert np.allclose(E, Eo, rtol=0, atol=1e-11), (E, Eo)
+    g.close()
+
+
+@pytest.mark.parametrize("nr,dr,pot", [((128,), 0.3, "H_1D"), ((32, 64), 0.4, "Ne_2D"), ((16, 8, 16), 0.5, "H_3D")])
+def test_observables_match_oracle(gpu, nr, dr, pot):
+    print(f"case observables nr={nr} pot={pot}", flush=True)
+    o = O.Oracle(nr, dr, pot, "BM4", 0.05)
+    g = q.Grid(nr, dr, "c128", n_slots=1)
+    g.set_potential(pot)
+    g.set_integrator("BM4", 0.05)
+    psi = smooth_state(o, 5)[0] * np.exp(0.3j * np.arange(nr[-1]) * dr)
+    g.upload(0, psi)
+    names = ["norm", "dipole_r", "dipole_a", "dipole_v", "energy", "Lz"]
+    dim = len(nr)
+    alpha = np.linspace(0.1, 0.2, dim)
+    Ef = np.linspace(-0.05, 0.03, dim)
+    got = g.observables(0, names, alpha=alpha, Efield=Ef)
+    ref = o.observables(psi, names, alpha=alpha, Efield=Ef)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12), (got, ref)
+    E = g.energy_moments(0, 3)
+    assert np.allclose(E, o.energies(psi, 3), rtol=1e-12, atol=1e-12)
+    g.close()
+
+
+def field_tables(o, nt, dim, I=1e14, lam=800.0, cycles=2, ellip=0.3):
+    w = 45.5637639947958 / lam
+    tau = cycles * 2 * np.pi / w
+    modes = dict(freq=[w], amp=[5.33803e-9 * np.sqrt(I)], ellip=[ellip], duration=[tau],
+                 cep=[0.4], delay=[0.0], env=[O.ENV["cos2"]])
+    a, asc = o.coefficients()
+    time = np.arange(nt) * o.dt
+    A, E = O.field_tables(dim, modes, asc, o.s, time)
+    Aobs = np.zeros((nt, dim))
+    Eobs = np.zeros((nt, dim))
+    for i, t in enumerate(time):
+        Aobs[i], Eobs[i] = O.field_vectors(dim, modes, t)
+    return E, A, Eobs, Aobs
+
+
+REAL_CASES = [
+    ((256,), 0.2, "H_1D", "BM4", "length", 0.0, (0.0, 0.0)),
+    ((256,), 0.2, "H_1D", "BM4", "velocity", 0.0, (0.0, 0.0)),
+    ((64, 32), 0.4, "Ne_2D", "BM4", "length", 0.1, (0.0, 0.0)),
+    ((64, 32), 0.4, "Ne_2D", "BM4", "velocity", 0.1, (0.0, 0.0)),
+    ((32, 64), 0.4, "H_2D", "RKN4", "length", 0.0, (0.1, 3.0)),
+    ((32, 64), 0.4, "H_2D", "BM6", "velocity", 0.0, (0.1, 3.0)),
+    ((16, 16, 16), 0.5, "H_3D", "BM4", "length", 0.1, (0.05, 1.0)),
+    ((16, 16, 16), 0.5, "H_3D", "Verlet", "velocity", 0.1, (0.05, 1.0)),
+]
+
+
+@pytest.mark.parametrize("nr,dr,pot,integ,gauge,mask,cap", REAL_CASES)
+def test_real_steps_match_oracle(gpu, nr, dr, pot, integ, gauge, mask, cap):
+    print(f"case real nr={nr} pot={pot} integ={integ} gauge={gauge} mask={mask} cap={cap}", flush=True)
+    dt, nt = 0.05, 24
+    dim = len(nr)
+    o = O.Oracle(nr, dr, pot, integ, dt, cap=cap)
+    if mask:
+        o.set_mask(mask)
+    g = q.Grid(nr, dr, "c128", n_slots=1)
+    g.set_potential(pot, cap=cap)
+    g.set_integrator(integ, dt)
+    g.set_mask(mask)
+    E, A, Eobs, Aobs = field_tables(o, nt, dim)
+    g.set_field(E, A, Eobs, Aobs)
+    psi0 = smooth_state(o, 3)[0]
+    names = ["dipole_r", "norm", "dipole_a"]
+    ref_psi, ref_obs = o.real_run(psi0, gauge, E, A, Eobs, Aobs, names)
+    g.upload(0, psi0)
+    obs1 = g.real_steps(0, gauge, 0, nt // 2, names)
+    obs2 = g.real_steps(0, gauge, nt // 2, nt - nt // 2, names)
+    got_obs = np.vstack([obs1, obs2])
+    got = g.download(0)
+    assert rel(got, ref_psi) < TOL128, rel(got, ref_psi)
+    assert np.allclose(got_obs, ref_obs, rtol=1e-11, atol=1e-12), np.abs(got_obs - ref_obs).max()
+    g.close()
+
+
+@pytest.mark.parametrize("gauge", ["length", "velocity"])
+def test_real_unfused_observables(gpu, gauge):
+    nr, dr, pot = (32, 32), 0.4, "H_2D"
+    print(f"case real-unfused gauge={gauge}", flush=True)
+    dt, nt = 0.05, 6
+    o = O.Oracle(nr, dr, pot, "BM4", dt)
+    g = q.Grid(nr, dr, "c128", n_slots=1)
+    g.set_potential(pot)
+    g.set_integrator("BM4", dt)
+    E, A, Eobs, Aobs = field_tables(o, nt, 2)
+    g.set_field(E, A, Eobs, Aobs)
+    psi0 = smooth_state(o, 9)[0]
+    names = ["Lz", "energy", "dipole_r", "dipole_v", "PMD", "norm"]
+    names = [n for n in names if n in O.OBS]
+    ref_psi, ref_obs = o.real_run(psi0, gauge, E, A, Eobs, Aobs, names)
+    g.upload(0, psi0)
+    got_obs = g.real_steps(0, gauge, 0, nt, names)
+    assert np.allclose(got_obs, ref_obs, rtol=1e-11, atol=1e-12), np.abs(got_obs - ref_obs).max()
+    assert rel(g.download(0), ref_psi) < TOL128
+    g.close()
+
+
+def test_final_observables(gpu):
+    nr, dr = (64, 64), 0.5
+    print("case final-observables", flush=True)
+    o = O.Oracle(nr, dr, "H_2D", "BM4", 0.05)
+    g = q.Grid(nr, dr, "c128", n_slots=1)
+    g.set_potential("H_2D")
+    g.set_integrator("BM4", 0.05)
+    psi = smooth_state(o, 2)[0] * np.exp(1j * 0.8 * (np.arange(64) * dr))[None, :]
+    g.upload(0, psi)
+    Rm, pm, Ne, de = (2.0, 6.0), (2.0, 1.5), 64, 0.02
+    ped, psi_p = g.final_observables(0, Rm, pm, Ne, de)
+    ref_p, ref_ped, ref_pmd, npr = o.final_observables(psi, Rm, pm, Ne, de)
+    assert rel(psi_p, ref_p) < TOL128
+    assert np.allclose(ped, ref_ped, rtol=1e-11, atol=1e-13)
+    g.close()
+
+
+@pytest.mark.parametrize("nr,dr,pot,gauge", [((256,), 0.2, "H_1D", "length"), ((64, 64), 0.4, "Ne_2D", "length"),
+                                             ((64, 64), 0.4, "Ne_2D", "velocity"),
+                                             ((16, 32, 16), 0.5, "H_3D", "length")])
+def test_complex64_path(gpu, nr, dr, pot, gauge):
+    print(f"case c64 nr={nr} gauge={gauge}", flush=True)
+    dt, nt = 0.05, 40
+    dim = len(nr)
+    o = O.Oracle(nr, dr, pot, "BM4", dt)
+    g = q.Grid(nr, dr, "c64", n_slots=2)
+    g.set_potential(pot)
+    g.set_integrator("BM4", dt)
+    psi0 = smooth_state(o, 4, 2)
+    # imaginary time with Gram-Schmidt
+    ref = psi0.copy()
+    for j in range(2):
+        g.upload(j, psi0[j])
+        for _ in range(10):
+            ref[j] = o.step_imag(ref[j])
+            rc, ref = o.gram_schmidt(ref, j)
+        g.imag_steps(j, 10)
+        assert rel(g.download(j), ref[j]) < TOL64
+    # real time
+    E, A, Eobs, Aobs = field_tables(o, nt, dim)
+    g.set_field(E, A, Eobs, Aobs)
+    names = ["dipole_r", "norm", "dipole_a"]
+    ref_psi, ref_obs = o.real_run(ref[0], gauge, E, A, Eobs, Aobs, names)
+    got_obs = g.real_steps(0, gauge, 0, nt, names)
+    assert rel(g.download(0), ref_psi) < 1e-4
+    scale = np.abs(ref_obs).max(axis=0) + 1e-3
+    assert (np.abs(got_obs - ref_obs) / scale).max() < 1e-4
+    g.close()
+
+
+def test_gram_schmidt_collapse_error(gpu):
+    nr, dr = (64,), 0.3
+    g = q.Grid(nr, dr, "c128", n_slots=2)
+    g.set_potential("H_1D")
+    g.set_integrator("BM4", 0.05)
+    o = O.Oracle(nr, dr, "H_1D", "BM4", 0.05)
+    s = smooth_state(o, 1)[0]
+    s = s / np.sqrt(np.sum(np.abs(s) ** 2) * dr)
+    g.upload(0, s)
+    g.upload(1, s)   # identical state: the projection leaves nothing
+    with pytest.raises(q.QNDError, match="norm decayed"):
+        g.gram_schmidt(1)
+    g.close()
