@@ -68,6 +68,9 @@ struct qnd_ctx {
     double *d_rec = nullptr;
     size_t rec_cap = 0;
     double *d_prmax = nullptr;
+    double *d_capt = nullptr;    // per-axis CAP table (sum over axes = Vcap)
+    double *d_maskt = nullptr;   // per-axis mask table (product over axes)
+    int tw_override[2] = {0, 0}; // lines per workgroup (contiguous, strided); QND_TW env
     std::vector<SlotState> slots;
     AxisPlan plan[QND_MAXD];
     std::vector<double> Et, At, Eobs, Aobs;
@@ -167,7 +170,7 @@ void prof_flush(qnd_ctx *c)
 
 // ---- pass geometry ----
 struct Geo {
-    int axis, n, log2n, T, TW, threads, grid;
+    int axis, n, log2n, T, log2T, TW, log2TW, threads, grid, ldsStride;
     int64_t S, nlines;
     size_t lds;
 };
@@ -197,6 +200,7 @@ Geo geometry(qnd_ctx *c, int axis)
         tw = pow2floor(g.nlines / 1024 > 0 ? g.nlines / 1024 : 1);
         int cap = 256 / g.T > 0 ? 256 / g.T : 1;
         if (tw > cap) tw = cap;
+        if (tw * g.T < 64) tw = 64 / g.T;  // at least one full wavefront
     } else {
         const int twmin = c->esz == 8 ? 4 : 2, twmaxseg = c->esz == 8 ? 16 : 8;
         tw = pow2floor(g.nlines / 256 > 0 ? g.nlines / 256 : 1);
@@ -204,15 +208,23 @@ Geo geometry(qnd_ctx *c, int axis)
         if (tw > twmaxseg) tw = twmaxseg;
         while (tw > 1 && (g.S % tw) != 0) tw /= 2;
     }
-    int thrcap = 1024 / g.T;
+    const int ov = c->tw_override[g.S == 1 ? 0 : 1];
+    if (ov > 0) tw = pow2floor(ov);
+    int thrcap = QND_PASS_THREADS / g.T;
     if (tw > thrcap) tw = thrcap;
     if (tw > twmax_lds) tw = pow2floor(twmax_lds);
     if ((int64_t)tw > g.nlines) tw = (int)g.nlines;
+    if (g.S > 1 && (int64_t)tw > g.S) tw = (int)g.S;
     if (tw < 1) tw = 1;
+    // one padding element per line spreads the strided tile rows over the banks
+    while (tw > 1 && (size_t)tw * (g.n + 1) * c->esz > c->lds_max) tw /= 2;
     g.TW = tw;
+    g.log2TW = ilog2(tw);
+    g.log2T = ilog2(g.T);
     g.threads = g.T * tw;
-    g.grid = (int)((g.nlines + tw - 1) / tw);
-    g.lds = (size_t)tw * g.n * c->esz;
+    g.grid = (int)(g.nlines / tw);
+    g.ldsStride = g.n + (tw > 1 ? 1 : 0);
+    g.lds = (size_t)tw * g.ldsStride * c->esz;
     return g;
 }
 
@@ -221,13 +233,15 @@ template <typename R> PassArgs<R> base_args(qnd_ctx *c, C<R> *data, const Geo &g
     PassArgs<R> a;
     memset(&a, 0, sizeof a);
     a.data = data;
-    a.tw = reinterpret_cast<const C<R> *>(c->plan[g.axis].tw);
+    a.tw = reinterpret_cast<const typename TwT<R>::T *>(c->plan[g.axis].tw);
     a.axis = g.axis;
     a.n = g.n;
     a.log2n = g.log2n;
     a.T = g.T;
+    a.log2T = g.log2T;
     a.TW = g.TW;
-    a.ldsStride = g.n;
+    a.log2TW = g.log2TW;
+    a.ldsStride = g.ldsStride;
     a.S = g.S;
     a.nlines = g.nlines;
     a.scale1 = 1.0;
@@ -237,12 +251,20 @@ template <typename R> PassArgs<R> base_args(qnd_ctx *c, C<R> *data, const Geo &g
     return a;
 }
 
-template <typename R> int launch(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, int cls)
+template <typename R, int KIND, int DIR>
+int launch(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, int cls)
 {
     ProfScope ps(c, cls);
-    hipLaunchKernelGGL(k_pass<R>, dim3(g.grid), dim3(g.threads), g.lds, c->stream, a);
+    hipLaunchKernelGGL((k_pass<R, KIND, DIR>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, a);
     HIPCK(c, hipGetLastError());
     return 0;
+}
+
+// axis-0 pass with r-space operators (IFFT first)
+template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R> &a)
+{
+    if (c->dim == 1) return launch<R, PK_POT1D, +1>(c, g, a, PC_POT);
+    return launch<R, PK_POT, +1>(c, g, a, PC_POT);
 }
 
 // plain transform of `data` along axis (dir -1 forward, +1 inverse), times scale
@@ -250,9 +272,9 @@ template <typename R> int plain_pass(qnd_ctx *c, C<R> *data, int axis, int dir, 
 {
     Geo g = geometry(c, axis);
     PassArgs<R> a = base_args<R>(c, data, g);
-    a.dir_first = dir;
     a.scale1 = scale;
-    return launch<R>(c, g, a, PC_PLAIN);
+    if (dir < 0) return launch<R, PK_PLAIN, -1>(c, g, a, PC_PLAIN);
+    return launch<R, PK_PLAIN, +1>(c, g, a, PC_PLAIN);
 }
 
 // full N-D transform (unnormalised)
@@ -267,23 +289,46 @@ template <typename R> int full_fft(qnd_ctx *c, C<R> *data, int dir)
 
 // kinetic block for dim >= 2: FFT axes 1..d-2, kinetic pass on axis d-1,
 // inverse on axes d-2..1.
-template <typename R> int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A)
+// A pending observable record (partials of the previous axis-0 pass) is
+// finalized by block 0 of the kinetic pass when fin is given.
+struct Finalize {
+    const double *part;
+    int nwg;
+    double *rec;
+    ObsSpec spec;
+};
+
+template <typename R>
+int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A, const Finalize *fin = nullptr)
 {
     const int d = c->dim;
+    int rc;
+    if (fin && d > 2) {  // a plain pass comes first: finalize separately
+        ProfScope ps(c, PC_SMALL);
+        hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(64), 0, c->stream, fin->part, fin->nwg, d, c->P.dNr,
+                           fin->spec, fin->rec);
+        HIPCK(c, hipGetLastError());
+        fin = nullptr;
+    }
     for (int ax = 1; ax <= d - 2; ++ax) {
-        int rc = plain_pass<R>(c, data, ax, -1, 1.0);
+        rc = plain_pass<R>(c, data, ax, -1, 1.0);
         if (rc) return rc;
     }
     Geo g = geometry(c, d - 1);
     PassArgs<R> a = base_args<R>(c, data, g);
-    a.dir_first = -1;
-    a.flags = PF_KMID | PF_SECOND;
     a.kmode = kmode;
     a.ck_mid = ck;
     a.scale2 = 1.0 / (double)c->nr[d - 1];
     if (A)
         for (int j = 0; j < d; ++j) a.Ak[j] = A[j];
-    int rc = launch<R>(c, g, a, PC_KIN);
+    if (fin) {
+        a.flags |= PF_FINALIZE;
+        a.fin_part = fin->part;
+        a.fin_nwg = fin->nwg;
+        a.fin_rec = fin->rec;
+        a.fin_spec = fin->spec;
+    }
+    rc = launch<R, PK_KIN, -1>(c, g, a, PC_KIN);
     if (rc) return rc;
     for (int ax = d - 2; ax >= 1; --ax) {
         rc = plain_pass<R>(c, data, ax, +1, 1.0 / (double)c->nr[ax]);
@@ -302,7 +347,6 @@ template <typename R> int ensure_rep(qnd_ctx *c, int slot, int want)
         // M -> R with the owed operators
         Geo g = geometry(c, 0);
         PassArgs<R> a = base_args<R>(c, psi, g);
-        a.dir_first = +1;
         a.scale1 = 1.0 / (double)c->nr[0];
         if (st.pend_vend) {
             a.flags |= PF_V1;
@@ -310,7 +354,7 @@ template <typename R> int ensure_rep(qnd_ctx *c, int slot, int want)
             a.cv1 = c->as_scaled[2 * c->s - 1];
         }
         if (st.pend_mask) a.flags |= PF_MASK;
-        int rc = launch<R>(c, g, a, PC_POT);
+        int rc = launch_pot<R>(c, g, a);
         if (rc) return rc;
         st.rep = REP_R;
         st.pend_mask = st.pend_vend = false;
@@ -336,8 +380,8 @@ int reduce(qnd_ctx *c, int mode, const C<R> *a, C<R> *b, const C<R> *cc, int axi
         hipLaunchKernelGGL(k_red<R>, dim3(grid), dim3(256), 0, c->stream, mode, a, b, cc, axis, fs,
                            c->N, c->P, alpha_dev, c->d_part, npart);
         HIPCK(c, hipGetLastError());
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, c->stream, c->d_part, grid, npart,
-                           nq, scale, c->d_small + SMALL_SUMS);
+        hipLaunchKernelGGL(k_sum_partials, dim3((nq + 3) / 4), dim3(256), 0, c->stream, c->d_part, grid,
+                           npart, nq, scale, c->d_small + SMALL_SUMS);
         HIPCK(c, hipGetLastError());
     }
     if (host_out) {
@@ -513,7 +557,6 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
     for (int64_t it = 0; it < nsteps; ++it) {
         for (int ia = 0; ia < s; ++ia) {
             PassArgs<R> a = base_args<R>(c, psi, g0);
-            a.dir_first = +1;
             a.scale1 = inv_n0;
             a.flags = PF_V2 | PF_SECOND;
             a.vmode = VM_IMAG;
@@ -535,12 +578,11 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
                 a.kmode = KM_IMAG;
                 a.ck_post = ck;
             }
-            if ((rc = launch<R>(c, g0, a, PC_POT))) return rc;
+            if ((rc = launch_pot<R>(c, g0, a))) return rc;
             if (d > 1 && (rc = kinetic_block<R>(c, psi, KM_IMAG, ck, nullptr))) return rc;
         }
         // last potential half-step, then the Gram-Schmidt partial dots
         PassArgs<R> a = base_args<R>(c, psi, g0);
-        a.dir_first = +1;
         a.scale1 = inv_n0;
         a.flags = PF_V1 | PF_SECOND;
         a.vmode = VM_IMAG;
@@ -552,10 +594,10 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
             a.partials = c->d_part;
             a.npart = npart;
         }
-        if ((rc = launch<R>(c, g0, a, PC_POT))) return rc;
+        if ((rc = launch_pot<R>(c, g0, a))) return rc;
         if (gs) {
             ProfScope ps(c, PC_SMALL);
-            hipLaunchKernelGGL(k_gs_coeffs, dim3(1), dim3(128), 0, c->stream, c->d_part, g0.grid, npart,
+            hipLaunchKernelGGL(k_gs_coeffs, dim3(1), dim3(256), 0, c->stream, c->d_part, g0.grid, npart,
                                nprev, c->P.dNr * inv_n0, c->d_small + SMALL_GRAM, coef);
             HIPCK(c, hipGetLastError());
             pending = true;
@@ -657,7 +699,6 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
     memset(&spec, 0, sizeof spec);
     spec.ncodes = ncodes < 8 ? ncodes : 8;
     for (int k = 0; k < spec.ncodes; ++k) spec.codes[k] = codes[k];
-    spec.dNr = c->P.dNr;
     if (ncodes > 8) return fail(c, QND_EINVAL, "at most 8 observables per step");
 
     for (int64_t it = step0; it < step0 + nsteps; ++it) {
@@ -671,35 +712,54 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
                                      obs_out + (it - step0) * width)))
                 return rc;
         }
-        if ((rc = ensure_rep<R>(c, slot, REP_M)) && st.rep != REP_M) return rc;
+        // a slot in position space owes nothing; the boundary pass pays the
+        // owed mask / half-step of a slot in the axis-0 Fourier representation
+        if (st.rep == REP_R) {
+            if ((rc = plain_pass<R>(c, psi, 0, -1, 1.0))) return rc;
+            st.rep = REP_M;
+        }
+        Finalize fin;
+        const Finalize *finp = nullptr;
+        if (want_obs && fused) {
+            for (int j = 0; j < d; ++j) spec.E[j] = Eo[j];
+            fin.part = c->d_part;
+            fin.nwg = g0.grid;
+            fin.rec = c->d_rec + (it - step0) * width;
+            fin.spec = spec;
+            finp = &fin;
+        }
+        auto finalize_now = [&]() -> int {
+            if (!finp) return 0;
+            ProfScope ps(c, PC_SMALL);
+            hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(64), 0, c->stream, fin.part, fin.nwg, d, c->P.dNr,
+                               fin.spec, fin.rec);
+            HIPCK(c, hipGetLastError());
+            finp = nullptr;
+            return 0;
+        };
         if (gauge == QND_GAUGE_LENGTH) {
             // boundary pass: owed mask, observables
             PassArgs<R> a = base_args<R>(c, psi, g0);
-            a.dir_first = +1;
             a.scale1 = inv_n0;
             a.flags = PF_SECOND;
             if (st.pend_mask) a.flags |= PF_MASK;
-            if (want_obs && fused) {
+            if (finp) {
                 a.flags |= PF_OBS;
                 a.partials = c->d_part;
                 a.npart = QND_NOBS;
             }
             if (a.flags != PF_SECOND) {
-                if ((rc = launch<R>(c, g0, a, PC_POT))) return rc;
+                if ((rc = launch_pot<R>(c, g0, a))) return rc;
             }
             st.pend_mask = false;
-            if (want_obs && fused) {
-                for (int j = 0; j < d; ++j) spec.E[j] = Eo[j];
-                ProfScope ps(c, PC_SMALL);
-                hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(64), 0, c->stream, c->d_part, g0.grid,
-                                   QND_NOBS, d, spec, c->d_rec + (it - step0) * width);
-                HIPCK(c, hipGetLastError());
-            }
+            if (d == 1 && (rc = finalize_now())) return rc;
             for (int ia = 0; ia < s; ++ia) {
                 const double a0 = c->as_scaled[2 * ia], a1 = c->as_scaled[2 * ia + 1];
-                if (d > 1 && ia == 0 && (rc = kinetic_block<R>(c, psi, KM_PHASE, a0, nullptr))) return rc;
+                if (d > 1 && ia == 0) {
+                    if ((rc = kinetic_block<R>(c, psi, KM_PHASE, a0, nullptr, finp))) return rc;
+                    finp = nullptr;
+                }
                 PassArgs<R> b = base_args<R>(c, psi, g0);
-                b.dir_first = +1;
                 b.scale1 = inv_n0;
                 b.flags = PF_V1 | PF_SECOND;
                 b.vmode = VM_LENGTH;
@@ -711,7 +771,7 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
                     b.ck_pre = a0;
                     b.ck_post = a1;
                 }
-                if ((rc = launch<R>(c, g0, b, PC_POT))) return rc;
+                if ((rc = launch_pot<R>(c, g0, b))) return rc;
                 if (d > 1) {
                     double ck = a1 + (ia < s - 1 ? c->as_scaled[2 * ia + 2] : 0.0);
                     if ((rc = kinetic_block<R>(c, psi, KM_PHASE, ck, nullptr))) return rc;
@@ -723,7 +783,6 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
             // the owed mask, the observables and the first half-step
             for (int ia = 0; ia < s; ++ia) {
                 PassArgs<R> a = base_args<R>(c, psi, g0);
-                a.dir_first = +1;
                 a.scale1 = inv_n0;
                 a.flags = PF_V2 | PF_SECOND;
                 a.vmode = VM_VELOCITY;
@@ -750,18 +809,16 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
                     a.ck_post = ck;
                     a.Ak[0] = At[ia * d];
                 }
-                if ((rc = launch<R>(c, g0, a, PC_POT))) return rc;
+                if ((rc = launch_pot<R>(c, g0, a))) return rc;
                 if (ia == 0) {
                     st.pend_vend = st.pend_mask = false;
-                    if (want_obs && fused) {
-                        for (int j = 0; j < d; ++j) spec.E[j] = Eo[j];
-                        ProfScope ps(c, PC_SMALL);
-                        hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(64), 0, c->stream, c->d_part,
-                                           g0.grid, QND_NOBS, d, spec, c->d_rec + (it - step0) * width);
-                        HIPCK(c, hipGetLastError());
-                    }
+                    if (d == 1 && (rc = finalize_now())) return rc;
                 }
-                if (d > 1 && (rc = kinetic_block<R>(c, psi, KM_VELOCITY, ck, &At[ia * d]))) return rc;
+                if (d > 1) {
+                    if ((rc = kinetic_block<R>(c, psi, KM_VELOCITY, ck, &At[ia * d], ia == 0 ? finp : nullptr)))
+                        return rc;
+                    if (ia == 0) finp = nullptr;
+                }
             }
             st.pend_vend = true;
             st.pend_mask = c->P.mask_on != 0;
@@ -785,33 +842,17 @@ int final_observables(qnd_ctx *c, int slot, const double *R_mask, const double *
     if (rc) return rc;
     C<R> *psi = state_ptr<R>(c, slot);
     C<R> *A = reinterpret_cast<C<R> *>(c->d_scr[0]);
-    // radial sin^2 mask on the host-visible grid, applied on the device
-    // through a host-built per-point table would cost N doubles; instead the
-    // mask is computed here on the host and the state is masked in place on
-    // a copy.
     const int64_t N = c->N;
-    std::vector<C<R>> h(N);
-    HIPCK(c, hipMemcpyAsync(h.data(), psi, N * sizeof(C<R>), hipMemcpyDeviceToHost, c->stream));
-    HIPCK(c, hipStreamSynchronize(c->stream));
-    for (int64_t f = 0; f < N; ++f) {
-        int64_t rem = f;
-        double rr = 0.0;
-        for (int j = c->dim - 1; j >= 0; --j) {
-            int64_t i = rem % c->nr[j];
-            rem /= c->nr[j];
-            double x = -.5 * c->dr[j] * c->nr[j] + i * c->dr[j];
-            rr += x * x;
-        }
-        double rad = std::sqrt(rr);
-        double fm = 0.0;
-        if (rad >= R_mask[0] && rad <= R_mask[1])
-            fm = std::pow(std::sin(.5 * M_PI * (rad - R_mask[0]) / (R_mask[1] - R_mask[0])), 2.);
-        else if (rad > R_mask[1])
-            fm = 1.0;
-        h[f].x = (R)(fm * (double)h[f].x);
-        h[f].y = (R)(fm * (double)h[f].y);
+    // radial sin^2 mask into a scratch copy (the slot itself is unchanged)
+    double *r1 = c->d_small + SMALL_SUMS + 128;
+    HIPCK(c, hipMemcpyAsync(r1, &R_mask[1], sizeof(double), hipMemcpyHostToDevice, c->stream));
+    {
+        ProfScope ps(c, PC_SMALL);
+        hipLaunchKernelGGL(k_elem<R>, dim3(1024), dim3(256), 0, c->stream, (int)EM_RMASK, A, (const C<R> *)psi, 0,
+                           R_mask[0], N, c->P, (const C<R> *)nullptr, (int64_t)0, 0, (const double *)r1);
+        HIPCK(c, hipGetLastError());
     }
-    HIPCK(c, hipMemcpyAsync(A, h.data(), N * sizeof(C<R>), hipMemcpyHostToDevice, c->stream));
+    std::vector<C<R>> h(psi_p ? N : 0);
     if ((rc = full_fft<R>(c, A, -1))) return rc;
     double fs = 1.0;
     for (int j = 0; j < c->dim; ++j) fs /= (double)c->nr[j];
@@ -831,7 +872,7 @@ int final_observables(qnd_ctx *c, int slot, const double *R_mask, const double *
         HIPCK(c, hipGetLastError());
         double *sums = nullptr;
         HIPCK(c, hipMalloc(&sums, Ne * sizeof(double)));
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->d_part, grid, Ne, Ne, 1.0,
+        hipLaunchKernelGGL(k_sum_partials, dim3((Ne + 3) / 4), dim3(256), 0, c->stream, c->d_part, grid, Ne, Ne, 1.0,
                            sums);
         HIPCK(c, hipGetLastError());
         HIPCK(c, hipMemcpyAsync(ped, sums, Ne * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -898,14 +939,26 @@ template <typename R> int make_twiddles(qnd_ctx *c, int axis)
             p.tw = c->plan[j].tw;
             return 0;
         }
-    std::vector<C<R>> t(p.n);
+    // W_n^k = exp(-2 pi i k/n) from long double; complex64 keeps the rounding
+    // remainder of each component as a second float (hi + lo)
+    using TT = typename TwT<R>::T;
+    std::vector<TT> t(p.n);
     for (int k = 0; k < p.n; ++k) {
-        long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)p.n;
-        t[k].x = (R)cosl(ang);
-        t[k].y = (R)sinl(ang);
+        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)p.n;
+        const long double cs = cosl(ang), sn = sinl(ang);
+        if constexpr (sizeof(R) == 8) {
+            t[k].x = (double)cs;
+            t[k].y = (double)sn;
+        } else {
+            const float ch = (float)cs, sh = (float)sn;
+            t[k].x = ch;
+            t[k].y = sh;
+            t[k].z = (float)(cs - (long double)ch);
+            t[k].w = (float)(sn - (long double)sh);
+        }
     }
-    HIPCK(c, hipMalloc(&p.tw, p.n * sizeof(C<R>)));
-    HIPCK(c, hipMemcpy(p.tw, t.data(), p.n * sizeof(C<R>), hipMemcpyHostToDevice));
+    HIPCK(c, hipMalloc(&p.tw, p.n * sizeof(TT)));
+    HIPCK(c, hipMemcpy(p.tw, t.data(), p.n * sizeof(TT), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -989,7 +1042,7 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
         c->lds_max = m > 65536 ? 65536 : m;
     }
     for (int j = 0; j < dim; ++j) {
-        if ((size_t)nr[j] * c->esz > c->lds_max || nr[j] / 8 > 1024) {
+        if ((size_t)nr[j] * c->esz > c->lds_max || nr[j] / 8 > QND_PASS_THREADS) {
             delete c;
             return fail(nullptr, QND_EUNSUPPORTED, "NR[%d] = %lld exceeds the single-pass line limit", j,
                         (long long)nr[j]);
@@ -1000,18 +1053,24 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     memset(&P, 0, sizeof P);
     P.dim = dim;
     P.dNr = 1.0;
-    for (int j = 0; j < QND_MAXD; ++j) {
-        P.n[j] = 1;
-        P.capL[j] = 1e300;
-        P.mask_x0[j] = 1e300;
-        P.mask_w[j] = 1.0;
-    }
+    for (int j = 0; j < QND_MAXD; ++j) P.n[j] = 1;
+    int off = 0;
     for (int j = 0; j < dim; ++j) {
         P.n[j] = (int)nr[j];
+        P.lg[j] = ilog2(nr[j]);
+        P.off[j] = off;
+        off += (int)nr[j];
         P.dr[j] = dr[j];
         P.L[j] = dr[j] * nr[j];
         P.dp[j] = 2. * M_PI / P.L[j];
         P.dNr *= dr[j];
+    }
+    if (const char *e = getenv("QND_TW")) {  // tuning knob: "contiguous,strided" lines per workgroup
+        int a = 0, b = 0;
+        if (sscanf(e, "%d,%d", &a, &b) >= 1) {
+            c->tw_override[0] = a;
+            c->tw_override[1] = b;
+        }
     }
     c->slots.resize(n_slots);
     int rc = 0;
@@ -1031,6 +1090,11 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     if (hipMalloc(&c->d_small, SMALL_TOTAL * sizeof(double)) != hipSuccess)
         return bail(fail(nullptr, QND_EHIP, "hipMalloc failed"));
     hipMemset(c->d_small, 0, SMALL_TOTAL * sizeof(double));
+    if (hipMalloc(&c->d_capt, off * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->d_maskt, off * sizeof(double)) != hipSuccess)
+        return bail(fail(nullptr, QND_EHIP, "hipMalloc failed"));
+    P.capt = c->d_capt;
+    P.maskt = c->d_maskt;
     for (int j = 0; j < dim; ++j) {
         c->plan[j].n = (int)nr[j];
         c->plan[j].log2n = ilog2(nr[j]);
@@ -1065,6 +1129,8 @@ void qnd_destroy(qnd_ctx *c)
     if (c->d_small) hipFree(c->d_small);
     if (c->d_rec) hipFree(c->d_rec);
     if (c->d_prmax) hipFree(c->d_prmax);
+    if (c->d_capt) hipFree(c->d_capt);
+    if (c->d_maskt) hipFree(c->d_maskt);
     for (int k = 0; k < 16; ++k)
         if (c->ev[k]) hipEventDestroy(c->ev[k]);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1082,11 +1148,19 @@ int qnd_set_potential(qnd_ctx *c, const char *name, double softcore, const doubl
     c->P.pot_kind = kind;
     c->P.pot_b = b;
     c->P.pot_c2 = c2;
-    double cap0 = cap ? cap[0] : 0.0, cap1 = cap ? cap[1] : 0.0;
-    c->P.cap0 = cap0;
-    for (int j = 0; j < c->dim; ++j) c->P.capL[j] = .5 * c->nr[j] * c->dr[j] - cap1;
-    if (cap0 == 0.0)
-        for (int j = 0; j < c->dim; ++j) c->P.capL[j] = 1e300;  // no CAP: skip the exp
+    // separable CAP: Vcap = sum_j cap0 (|x_j| - L_j)^2 for |x_j| >= L_j,
+    // L_j = n_j dr_j / 2 - width (src/meshgrid.c:132-136)
+    const double cap0 = cap ? cap[0] : 0.0, cap1 = cap ? cap[1] : 0.0;
+    std::vector<double> t(c->P.off[c->dim - 1] + c->nr[c->dim - 1], 0.0);
+    for (int j = 0; j < c->dim; ++j) {
+        const double Lc = .5 * c->nr[j] * c->dr[j] - cap1;
+        for (int64_t i = 0; i < c->nr[j]; ++i) {
+            const double x = -.5 * (c->dr[j] * c->nr[j]) + i * c->dr[j];
+            if (fabs(x) >= Lc) t[c->P.off[j] + i] = cap0 * (fabs(x) - Lc) * (fabs(x) - Lc);
+        }
+    }
+    HIPCK(c, hipMemcpy(c->d_capt, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->P.cap_on = cap0 != 0.0 ? 1 : 0;
     c->have_pot = true;
     return 0;
 }
@@ -1122,13 +1196,19 @@ int qnd_set_integrator(qnd_ctx *c, const char *name, double dt)
 int qnd_set_mask(qnd_ctx *c, double frac)
 {
     if (!c) return QND_EINVAL;
+    // m_j(x) = cos(pi/2 (|x| - x0)/(L - x0))^(1/8) for |x| > x0 = (1-frac) L,
+    // L = n_j dr_j / 2; the mask is the product over axes
     c->mask_frac = frac;
-    c->P.mask_on = frac > 0 ? 1 : 0;
-    for (int j = 0; j < c->dim; ++j) {
-        double L = .5 * c->nr[j] * c->dr[j];
-        c->P.mask_x0[j] = frac > 0 ? (1. - frac) * L : 1e300;
-        c->P.mask_w[j] = frac > 0 ? L - c->P.mask_x0[j] : 1.0;
+    std::vector<double> t(c->P.off[c->dim - 1] + c->nr[c->dim - 1], 1.0);
+    for (int j = 0; j < c->dim && frac > 0; ++j) {
+        const double L = .5 * c->nr[j] * c->dr[j], x0 = (1. - frac) * L;
+        for (int64_t i = 0; i < c->nr[j]; ++i) {
+            const double ax = fabs(-.5 * (c->dr[j] * c->nr[j]) + i * c->dr[j]);
+            if (ax > x0) t[c->P.off[j] + i] = pow(cos(.5 * M_PI * (ax - x0) / (L - x0)), .125);
+        }
     }
+    HIPCK(c, hipMemcpy(c->d_maskt, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->P.mask_on = frac > 0 ? 1 : 0;
     return 0;
 }
 

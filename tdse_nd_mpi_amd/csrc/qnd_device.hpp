@@ -2,16 +2,19 @@
 //
 // The split-operator step is built from "line passes": a workgroup loads a
 // tile of TW lines of length n along one axis into LDS, runs an in-place
-// Stockham FFT on them (radix 8/4/2, one butterfly set per thread, twiddles
-// from an FP64/FP32 table), applies the fused pointwise operators of the
-// step in registers, optionally transforms back, reduces observables in FP64
-// and writes the tile back.  Only the state crosses HBM; potential, CAP,
-// kinetic, laser-coupling and mask factors are evaluated analytically.
+// Stockham FFT on them (radix 8/4/2, eight values per thread, twiddles from an
+// FP64/FP32 table), applies the fused pointwise operators of the step,
+// optionally transforms back, reduces observables in FP64 and writes the tile
+// back.  Only the state crosses HBM; potential, CAP, kinetic, laser-coupling
+// and mask factors are analytic or per-axis tables of n_j values.
 //
-// Pass families (see qnd_api.hip for the schedules):
-//   axis-0 pass ("potential pass"): IFFT_0 -> r-space ops -> FFT_0
-//   last-axis pass ("kinetic pass"): FFT_{d-1} -> k-space ops -> IFFT_{d-1}
-//   plain pass: one transform along an axis (representation changes)
+// Pass kinds (one small kernel each, so the instruction stream of a pass
+// holds only its own operators):
+//   PK_PLAIN : one transform along an axis, times a scale
+//   PK_KIN   : FFT -> kinetic factor (all axes in k) -> IFFT    (last axis)
+//   PK_POT   : IFFT -> r-space operators -> FFT                  (axis 0)
+//   PK_POT1D : PK_POT with the kinetic factors before/after (dim = 1)
+// All grid extents are powers of two >= 8, so every index is a shift/mask.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -32,10 +35,6 @@ template <typename R> __device__ __forceinline__ C<R> sub(C<R> a, C<R> b) { retu
 template <typename R> __device__ __forceinline__ C<R> mul(C<R> a, C<R> b)
 {
     return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-template <typename R> __device__ __forceinline__ C<R> cmulconj(C<R> a, C<R> b)  // a * conj(b)
-{
-    return mk<R>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 template <typename R> __device__ __forceinline__ C<R> scl(C<R> a, R s) { return mk<R>(a.x * s, a.y * s); }
 // multiply by (DIR * i): DIR=-1 -> -i z, DIR=+1 -> +i z
@@ -64,17 +63,32 @@ template <typename R, int DIR> __device__ __forceinline__ void dft4(C<R> *x)
     x[3] = sub<R>(a1, b1);
 }
 
+// t * (1/sqrt 2).  In FP32 the constant is a hi+lo pair: a single rounded
+// 1/sqrt2 has 2 h^2 = 1 - 3.4e-8, a bias that every radix-8 stage of every
+// step would apply to the norm.  The exact product t*hi and the small t*lo
+// enter one fused rounding (fma(t, hi, t*lo)); rounding t*hi first would
+// discard the correction (it is below half an ulp of the result).
+template <typename R> __device__ __forceinline__ R mul_rsqrt2(R t);
+template <> __device__ __forceinline__ double mul_rsqrt2<double>(double t)
+{
+    return t * 0.70710678118654752440084436210484903928;
+}
+template <> __device__ __forceinline__ float mul_rsqrt2<float>(float t)
+{
+    const float hi = 0.70710677f, lo = 1.2101617e-08f;  // hi + lo = 1/sqrt(2) to ~1e-15
+    return fmaf(t, hi, t * lo);
+}
+
+// radix 8 as two radix-4 halves (even/odd inputs) and the w_8^q combine
 template <typename R, int DIR> __device__ __forceinline__ void dft8(C<R> *x)
 {
-    const R h = (R)0.70710678118654752440084436210484903928;
     C<R> e[4] = {x[0], x[2], x[4], x[6]};
     C<R> o[4] = {x[1], x[3], x[5], x[7]};
     dft4<R, DIR>(e);
     dft4<R, DIR>(o);
-    // twiddles w^q, w = exp(DIR i pi/4)
-    C<R> o1 = mk<R>(h * (o[1].x - (R)DIR * o[1].y), h * (o[1].y + (R)DIR * o[1].x));
+    C<R> o1 = mk<R>(mul_rsqrt2<R>(o[1].x - (R)DIR * o[1].y), mul_rsqrt2<R>(o[1].y + (R)DIR * o[1].x));
     C<R> o2 = mul_di<R, DIR>(o[2]);
-    C<R> o3 = mk<R>(h * (-o[3].x - (R)DIR * o[3].y), h * (-o[3].y + (R)DIR * o[3].x));
+    C<R> o3 = mk<R>(mul_rsqrt2<R>(-o[3].x - (R)DIR * o[3].y), mul_rsqrt2<R>(-o[3].y + (R)DIR * o[3].x));
     x[0] = add<R>(e[0], o[0]);
     x[4] = sub<R>(e[0], o[0]);
     x[1] = add<R>(e[1], o1);
@@ -92,6 +106,51 @@ template <typename R, int RAD, int DIR> __device__ __forceinline__ void dftR(C<R
     else dft8<R, DIR>(x);
 }
 
+// Twiddle table entries W_n^m = exp(-2 pi i m/n): double2 for complex128;
+// for complex64 a float4 (cos_hi, sin_hi, cos_lo, sin_lo) so that the
+// rounded table does not bias the modulus of the transform.
+template <typename R> struct TwT;
+template <> struct TwT<double> { using T = double2; };
+template <> struct TwT<float> { using T = float4; };
+
+template <typename R, int DIR> __device__ __forceinline__ C<R> twmul(C<R> x, typename TwT<R>::T w);
+template <> __device__ __forceinline__ C<double> twmul<double, -1>(C<double> x, double2 w) { return mul<double>(x, w); }
+template <> __device__ __forceinline__ C<double> twmul<double, +1>(C<double> x, double2 w)
+{
+    w.y = -w.y;
+    return mul<double>(x, w);
+}
+template <int DIR> __device__ __forceinline__ C<float> twmul_f(C<float> x, float4 w)
+{
+    // each exact hi product enters an fma; the lo terms are added before the
+    // roundings they correct
+    const float c = w.x, s = DIR < 0 ? w.y : -w.y, cl = w.z, sl = DIR < 0 ? w.w : -w.w;
+    const float re = fmaf(x.x, c, fmaf(-x.y, s, x.x * cl - x.y * sl));
+    const float im = fmaf(x.x, s, fmaf(x.y, c, x.x * sl + x.y * cl));
+    return mk<float>(re, im);
+}
+template <> __device__ __forceinline__ C<float> twmul<float, -1>(C<float> x, float4 w) { return twmul_f<-1>(x, w); }
+template <> __device__ __forceinline__ C<float> twmul<float, +1>(C<float> x, float4 w) { return twmul_f<+1>(x, w); }
+
+// Phase factor e^{arg}(cos ph - i sin ph) applied to a value: complex128
+// multiplies by the FP64 factor; complex64 carries the FP64 factor as an
+// FP32 hi+lo pair (compensated product), which keeps the propagation
+// unitary to FP32 rounding instead of drifting by a per-angle bias.
+template <typename R> struct Fac;
+template <> struct Fac<double> { double2 v; };
+template <> struct Fac<float> { float2 hi, lo; };
+template <typename R> __device__ __forceinline__ C<R> fac_mul(C<R> x, const Fac<R> &f);
+template <> __device__ __forceinline__ C<double> fac_mul<double>(C<double> x, const Fac<double> &f)
+{
+    return mul<double>(x, f.v);
+}
+template <> __device__ __forceinline__ C<float> fac_mul<float>(C<float> x, const Fac<float> &f)
+{
+    const float re = fmaf(x.x, f.hi.x, fmaf(-x.y, f.hi.y, x.x * f.lo.x - x.y * f.lo.y));
+    const float im = fmaf(x.x, f.hi.y, fmaf(x.y, f.hi.x, x.x * f.lo.y + x.y * f.lo.x));
+    return mk<float>(re, im);
+}
+
 // ---------------------------------------------------------------------
 // Stockham radix stage on one line held in LDS.  The thread owns butterflies
 // j = t + b*T (b < 8/RAD); every thread holds exactly 8 values.
@@ -101,58 +160,51 @@ template <typename R, int RAD, int DIR> __device__ __forceinline__ void dftR(C<R
 // ---------------------------------------------------------------------
 template <typename R, int RAD, int DIR>
 __device__ __forceinline__ void stockham_stage(C<R> *line, int n, int T, int t, int Ns,
-                                               const C<R> *__restrict__ tw, bool active)
+                                               const typename TwT<R>::T *__restrict__ tw)
 {
     constexpr int NB = 8 / RAD;
     C<R> x[8];
     const int nR = n / RAD;
-    if (active) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int j = t + b * T;
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + b * T;
 #pragma unroll
-            for (int r = 0; r < RAD; ++r) x[b * RAD + r] = line[j + r * nR];
-        }
+        for (int r = 0; r < RAD; ++r) x[b * RAD + r] = line[j + r * nR];
     }
     __syncthreads();
-    if (active) {
-        const int stride = n / (Ns * RAD);
+    const int stride = n / (Ns * RAD);
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int j = t + b * T;
-            const int k = j & (Ns - 1);
-            C<R> *xb = x + b * RAD;
-            if (Ns > 1) {
+    for (int b = 0; b < NB; ++b) {
+        const int j = t + b * T;
+        const int k = j & (Ns - 1);
+        C<R> *xb = x + b * RAD;
+        if (Ns > 1) {
 #pragma unroll
-                for (int r = 1; r < RAD; ++r) {
-                    C<R> w = tw[r * k * stride];
-                    if (DIR > 0) w.y = -w.y;
-                    xb[r] = mul<R>(xb[r], w);
-                }
-            }
-            dftR<R, RAD, DIR>(xb);
-            const int o = (j - k) * RAD + k;
-#pragma unroll
-            for (int r = 0; r < RAD; ++r) line[o + r * Ns] = xb[r];
+            for (int r = 1; r < RAD; ++r) xb[r] = twmul<R, DIR>(xb[r], tw[r * k * stride]);
         }
+        dftR<R, RAD, DIR>(xb);
+        const int o = (j - k) * RAD + k;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) line[o + r * Ns] = xb[r];
     }
     __syncthreads();
 }
 
 // Full transform of one line (n = 2^m >= 8): radix-8 stages, then a final
-// radix-4 or radix-2 stage for the remainder.
+// radix-4 or radix-2 stage for the remainder.  Every thread of the block
+// calls it (barriers inside).
 template <typename R, int DIR>
 __device__ __forceinline__ void fft_line(C<R> *line, int n, int log2n, int T, int t,
-                                         const C<R> *__restrict__ tw, bool active)
+                                         const typename TwT<R>::T *__restrict__ tw)
 {
     int Ns = 1;
     const int n8 = log2n / 3, rem = log2n - 3 * n8;
     for (int s = 0; s < n8; ++s) {
-        stockham_stage<R, 8, DIR>(line, n, T, t, Ns, tw, active);
+        stockham_stage<R, 8, DIR>(line, n, T, t, Ns, tw);
         Ns *= 8;
     }
-    if (rem == 2) stockham_stage<R, 4, DIR>(line, n, T, t, Ns, tw, active);
-    else if (rem == 1) stockham_stage<R, 2, DIR>(line, n, T, t, Ns, tw, active);
+    if (rem == 2) stockham_stage<R, 4, DIR>(line, n, T, t, Ns, tw);
+    else if (rem == 1) stockham_stage<R, 2, DIR>(line, n, T, t, Ns, tw);
 }
 
 // ---------------------------------------------------------------------
@@ -162,47 +214,37 @@ __device__ __forceinline__ void fft_line(C<R> *line, int n, int log2n, int T, in
 struct Phys {
     int dim;
     int n[QND_MAXD];
+    int lg[QND_MAXD];        // log2 n
+    int off[QND_MAXD];       // offsets of each axis in the per-axis tables
     double dr[QND_MAXD];
-    double L[QND_MAXD];   // n*dr
-    double dp[QND_MAXD];  // 2 pi / L
-    int pot_kind;         // 0 soft-core, 1 gaussian (Ne_short_2D)
+    double L[QND_MAXD];      // n*dr
+    double dp[QND_MAXD];     // 2 pi / L
+    int pot_kind;            // 0 soft-core, 1 gaussian (Ne_short_2D)
     double pot_b, pot_c2;
-    double cap0;
-    double capL[QND_MAXD];  // edge where the CAP starts: n dr/2 - width
-    int mask_on;
-    double mask_x0[QND_MAXD], mask_w[QND_MAXD];
+    int cap_on, mask_on;
+    const double *capt;      // per-axis CAP contributions cap0 (|x|-L_c)^2 or 0
+    const double *maskt;     // per-axis cos^(1/8) mask factors
     double dNr;
 };
 
 __device__ __forceinline__ double coord_x(const Phys &P, int j, int i) { return -.5 * P.L[j] + i * P.dr[j]; }
 __device__ __forceinline__ double coord_p(const Phys &P, int j, int i)
 {
-    return (i < P.n[j] / 2) ? i * P.dp[j] : (i - P.n[j]) * P.dp[j];
-}
-__device__ __forceinline__ double cap_axis(const Phys &P, int j, double x)
-{
-    double ax = fabs(x);
-    return (ax >= P.capL[j]) ? P.cap0 * (ax - P.capL[j]) * (ax - P.capL[j]) : 0.0;
-}
-__device__ __forceinline__ double mask_axis(const Phys &P, int j, double x)
-{
-    double ax = fabs(x);
-    if (ax <= P.mask_x0[j]) return 1.0;
-    return pow(cos(1.5707963267948966 * (ax - P.mask_x0[j]) / P.mask_w[j]), 0.125);
+    return (i < (P.n[j] >> 1)) ? i * P.dp[j] : (i - P.n[j]) * P.dp[j];
 }
 // V(r) and dV/du (u = r^2) for the named potentials (src/meshgrid.c:140-153)
-__device__ __forceinline__ double potential(const Phys &P, double rr, double *dVdu)
+__device__ __forceinline__ double potential(const Phys &P, double rr, double &dVdu)
 {
     double rad = sqrt(rr);
     double u = rad * rad;
     if (P.pot_kind == 0) {
         double den = sqrt(u + P.pot_c2);
         double g = (P.pot_b != 0.0) ? P.pot_b * exp(-u) : 0.0;
-        if (dVdu) *dVdu = g / den + .5 * (1. + g) / (den * (u + P.pot_c2));
+        dVdu = g / den + .5 * (1. + g) / (den * (u + P.pot_c2));
         return -(1. + g) / den;
     }
     double g = 2.7459 * exp(-.5 * rad * rad / 1.96);
-    if (dVdu) *dVdu = g * .5 / 1.96;
+    dVdu = g * .5 / 1.96;
     return -g;
 }
 
@@ -213,24 +255,30 @@ template <typename R> struct Prec;
 template <> struct Prec<double> { static constexpr double clamp = QND_MAX_EXP_ARG; };
 template <> struct Prec<float> { static constexpr double clamp = QND_MAX_EXP_ARG_F32; };
 
-// e^{arg} (cos ph - i sin ph): FP64 for complex128; for complex64 the angle is
-// reduced mod 2 pi in FP64 and sin/cos are evaluated in FP32.
-template <typename R> __device__ __forceinline__ C<R> phase_factor(double arg, double ph);
-template <> __device__ __forceinline__ C<double> phase_factor<double>(double arg, double ph)
+// e^{arg} (cos ph - i sin ph), the angle reduced mod 2 pi in FP64 and the
+// factor formed in FP64; complex64 receives it as an FP32 hi+lo pair.
+template <typename R> __device__ Fac<R> phase_factor(double arg, double ph);
+template <> __device__ __forceinline__ Fac<double> phase_factor<double>(double arg, double ph)
 {
     double s, c;
     sincos(ph, &s, &c);
-    double e = (arg == 0.0) ? 1.0 : exp(arg > QND_MAX_EXP_ARG ? QND_MAX_EXP_ARG : arg);
-    return mk<double>(e * c, -e * s);
+    const double e = (arg == 0.0) ? 1.0 : exp(arg > QND_MAX_EXP_ARG ? QND_MAX_EXP_ARG : arg);
+    Fac<double> f;
+    f.v = mk<double>(e * c, -e * s);
+    return f;
 }
-template <> __device__ __forceinline__ C<float> phase_factor<float>(double arg, double ph)
+template <> __device__ __forceinline__ Fac<float> phase_factor<float>(double arg, double ph)
 {
     const double twopi = 6.283185307179586476925286766559;
-    double red = ph - twopi * rint(ph * (1.0 / twopi));
-    float s, c;
-    sincosf((float)red, &s, &c);
-    float e = (arg == 0.0) ? 1.0f : (float)exp(arg > QND_MAX_EXP_ARG_F32 ? QND_MAX_EXP_ARG_F32 : arg);
-    return mk<float>(e * c, -e * s);
+    const double red = ph - twopi * rint(ph * (1.0 / twopi));
+    double s, c;
+    sincos(red, &s, &c);
+    const double e = (arg == 0.0) ? 1.0 : exp(arg > QND_MAX_EXP_ARG_F32 ? QND_MAX_EXP_ARG_F32 : arg);
+    const double fr = e * c, fi = -e * s;
+    Fac<float> f;
+    f.hi = mk<float>((float)fr, (float)fi);
+    f.lo = mk<float>((float)(fr - (double)f.hi.x), (float)(fi - (double)f.hi.y));
+    return f;
 }
 template <typename R> __device__ __forceinline__ R real_exp(double arg)
 {
@@ -239,12 +287,16 @@ template <typename R> __device__ __forceinline__ R real_exp(double arg)
 }
 
 // ---------------------------------------------------------------------
-// wavefront / workgroup FP64 reductions (deterministic butterfly order)
+// wavefront / workgroup FP64 reductions (fixed butterfly order)
 // ---------------------------------------------------------------------
+// Butterfly sum over the active lanes of a wavefront: all 64 when the block
+// has >= 64 threads, else the block's power-of-two size.
 __device__ __forceinline__ double wave_sum(double v)
 {
+    const int width = blockDim.x < 64 ? (int)blockDim.x : 64;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    for (int o = 32; o > 0; o >>= 1)
+        if (o < width) v += __shfl_xor(v, o, 64);
     return v;
 }
 // Reduce v over the block; thread 0 gets the result.  red = LDS scratch of
@@ -262,34 +314,51 @@ __device__ __forceinline__ double block_sum(double v, double *red)
         for (int i = 0; i < nw; ++i) s += red[i];
     return s;
 }
+// Sum column q of a [nwg][npart] partials array with the threads of one
+// wavefront (lane-strided, then butterfly): deterministic for a given nwg.
+__device__ __forceinline__ double wave_sum_column(const double *part, int nwg, int npart, int q)
+{
+    const int lane = threadIdx.x & 63;
+    const int width = blockDim.x < 64 ? (int)blockDim.x : 64;
+    double s = 0.0;
+    for (int w = lane; w < nwg; w += width) s += part[(int64_t)w * npart + q];
+    return wave_sum(s);
+}
 
 // ---------------------------------------------------------------------
-// line pass
+// pass arguments
 // ---------------------------------------------------------------------
+enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
-    PF_KPRE = 1u << 1,       // (d = 1) kinetic factor before the first transform
+    PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
     PF_V1 = 1u << 2,         // potential factor #1 (after the first transform)
     PF_MASK = 1u << 3,       // cos^(1/8) mask
     PF_OBS = 1u << 4,        // r-space observable partial sums
     PF_V2 = 1u << 5,         // potential factor #2
     PF_SECOND = 1u << 6,     // second (reverse) transform
-    PF_KPOST = 1u << 7,      // (d = 1) kinetic factor after the second transform
+    PF_KPOST = 1u << 7,      // (dim = 1) kinetic factor after the second transform
     PF_GS_DOTS = 1u << 8,    // Gram-Schmidt partial dots after the second transform
-    PF_KMID = 1u << 9,       // kinetic factor after the first transform
+    PF_FINALIZE = 1u << 9,   // block 0 first turns the previous pass's observable partials into a record
 };
 enum KMode { KM_PHASE = 0, KM_VELOCITY = 1, KM_IMAG = 2 };
 enum VMode { VM_LENGTH = 0, VM_VELOCITY = 1, VM_IMAG = 2 };
 
 #define QND_NOBS (1 + 2 * QND_MAXD)
 
+// observable record assembled from r-space partial sums [n0, d_j, g_j]
+struct ObsSpec {
+    int ncodes;
+    int codes[8];
+    double E[QND_MAXD];
+};
+
 template <typename R> struct PassArgs {
     C<R> *data;
-    const C<R> *tw;
-    int axis, n, log2n, T, TW, ldsStride;
+    const typename TwT<R>::T *tw;
+    int axis, n, log2n, T, log2T, TW, log2TW, ldsStride;
     int64_t S, nlines;
     unsigned flags;
-    int dir_first;           // -1: forward first, +1: inverse first
     double scale1, scale2;   // after the first / second transform
     int kmode;
     double ck_pre, ck_mid, ck_post;
@@ -301,30 +370,384 @@ template <typename R> struct PassArgs {
     const C<R> *prev;
     int64_t slot_stride;
     int nprev;
-    const double *gs_coef;   // [2*nprev] c_j, then invnorm, then error flag
-    double *partials;        // per-workgroup outputs
+    const double *gs_coef;   // [2*nprev] c_j, invnorm, flag
+    double *partials;        // per-workgroup outputs of this pass
     int npart;
+    // observable finalize (of the previous pass's partials)
+    const double *fin_part;
+    int fin_nwg;
+    double *fin_rec;
+    ObsSpec fin_spec;
     Phys P;
 };
 
-// Line multi-index helpers: a line is identified by L in [0, nlines):
-// o = L / S enumerates the axes before `axis`, L % S the axes after it.
-__device__ __forceinline__ void line_index(const Phys &P, int axis, int64_t S, int64_t L, int *idx)
+// Multi-index of the line L (all axes but `axis`); all extents are 2^lg.
+__device__ __forceinline__ void line_index(const Phys &P, int axis, int64_t S, int log2S, int64_t L, int *idx)
 {
-    int64_t o = L / S, in = L - o * S;
+    int64_t o = L >> log2S, in = L & (S - 1);
+    int sh = 0;
 #pragma unroll
     for (int j = QND_MAXD - 1; j >= 0; --j) {
         idx[j] = 0;
         if (j < P.dim && j > axis) {
-            idx[j] = (int)(in % P.n[j]);
-            in /= P.n[j];
+            idx[j] = (int)((in >> sh) & (P.n[j] - 1));
+            sh += P.lg[j];
         }
     }
+    sh = 0;
 #pragma unroll
     for (int j = QND_MAXD - 1; j >= 0; --j) {
         if (j < axis) {
-            idx[j] = (int)(o % P.n[j]);
-            o /= P.n[j];
+            idx[j] = (int)((o >> sh) & (P.n[j] - 1));
+            sh += P.lg[j];
+        }
+    }
+}
+
+// turn per-workgroup [n0, d_j, g_j] sums into the observable record (wave 0
+// of block 0)
+__device__ __forceinline__ void obs_finalize(const double *part, int nwg, int dim, double dNr,
+                                             const ObsSpec &spec, double *rec)
+{
+    if (threadIdx.x >= 64) return;
+    double tot[QND_NOBS];
+#pragma unroll
+    for (int q = 0; q < QND_NOBS; ++q) tot[q] = wave_sum_column(part, nwg, QND_NOBS, q) * dNr;
+    if (threadIdx.x == 0) {
+        int off = 0;
+        for (int k = 0; k < spec.ncodes; ++k) {
+            const int c = spec.codes[k];
+            if (c == 5) rec[off++] = tot[0];                        // norm
+            else if (c == 3) {                                      // dipole_r
+#pragma unroll
+                for (int j = 0; j < QND_MAXD; ++j)
+                    if (j < dim) rec[off++] = tot[1 + j];
+            } else if (c == 6) {                                    // dipole_a
+#pragma unroll
+                for (int j = 0; j < QND_MAXD; ++j)
+                    if (j < dim) rec[off++] = -tot[1 + QND_MAXD + j] - spec.E[j] * tot[0];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// the line-pass kernel
+// ---------------------------------------------------------------------
+#define QND_PASS_THREADS 512   // workgroup cap: leaves 256 VGPRs per lane
+template <typename R, int KIND, int DIR1>
+__global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double red[16];
+    C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);
+    constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D);
+
+    const int tid = threadIdx.x;
+    const int n = a.n, T = a.T, TW = a.TW, ldsS = a.ldsStride;
+    const Phys &P = a.P;
+    const unsigned fl = a.flags;
+
+    if (KIND == PK_KIN && (fl & PF_FINALIZE) && blockIdx.x == 0)
+        obs_finalize(a.fin_part, a.fin_nwg, P.dim, P.dNr, a.fin_spec, a.fin_rec);
+
+    // ---- tile geometry: TW lines, full tiles (TW | S or S == 1) ----
+    const int64_t line0 = (int64_t)blockIdx.x * TW;
+    const bool contig = (a.S == 1);
+    int log2S = 0;
+    while (((int64_t)1 << log2S) < a.S) ++log2S;
+    int64_t base;
+    if (contig) base = line0 << a.log2n;
+    else {
+        const int64_t o = line0 >> log2S;
+        base = ((o << a.log2n) << log2S) + (line0 & (a.S - 1));
+    }
+    const int tot = TW << a.log2n;
+
+    // ---- load (with the Gram-Schmidt update prologue) ----
+    double gsinv = 1.0;
+    if (POTK && (fl & PF_GS_UPDATE)) gsinv = a.gs_coef[2 * a.nprev];
+    for (int q = tid; q < tot; q += blockDim.x) {
+        int l, i;
+        int64_t g;
+        if (contig) { l = q >> a.log2n; i = q & (n - 1); g = base + q; }
+        else { i = q >> a.log2TW; l = q & (TW - 1); g = base + ((int64_t)i << log2S) + l; }
+        C<R> v = a.data[g];
+        if (POTK && (fl & PF_GS_UPDATE)) {
+            double vr = (double)v.x, vi = (double)v.y;
+            for (int j = 0; j < a.nprev; ++j) {
+                const C<R> pj = a.prev[(int64_t)j * a.slot_stride + g];
+                const double cr = a.gs_coef[2 * j], ci = a.gs_coef[2 * j + 1];
+                vr -= cr * (double)pj.x - ci * (double)pj.y;
+                vi -= cr * (double)pj.y + ci * (double)pj.x;
+            }
+            v = mk<R>((R)(vr * gsinv), (R)(vi * gsinv));
+        }
+        lds[l * ldsS + i] = v;
+    }
+    __syncthreads();
+
+    // ---- thread <-> (line, t) ----
+    const int l = tid >> a.log2T, t = tid & (T - 1);
+    C<R> *line = lds + l * ldsS;
+    const int ax = a.axis;
+
+    if (KIND == PK_PLAIN) {
+        fft_line<R, DIR1>(line, n, a.log2n, T, t, a.tw);
+        if (a.scale1 != 1.0) {
+            const R s1 = (R)a.scale1;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) line[t + e * T] = scl<R>(line[t + e * T], s1);
+        }
+        __syncthreads();
+    } else {
+        int idx[QND_MAXD];
+        line_index(P, ax, a.S, log2S, line0 + l, idx);
+
+        if (KIND == PK_POT1D && (fl & PF_KPRE)) {
+#pragma unroll 1
+            for (int e = 0; e < 8; ++e) {
+                const int i = t + e * T;
+                const double p = coord_p(P, ax, i);
+                const double K = .5 * p * p;
+                if (a.kmode == KM_IMAG) line[i] = scl<R>(line[i], real_exp<R>(-a.ck_pre * K));
+                else line[i] = fac_mul<R>(line[i], phase_factor<R>(0.0, a.ck_pre * K));
+            }
+            __syncthreads();
+        }
+
+        // ---- first transform ----
+        fft_line<R, DIR1>(line, n, a.log2n, T, t, a.tw);
+
+        if (KIND == PK_KIN) {
+            // all axes in k-space: kinetic factor (reference order of the sums)
+            double Ko = 0.0, Wk = 0.0;
+#pragma unroll
+            for (int j = 0; j < QND_MAXD; ++j) {
+                if (j < P.dim && j != ax) {
+                    const double pj = coord_p(P, j, idx[j]);
+                    Ko += .5 * pj * pj;
+                    Wk += (pj + .5 * a.Ak[j]) * a.Ak[j];
+                }
+            }
+            const double Aax = a.Ak[ax];
+#pragma unroll 1
+            for (int e = 0; e < 8; ++e) {
+                const int i = t + e * T;
+                const double p = coord_p(P, ax, i);
+                const double K = Ko + .5 * p * p;
+                if (a.kmode == KM_IMAG) line[i] = scl<R>(line[i], real_exp<R>(-a.ck_mid * K));
+                else if (a.kmode == KM_VELOCITY)
+                    line[i] = fac_mul<R>(line[i], phase_factor<R>(0.0, a.ck_mid * (K + (Wk + (p + .5 * Aax) * Aax))));
+                else line[i] = fac_mul<R>(line[i], phase_factor<R>(0.0, a.ck_mid * K));
+            }
+            __syncthreads();
+        } else {
+            // all axes in r-space: potential, CAP, field, mask, observables
+            double rro = 0.0, Vco = 0.0, Wv = 0.0, mko = 1.0;
+            double xo[QND_MAXD] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < QND_MAXD; ++j) {
+                if (j < P.dim && j != ax) {
+                    const double xj = coord_x(P, j, idx[j]);
+                    xo[j] = xj;
+                    rro += xj * xj;
+                    Wv += xj * a.Ev[j];
+                    if (P.cap_on) Vco += P.capt[P.off[j] + idx[j]];
+                    if (P.mask_on) mko *= P.maskt[P.off[j] + idx[j]];
+                }
+            }
+            double obs[QND_NOBS];
+#pragma unroll
+            for (int q = 0; q < QND_NOBS; ++q) obs[q] = 0.0;
+            const double s1 = a.scale1;
+            const bool need_v = (fl & (PF_V1 | PF_V2 | PF_OBS)) != 0;
+#pragma unroll 1
+            for (int e = 0; e < 8; ++e) {
+                const int i = t + e * T;
+                C<R> v = scl<R>(line[i], (R)s1);
+                const double x = coord_x(P, ax, i);
+                double dVdu = 0.0, V = 0.0;
+                if (need_v) V = potential(P, x * x + rro, dVdu);
+                const double Vc = P.cap_on ? (P.capt[P.off[ax] + i] + Vco) : 0.0;
+                if (fl & PF_V1) {
+                    if (a.vmode == VM_IMAG) v = scl<R>(v, real_exp<R>(-a.cv1 * V));
+                    else if (a.vmode == VM_LENGTH)
+                        v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, a.cv1 * (V + (x * a.Ev[ax] + Wv))));
+                    else v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, a.cv1 * V));
+                }
+                if ((fl & PF_MASK) && P.mask_on) v = scl<R>(v, (R)(P.maskt[P.off[ax] + i] * mko));
+                if (fl & PF_OBS) {
+                    const double d = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+                    obs[0] += d;
+#pragma unroll
+                    for (int j = 0; j < QND_MAXD; ++j) {
+                        if (j < P.dim) {
+                            const double xj = (j == ax) ? x : xo[j];
+                            obs[1 + j] += d * xj;
+                            obs[1 + QND_MAXD + j] += d * dVdu * 2.0 * xj;
+                        }
+                    }
+                }
+                if (fl & PF_V2) {
+                    if (a.vmode == VM_IMAG) v = scl<R>(v, real_exp<R>(-a.cv2 * V));
+                    else v = fac_mul<R>(v, phase_factor<R>(-a.cv2 * Vc, a.cv2 * V));
+                }
+                line[i] = v;
+            }
+            if (fl & PF_OBS) {
+#pragma unroll
+                for (int q = 0; q < QND_NOBS; ++q) {
+                    const double s = block_sum(obs[q], red);
+                    if (tid == 0) a.partials[(int64_t)blockIdx.x * a.npart + q] = s;
+                }
+            }
+            __syncthreads();
+        }
+
+        // ---- second transform ----
+        if (KIND == PK_KIN || (fl & PF_SECOND)) {
+            fft_line<R, -DIR1>(line, n, a.log2n, T, t, a.tw);
+            const R s2 = (R)a.scale2;
+            if (KIND == PK_POT1D && (fl & PF_KPOST)) {
+#pragma unroll 1
+                for (int e = 0; e < 8; ++e) {
+                    const int i = t + e * T;
+                    const double p = coord_p(P, ax, i);
+                    const double K = .5 * p * p;
+                    const C<R> v = scl<R>(line[i], s2);
+                    if (a.kmode == KM_IMAG) line[i] = scl<R>(v, real_exp<R>(-a.ck_post * K));
+                    else if (a.kmode == KM_VELOCITY)
+                        line[i] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * (K + (p + .5 * a.Ak[ax]) * a.Ak[ax])));
+                    else line[i] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * K));
+                }
+            } else if (s2 != (R)1) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) line[t + e * T] = scl<R>(line[t + e * T], s2);
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- store ----
+    for (int q = tid; q < tot; q += blockDim.x) {
+        int ll, i;
+        int64_t g;
+        if (contig) { ll = q >> a.log2n; i = q & (n - 1); g = base + q; }
+        else { i = q >> a.log2TW; ll = q & (TW - 1); g = base + ((int64_t)i << log2S) + ll; }
+        a.data[g] = lds[ll * ldsS + i];
+    }
+
+    // ---- Gram-Schmidt dots on the stored representation ----
+    if (POTK && (fl & PF_GS_DOTS)) {
+        for (int j = 0; j <= a.nprev; ++j) {
+            double sr = 0.0, si = 0.0;
+            for (int q = tid; q < tot; q += blockDim.x) {
+                int ll, i;
+                int64_t g;
+                if (contig) { ll = q >> a.log2n; i = q & (n - 1); g = base + q; }
+                else { i = q >> a.log2TW; ll = q & (TW - 1); g = base + ((int64_t)i << log2S) + ll; }
+                const C<R> v = lds[ll * ldsS + i];
+                if (j < a.nprev) {
+                    const C<R> pj = a.prev[(int64_t)j * a.slot_stride + g];
+                    sr += (double)pj.x * (double)v.x + (double)pj.y * (double)v.y;
+                    si += (double)pj.x * (double)v.y - (double)pj.y * (double)v.x;
+                } else {
+                    sr += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+                }
+            }
+            const double tr = block_sum(sr, red);
+            const double ti = block_sum(si, red);
+            if (tid == 0) {
+                a.partials[(int64_t)blockIdx.x * a.npart + 2 * j] = tr;
+                a.partials[(int64_t)blockIdx.x * a.npart + 2 * j + 1] = ti;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------
+
+// out[q] = scale * sum_w part[w*npart + q], one wavefront per q
+__global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__ part, int nwg, int npart,
+                                                      int nq, double scale, double *out)
+{
+    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    for (int q = blockIdx.x * nwv + wv; q < nq; q += gridDim.x * nwv) {
+        const double s = wave_sum_column(part, nwg, npart, q);
+        if ((threadIdx.x & 63) == 0) out[q] = s * scale;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_obs_finalize(const double *__restrict__ part, int nwg, int dim,
+                                                     double dNr, ObsSpec spec, double *rec)
+{
+    obs_finalize(part, nwg, dim, dNr, spec, rec);
+}
+
+// Modified Gram-Schmidt coefficients from partial dots in the stored
+// representation (scale = dNr / n0 for the axis-0 Fourier representation):
+// b_j = <prev_j|v>, nv = <v|v>; c_j = b_j - sum_{i<j} c_i G_{j i};
+// |v'|^2 = nv - 2 Re sum_j c_j conj(b_j) + sum_{ij} conj(c_i) c_j G_{ij}.
+// coef = [c_0 .. c_{N-1} (re,im), invnorm, flag, norm]
+__global__ void __launch_bounds__(256) k_gs_coeffs(const double *__restrict__ part, int nwg, int npart,
+                                                   int nprev, double scale, const double *__restrict__ gram,
+                                                   double *coef)
+{
+    __shared__ double b[2 * 65];
+    __shared__ double cre[64], cim[64];
+    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    for (int q = wv; q < 2 * (nprev + 1); q += nwv) {
+        const double s = wave_sum_column(part, nwg, npart, q);
+        if ((threadIdx.x & 63) == 0) b[q] = s * scale;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double nv = b[2 * nprev];
+        for (int j = 0; j < nprev; ++j) {
+            double r = b[2 * j], im = b[2 * j + 1];
+            for (int i = 0; i < j; ++i) {
+                const double gr = gram[2 * (j * nprev + i)], gi = gram[2 * (j * nprev + i) + 1];
+                r -= cre[i] * gr - cim[i] * gi;
+                im -= cre[i] * gi + cim[i] * gr;
+            }
+            cre[j] = r;
+            cim[j] = im;
+        }
+        double n2 = nv;
+        for (int j = 0; j < nprev; ++j) n2 -= 2.0 * (cre[j] * b[2 * j] + cim[j] * b[2 * j + 1]);
+        for (int i = 0; i < nprev; ++i)
+            for (int j = 0; j < nprev; ++j) {
+                const double gr = gram[2 * (i * nprev + j)], gi = gram[2 * (i * nprev + j) + 1];
+                const double qr = cre[i] * cre[j] + cim[i] * cim[j];
+                const double qi = cre[i] * cim[j] - cim[i] * cre[j];
+                n2 += qr * gr - qi * gi;
+            }
+        for (int j = 0; j < nprev; ++j) {
+            coef[2 * j] = cre[j];
+            coef[2 * j + 1] = cim[j];
+        }
+        if (n2 < 1.e-12) { coef[2 * nprev] = 0.0; coef[2 * nprev + 1] = 1.0; }  // sticky flag
+        else coef[2 * nprev] = 1.0 / sqrt(n2);
+        coef[2 * nprev + 2] = n2;
+    }
+}
+
+// ---------------------------------------------------------------------
+// flat elementwise and reduction kernels (representation-agnostic helpers
+// for the per-state observables, Gram matrix and standalone updates)
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void flat_index(const Phys &P, int64_t f, int *idx)
+{
+    int sh = 0;
+#pragma unroll
+    for (int j = QND_MAXD - 1; j >= 0; --j) {
+        idx[j] = 0;
+        if (j < P.dim) {
+            idx[j] = (int)((f >> sh) & (P.n[j] - 1));
+            sh += P.lg[j];
         }
     }
 }
@@ -337,349 +760,7 @@ __device__ __forceinline__ int pick(const int *idx, int j)
     return v;
 }
 
-template <typename R, int DIR1>
-__device__ __forceinline__ void pass_body(const PassArgs<R> &a, C<R> *lds, double *red)
-{
-    const int tid = threadIdx.x;
-    const int n = a.n, T = a.T, TW = a.TW, ldsS = a.ldsStride;
-    const int64_t line0 = (int64_t)blockIdx.x * TW;
-    const int nl = (int)((a.nlines - line0) < TW ? (a.nlines - line0) : TW);
-    const Phys &P = a.P;
-    const unsigned fl = a.flags;
-
-    // ---- load the tile (with the Gram-Schmidt update prologue) ----
-    const bool contig = (a.S == 1);
-    int64_t base;
-    if (contig) base = line0 * n;
-    else {
-        int64_t o = line0 / a.S;
-        base = o * n * a.S + (line0 - o * a.S);
-    }
-    const int tot = nl * n;
-    double gsinv = 1.0;
-    if (fl & PF_GS_UPDATE) gsinv = a.gs_coef[2 * a.nprev];
-    for (int idx = tid; idx < tot; idx += blockDim.x) {
-        int l, i;
-        int64_t g;
-        if (contig) { l = idx / n; i = idx - l * n; g = base + idx; }
-        else { i = idx / nl; l = idx - i * nl; g = base + (int64_t)i * a.S + l; }
-        C<R> v = a.data[g];
-        if (fl & PF_GS_UPDATE) {
-            C<double> vd = mk<double>((double)v.x, (double)v.y);
-            for (int j = 0; j < a.nprev; ++j) {
-                C<R> pj = a.prev[(int64_t)j * a.slot_stride + g];
-                double cr = a.gs_coef[2 * j], ci = a.gs_coef[2 * j + 1];
-                vd.x -= cr * (double)pj.x - ci * (double)pj.y;
-                vd.y -= cr * (double)pj.y + ci * (double)pj.x;
-            }
-            v = mk<R>((R)(vd.x * gsinv), (R)(vd.y * gsinv));
-        }
-        lds[l * ldsS + i] = v;
-    }
-    __syncthreads();
-
-    // ---- thread <-> (line, t) ----
-    const int l = tid / T, t = tid - (tid / T) * T;
-    const bool active = l < nl;
-    C<R> *line = lds + l * ldsS;
-    int idx[QND_MAXD] = {0, 0, 0, 0};
-    if (active) line_index(P, a.axis, a.S, line0 + l, idx);
-    const int ax = a.axis;
-
-    // per-line partial sums over the other axes (axis order as the reference)
-    double Ko = 0.0, Wk = 0.0, rro = 0.0, Vco = 0.0, Wv = 0.0, mko = 1.0;
-    double xo[QND_MAXD] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int j = 0; j < QND_MAXD; ++j) {
-        if (j >= P.dim || j == ax) continue;
-        if (fl & (PF_KMID)) {
-            double pj = coord_p(P, j, idx[j]);
-            Ko += .5 * pj * pj;
-            if (a.kmode == KM_VELOCITY) Wk += (pj + .5 * a.Ak[j]) * a.Ak[j];
-        }
-        if (fl & (PF_V1 | PF_V2 | PF_MASK | PF_OBS)) {
-            double xj = coord_x(P, j, idx[j]);
-            xo[j] = xj;
-            rro += xj * xj;
-            Vco += cap_axis(P, j, xj);
-            Wv += xj * a.Ev[j];
-            if ((fl & PF_MASK) && P.mask_on) mko *= mask_axis(P, j, xj);
-        }
-    }
-
-    // ---- kinetic factor before the first transform (d == 1) ----
-    if ((fl & PF_KPRE) && active) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int i = t + e * T;
-            double p = coord_p(P, ax, i);
-            double K = .5 * p * p;
-            C<R> f;
-            if (a.kmode == KM_IMAG) { R r = real_exp<R>(-a.ck_pre * K); f = mk<R>(r, 0); }
-            else f = phase_factor<R>(0.0, a.ck_pre * K);
-            line[i] = mul<R>(line[i], f);
-        }
-    }
-    if (fl & PF_KPRE) __syncthreads();
-
-    // ---- first transform ----
-    fft_line<R, DIR1>(line, n, a.log2n, T, t, a.tw, active);
-
-    // ---- middle: scale, kinetic or potential operators, observables ----
-    double obs[QND_NOBS];
-#pragma unroll
-    for (int q = 0; q < QND_NOBS; ++q) obs[q] = 0.0;
-    if (active) {
-        const R s1 = (R)a.scale1;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int i = t + e * T;
-            C<R> v = line[i];
-            if (s1 != (R)1) v = scl<R>(v, s1);
-            if (fl & PF_KMID) {
-                double p = coord_p(P, ax, i);
-                double K = Ko + .5 * p * p;
-                C<R> f;
-                if (a.kmode == KM_IMAG) { R r = real_exp<R>(-a.ck_mid * K); f = mk<R>(r, 0); }
-                else if (a.kmode == KM_VELOCITY) {
-                    double W = Wk + (p + .5 * a.Ak[ax]) * a.Ak[ax];
-                    f = phase_factor<R>(0.0, a.ck_mid * (K + W));
-                } else f = phase_factor<R>(0.0, a.ck_mid * K);
-                v = mul<R>(v, f);
-            }
-            if (fl & (PF_V1 | PF_V2 | PF_MASK | PF_OBS)) {
-                const double x = coord_x(P, ax, i);
-                const double rr = x * x + rro;
-                double dVdu = 0.0;
-                const double V = potential(P, rr, (fl & PF_OBS) ? &dVdu : nullptr);
-                const double Vc = cap_axis(P, ax, x) + Vco;
-                if (fl & PF_V1) {
-                    C<R> f;
-                    if (a.vmode == VM_IMAG) { R r = real_exp<R>(-a.cv1 * V); f = mk<R>(r, 0); }
-                    else if (a.vmode == VM_LENGTH) {
-                        double W = x * a.Ev[ax] + Wv;
-                        f = phase_factor<R>(-a.cv1 * Vc, a.cv1 * (V + W));
-                    } else f = phase_factor<R>(-a.cv1 * Vc, a.cv1 * V);
-                    v = mul<R>(v, f);
-                }
-                if ((fl & PF_MASK) && P.mask_on) {
-                    R m = (R)(mask_axis(P, ax, x) * mko);
-                    v = scl<R>(v, m);
-                }
-                if (fl & PF_OBS) {
-                    double d = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-                    obs[0] += d;
-#pragma unroll
-                    for (int j = 0; j < QND_MAXD; ++j) {
-                        if (j < P.dim) {
-                            double xj = (j == ax) ? x : xo[j];
-                            obs[1 + j] += d * xj;
-                            obs[1 + QND_MAXD + j] += d * dVdu * 2.0 * xj;
-                        }
-                    }
-                }
-                if (fl & PF_V2) {
-                    C<R> f;
-                    if (a.vmode == VM_IMAG) { R r = real_exp<R>(-a.cv2 * V); f = mk<R>(r, 0); }
-                    else f = phase_factor<R>(-a.cv2 * Vc, a.cv2 * V);
-                    v = mul<R>(v, f);
-                }
-            }
-            line[i] = v;
-        }
-    }
-    if (fl & PF_OBS) {
-        for (int q = 0; q < 1 + 2 * QND_MAXD; ++q) {
-            double s = block_sum(obs[q], red);
-            if (tid == 0) a.partials[(int64_t)blockIdx.x * a.npart + q] = s;
-        }
-    }
-    __syncthreads();
-
-    // ---- second transform ----
-    if (fl & PF_SECOND) {
-        fft_line<R, -DIR1>(line, n, a.log2n, T, t, a.tw, active);
-        if (active) {
-            const R s2 = (R)a.scale2;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int i = t + e * T;
-                C<R> v = line[i];
-                if (s2 != (R)1) v = scl<R>(v, s2);
-                if (fl & PF_KPOST) {
-                    double p = coord_p(P, ax, i);
-                    double K = .5 * p * p;
-                    C<R> f;
-                    if (a.kmode == KM_IMAG) { R r = real_exp<R>(-a.ck_post * K); f = mk<R>(r, 0); }
-                    else if (a.kmode == KM_VELOCITY) {
-                        double W = (p + .5 * a.Ak[ax]) * a.Ak[ax];
-                        f = phase_factor<R>(0.0, a.ck_post * (K + W));
-                    } else f = phase_factor<R>(0.0, a.ck_post * K);
-                    v = mul<R>(v, f);
-                }
-                line[i] = v;
-            }
-        }
-        __syncthreads();
-    }
-
-    // ---- store (and Gram-Schmidt dots on the stored representation) ----
-    for (int idx2 = tid; idx2 < tot; idx2 += blockDim.x) {
-        int ll, i;
-        int64_t g;
-        if (contig) { ll = idx2 / n; i = idx2 - ll * n; g = base + idx2; }
-        else { i = idx2 / nl; ll = idx2 - i * nl; g = base + (int64_t)i * a.S + ll; }
-        a.data[g] = lds[ll * ldsS + i];
-    }
-    if (fl & PF_GS_DOTS) {
-        // <prev_j | v> partials and |v|^2 over this tile (same element order
-        // for every j: deterministic)
-        for (int j = 0; j <= a.nprev; ++j) {
-            double sr = 0.0, si = 0.0;
-            for (int idx2 = tid; idx2 < tot; idx2 += blockDim.x) {
-                int ll, i;
-                int64_t g;
-                if (contig) { ll = idx2 / n; i = idx2 - ll * n; g = base + idx2; }
-                else { i = idx2 / nl; ll = idx2 - i * nl; g = base + (int64_t)i * a.S + ll; }
-                C<R> v = lds[ll * ldsS + i];
-                if (j < a.nprev) {
-                    C<R> pj = a.prev[(int64_t)j * a.slot_stride + g];
-                    sr += (double)pj.x * (double)v.x + (double)pj.y * (double)v.y;
-                    si += (double)pj.x * (double)v.y - (double)pj.y * (double)v.x;
-                } else {
-                    sr += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-                }
-            }
-            double tr = block_sum(sr, red);
-            double ti = block_sum(si, red);
-            if (tid == 0) {
-                a.partials[(int64_t)blockIdx.x * a.npart + 2 * j] = tr;
-                a.partials[(int64_t)blockIdx.x * a.npart + 2 * j + 1] = ti;
-            }
-        }
-    }
-}
-
-template <typename R>
-__global__ void __launch_bounds__(1024) k_pass(PassArgs<R> a)
-{
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double red[16];
-    C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);
-    if (a.dir_first < 0) pass_body<R, -1>(a, lds, red);
-    else pass_body<R, +1>(a, lds, red);
-}
-
-// ---------------------------------------------------------------------
-// small kernels
-// ---------------------------------------------------------------------
-
-// Sum per-workgroup partials: out[q] = scale * sum_w part[w*npart + q]
-// (one thread per q, sequential over w: deterministic).
-__global__ void k_sum_partials(const double *__restrict__ part, int nwg, int npart, int nq,
-                               double scale, double *out)
-{
-    for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-        double s = 0.0;
-        for (int w = 0; w < nwg; ++w) s += part[(int64_t)w * npart + q];
-        out[q] = s * scale;
-    }
-}
-
-// Observables from r-space partial sums [n0, d_j, g_j] of one step:
-// writes rec[0..width) per the code list (norm, dipole_r, dipole_a).
-struct ObsSpec {
-    int ncodes;
-    int codes[8];
-    double E[QND_MAXD];
-    double dNr;
-};
-__global__ void k_obs_finalize(const double *__restrict__ part, int nwg, int npart, int dim,
-                               ObsSpec spec, double *rec)
-{
-    __shared__ double tot[QND_NOBS];
-    for (int q = threadIdx.x; q < QND_NOBS; q += blockDim.x) {
-        double s = 0.0;
-        for (int w = 0; w < nwg; ++w) s += part[(int64_t)w * npart + q];
-        tot[q] = s * spec.dNr;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int off = 0;
-        for (int k = 0; k < spec.ncodes; ++k) {
-            int c = spec.codes[k];
-            if (c == 5) { rec[off++] = tot[0]; }                 // norm
-            else if (c == 3) { for (int j = 0; j < dim; ++j) rec[off++] = tot[1 + j]; }
-            else if (c == 6) { for (int j = 0; j < dim; ++j) rec[off++] = -tot[1 + QND_MAXD + j] - spec.E[j] * tot[0]; }
-            else if (c == 1 || c == 2) { off += 1; }             // filled by the host path
-            else if (c == 4) { off += dim; }
-        }
-    }
-}
-
-// Modified Gram-Schmidt coefficients from partial dots in the stored
-// representation (scale = dNr / n0 for the axis-0 Fourier representation):
-// b_j = <prev_j|v>, nv = <v|v>; c_j = b_j - sum_{i<j} c_i G_{j i};
-// |v'|^2 = nv - 2 Re sum_j c_j conj(b_j) + sum_{ij} conj(c_i) c_j G_{ij}.
-// coef = [c_0 .. c_{N-1} (re,im), invnorm, flag, norm]
-__global__ void k_gs_coeffs(const double *__restrict__ part, int nwg, int npart, int nprev,
-                            double scale, const double *__restrict__ gram, double *coef)
-{
-    __shared__ double b[2 * 65];
-    for (int q = threadIdx.x; q < 2 * (nprev + 1); q += blockDim.x) {
-        double s = 0.0;
-        for (int w = 0; w < nwg; ++w) s += part[(int64_t)w * npart + q];
-        b[q] = s * scale;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double nv = b[2 * nprev];
-        double cr[64], ci[64];
-        for (int j = 0; j < nprev; ++j) {
-            double r = b[2 * j], im = b[2 * j + 1];
-            for (int i = 0; i < j; ++i) {
-                // G_{j i} = <prev_j|prev_i>
-                double gr = gram[2 * (j * nprev + i)], gi = gram[2 * (j * nprev + i) + 1];
-                r -= cr[i] * gr - ci[i] * gi;
-                im -= cr[i] * gi + ci[i] * gr;
-            }
-            cr[j] = r;
-            ci[j] = im;
-        }
-        double n2 = nv;
-        for (int j = 0; j < nprev; ++j) n2 -= 2.0 * (cr[j] * b[2 * j] + ci[j] * b[2 * j + 1]);
-        for (int i = 0; i < nprev; ++i)
-            for (int j = 0; j < nprev; ++j) {
-                double gr = gram[2 * (i * nprev + j)], gi = gram[2 * (i * nprev + j) + 1];
-                // conj(c_i) c_j G_ij
-                double qr = cr[i] * cr[j] + ci[i] * ci[j];
-                double qi = cr[i] * ci[j] - ci[i] * cr[j];
-                n2 += qr * gr - qi * gi;
-            }
-        for (int j = 0; j < nprev; ++j) { coef[2 * j] = cr[j]; coef[2 * j + 1] = ci[j]; }
-        if (n2 < 1.e-12) { coef[2 * nprev] = 0.0; coef[2 * nprev + 1] = 1.0; }  // sticky flag
-        else { coef[2 * nprev] = 1.0 / sqrt(n2); }
-        coef[2 * nprev + 2] = n2;
-    }
-}
-
-// ---------------------------------------------------------------------
-// flat elementwise and reduction kernels (representation-agnostic helpers
-// for the per-state observables, Gram matrix and standalone updates)
-// ---------------------------------------------------------------------
-__device__ __forceinline__ void flat_index(const Phys &P, int64_t f, int *idx)
-{
-#pragma unroll
-    for (int j = QND_MAXD - 1; j >= 0; --j) {
-        idx[j] = 0;
-        if (j < P.dim) {
-            idx[j] = (int)(f % P.n[j]);
-            f /= P.n[j];
-        }
-    }
-}
-
-enum ElemMode { EM_COPY = 0, EM_MUL_KIN = 1, EM_MUL_PSHIFT = 2, EM_GS_UPDATE = 3, EM_SCALE = 4 };
+enum ElemMode { EM_COPY = 0, EM_MUL_KIN = 1, EM_MUL_PSHIFT = 2, EM_GS_UPDATE = 3, EM_SCALE = 4, EM_RMASK = 5 };
 
 template <typename R>
 __global__ void k_elem(int mode, C<R> *dst, const C<R> *src, int axis, double shift, int64_t N,
@@ -695,7 +776,7 @@ __global__ void k_elem(int mode, C<R> *dst, const C<R> *src, int axis, double sh
 #pragma unroll
             for (int j = 0; j < QND_MAXD; ++j) {
                 if (j < P.dim) {
-                    double p = coord_p(P, j, idx[j]);
+                    const double p = coord_p(P, j, idx[j]);
                     K += .5 * p * p;
                 }
             }
@@ -707,15 +788,35 @@ __global__ void k_elem(int mode, C<R> *dst, const C<R> *src, int axis, double sh
         } else if (mode == EM_GS_UPDATE) {
             double vr = v.x, vi = v.y;
             for (int j = 0; j < nprev; ++j) {
-                C<R> pj = prev[(int64_t)j * slot_stride + f];
-                double cr = coef[2 * j], ci = coef[2 * j + 1];
+                const C<R> pj = prev[(int64_t)j * slot_stride + f];
+                const double cr = coef[2 * j], ci = coef[2 * j + 1];
                 vr -= cr * (double)pj.x - ci * (double)pj.y;
                 vi -= cr * (double)pj.y + ci * (double)pj.x;
             }
-            double inv = coef[2 * nprev];
+            const double inv = coef[2 * nprev];
             v = mk<R>((R)(vr * inv), (R)(vi * inv));
         } else if (mode == EM_SCALE) {
             v = scl<R>(v, (R)shift);
+        } else if (mode == EM_RMASK) {
+            // radial sin^2 mask of the final observables (src/io_functions.c:159-167);
+            // R0 = shift, R1 = coef[0]
+            int idx[QND_MAXD];
+            flat_index(P, f, idx);
+            double rr = 0.0;
+#pragma unroll
+            for (int j = 0; j < QND_MAXD; ++j) {
+                if (j < P.dim) {
+                    const double x = coord_x(P, j, idx[j]);
+                    rr += x * x;
+                }
+            }
+            const double rad = sqrt(rr), R0 = shift, R1 = coef[0];
+            double fm = 0.0;
+            if (rad >= R0 && rad <= R1) {
+                const double sn = sin(.5 * 3.141592653589793 * (rad - R0) / (R1 - R0));
+                fm = sn * sn;
+            } else if (rad > R1) fm = 1.0;
+            v = mk<R>((R)(fm * (double)v.x), (R)(fm * (double)v.y));
         }
         dst[f] = v;
     }
@@ -728,7 +829,7 @@ enum RedMode { RM_DOT = 0, RM_DOTX = 1, RM_PMOM = 2, RM_OBS = 3, RM_HCOMB = 4 };
 //  RM_DOTX  : sum conj(a) x_axis b               -> 2
 //  RM_PMOM  : sum |a|^2 (p_j + alpha_j), j<dim   -> dim
 //  RM_OBS   : [ |a|^2, |a|^2 x_j, |a|^2 dV/dx_j ] -> 1 + 2*QND_MAXD
-//  RM_HCOMB : b <- b_in * fs + V b ; sum conj(a) b  -> 2   (b = H psi, b_in = kinetic part in c)
+//  RM_HCOMB : b <- c * fs + V b ; sum conj(a) b  -> 2   (b = H psi, c = kinetic part)
 template <typename R>
 __global__ void __launch_bounds__(256) k_red(int mode, const C<R> *a, C<R> *b, const C<R> *c, int axis,
                                              double fs, int64_t N, Phys P, const double *alpha,
@@ -740,10 +841,10 @@ __global__ void __launch_bounds__(256) k_red(int mode, const C<R> *a, C<R> *b, c
     for (int q = 0; q < QND_NOBS; ++q) acc[q] = 0.0;
     for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < N;
          f += (int64_t)gridDim.x * blockDim.x) {
-        C<R> va = a[f];
-        double ar = va.x, ai = va.y;
+        const C<R> va = a[f];
+        const double ar = va.x, ai = va.y;
         if (mode == RM_DOT || mode == RM_DOTX) {
-            C<R> vb = b[f];
+            const C<R> vb = b[f];
             double w = 1.0;
             if (mode == RM_DOTX) {
                 int idx[QND_MAXD];
@@ -755,58 +856,51 @@ __global__ void __launch_bounds__(256) k_red(int mode, const C<R> *a, C<R> *b, c
         } else if (mode == RM_PMOM) {
             int idx[QND_MAXD];
             flat_index(P, f, idx);
-            double d = ar * ar + ai * ai;
+            const double d = ar * ar + ai * ai;
 #pragma unroll
             for (int j = 0; j < QND_MAXD; ++j)
                 if (j < P.dim) acc[j] += d * (coord_p(P, j, idx[j]) + alpha[j]);
-        } else if (mode == RM_OBS) {
+        } else {
             int idx[QND_MAXD];
             flat_index(P, f, idx);
             double rr = 0.0;
 #pragma unroll
             for (int j = 0; j < QND_MAXD; ++j) {
                 if (j < P.dim) {
-                    double x = coord_x(P, j, idx[j]);
+                    const double x = coord_x(P, j, idx[j]);
                     rr += x * x;
                 }
             }
-            double dVdu;
-            (void)potential(P, rr, &dVdu);
-            double d = ar * ar + ai * ai;
-            acc[0] += d;
+            if (mode == RM_OBS) {
+                double dVdu;
+                (void)potential(P, rr, dVdu);
+                const double d = ar * ar + ai * ai;
+                acc[0] += d;
 #pragma unroll
-            for (int j = 0; j < QND_MAXD; ++j) {
-                if (j < P.dim) {
-                    double x = coord_x(P, j, idx[j]);
-                    acc[1 + j] += d * x;
-                    acc[1 + QND_MAXD + j] += d * dVdu * 2.0 * x;
+                for (int j = 0; j < QND_MAXD; ++j) {
+                    if (j < P.dim) {
+                        const double x = coord_x(P, j, idx[j]);
+                        acc[1 + j] += d * x;
+                        acc[1 + QND_MAXD + j] += d * dVdu * 2.0 * x;
+                    }
                 }
+            } else {  // RM_HCOMB
+                double dVdu;
+                const double V = potential(P, rr, dVdu);
+                const C<R> hk = c[f], hb = b[f];
+                const double hr = (double)hk.x * fs + V * (double)hb.x;
+                const double hi = (double)hk.y * fs + V * (double)hb.y;
+                b[f] = mk<R>((R)hr, (R)hi);
+                acc[0] += ar * hr + ai * hi;
+                acc[1] += ar * hi - ai * hr;
             }
-        } else if (mode == RM_HCOMB) {
-            int idx[QND_MAXD];
-            flat_index(P, f, idx);
-            double rr = 0.0;
-#pragma unroll
-            for (int j = 0; j < QND_MAXD; ++j) {
-                if (j < P.dim) {
-                    double x = coord_x(P, j, idx[j]);
-                    rr += x * x;
-                }
-            }
-            double V = potential(P, rr, nullptr);
-            C<R> hk = c[f], hb = b[f];
-            double hr = (double)hk.x * fs + V * (double)hb.x;
-            double hi = (double)hk.y * fs + V * (double)hb.y;
-            b[f] = mk<R>((R)hr, (R)hi);
-            acc[0] += ar * hr + ai * hi;
-            acc[1] += ar * hi - ai * hr;
         }
     }
     const int nv = (mode == RM_OBS) ? QND_NOBS : (mode == RM_PMOM ? P.dim : 2);
 #pragma unroll
     for (int q = 0; q < QND_NOBS; ++q) {
         if (q < nv) {
-            double s = block_sum(acc[q], red);
+            const double s = block_sum(acc[q], red);
             if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * npart + q] = s;
         }
     }
@@ -825,7 +919,7 @@ __global__ void __launch_bounds__(256) k_ped(const C<R> *psi, int64_t N, Phys P,
     const int64_t p0 = (int64_t)blockIdx.x * chunk;
     const int cnt = (int)((N - p0) < chunk ? (N - p0) : chunk);
     for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-        int64_t f = p0 + q;
+        const int64_t f = p0 + q;
         int idx[QND_MAXD];
         flat_index(P, f, idx);
         bool inside = true;
@@ -833,12 +927,12 @@ __global__ void __launch_bounds__(256) k_ped(const C<R> *psi, int64_t N, Phys P,
 #pragma unroll
         for (int j = 0; j < QND_MAXD; ++j) {
             if (j < P.dim) {
-                double p = coord_p(P, j, idx[j]);
+                const double p = coord_p(P, j, idx[j]);
                 if (fabs(p) > pr_max4[j]) inside = false;
                 K += .5 * p * p;
             }
         }
-        C<R> v = psi[f];
+        const C<R> v = psi[f];
         dens[q] = inside ? ((double)v.x * (double)v.x + (double)v.y * (double)v.y) : 0.0;
         kin[q] = K;
     }
@@ -848,7 +942,7 @@ __global__ void __launch_bounds__(256) k_ped(const C<R> *psi, int64_t N, Phys P,
         double s = 0.0;
         const double Ej = j * de;
         for (int q = 0; q < cnt; ++q) {
-            double dk = kin[q] - Ej;
+            const double dk = kin[q] - Ej;
             if (dens[q] != 0.0) s += dens[q] * exp(-.5 * dk * dk * inv2s2) * nf;
         }
         partials[(int64_t)blockIdx.x * Ne + j] = s;
