@@ -71,6 +71,7 @@ struct qnd_ctx {
     double *d_capt = nullptr;    // per-axis CAP table (sum over axes = Vcap)
     double *d_maskt = nullptr;   // per-axis mask table (product over axes)
     int tw_override[2] = {0, 0}; // lines per workgroup (contiguous, strided); QND_TW env
+    int vpt_override = 0;        // values per thread (4 or 8); QND_VPT env
     std::vector<SlotState> slots;
     AxisPlan plan[QND_MAXD];
     std::vector<double> Et, At, Eobs, Aobs;
@@ -170,7 +171,7 @@ void prof_flush(qnd_ctx *c)
 
 // ---- pass geometry ----
 struct Geo {
-    int axis, n, log2n, T, log2T, TW, log2TW, threads, grid, ldsStride;
+    int axis, n, log2n, vpt, T, log2T, TW, log2TW, threads, grid, ldsStride;
     int64_t S, nlines;
     size_t lds;
 };
@@ -188,11 +189,16 @@ Geo geometry(qnd_ctx *c, int axis)
     g.axis = axis;
     g.n = (int)c->nr[axis];
     g.log2n = ilog2(g.n);
-    g.T = g.n / 8;
     g.S = 1;
     for (int j = axis + 1; j < c->dim; ++j) g.S *= c->nr[j];
     g.nlines = c->N / g.n;
-    const size_t line_bytes = (size_t)g.n * c->esz;
+    // values per thread: 8 (radix-8 stages) unless the grid is too small to
+    // give every SIMD a wavefront that way, then 4 (twice the wavefronts)
+    g.vpt = 8;
+    const bool few_waves = c->N / 8 / 64 < 4096;
+    if ((c->vpt_override == 4 || (c->vpt_override == 0 && few_waves)) && g.n / 4 <= QND_PASS_THREADS) g.vpt = 4;
+    g.T = g.n / g.vpt;
+    const size_t line_bytes = 2 * (size_t)g.n * c->esz;  // two exchange buffers per line
     int twmax_lds = (int)(c->lds_max / line_bytes);
     if (twmax_lds < 1) twmax_lds = 1;
     int tw;
@@ -217,14 +223,14 @@ Geo geometry(qnd_ctx *c, int axis)
     if (g.S > 1 && (int64_t)tw > g.S) tw = (int)g.S;
     if (tw < 1) tw = 1;
     // one padding element per line spreads the strided tile rows over the banks
-    while (tw > 1 && (size_t)tw * (g.n + 1) * c->esz > c->lds_max) tw /= 2;
+    while (tw > 1 && 2 * (size_t)tw * (g.n + 1) * c->esz > c->lds_max) tw /= 2;
     g.TW = tw;
     g.log2TW = ilog2(tw);
     g.log2T = ilog2(g.T);
     g.threads = g.T * tw;
     g.grid = (int)(g.nlines / tw);
     g.ldsStride = g.n + (tw > 1 ? 1 : 0);
-    g.lds = (size_t)tw * g.ldsStride * c->esz;
+    g.lds = 2 * (size_t)tw * g.ldsStride * c->esz;
     return g;
 }
 
@@ -255,7 +261,8 @@ template <typename R, int KIND, int DIR>
 int launch(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, int cls)
 {
     ProfScope ps(c, cls);
-    hipLaunchKernelGGL((k_pass<R, KIND, DIR>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, a);
+    if (g.vpt == 4) hipLaunchKernelGGL((k_pass<R, KIND, DIR, 4>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, a);
+    else hipLaunchKernelGGL((k_pass<R, KIND, DIR, 8>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, a);
     HIPCK(c, hipGetLastError());
     return 0;
 }
@@ -1038,11 +1045,13 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     hipGetDevice(&c->device);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
+        // dynamic LDS per workgroup, less room for the kernels' static arrays
         size_t m = prop.sharedMemPerBlock;
-        c->lds_max = m > 65536 ? 65536 : m;
+        if (m > 163840) m = 163840;
+        c->lds_max = m > 4096 ? m - 4096 : m;
     }
     for (int j = 0; j < dim; ++j) {
-        if ((size_t)nr[j] * c->esz > c->lds_max || nr[j] / 8 > QND_PASS_THREADS) {
+        if (2 * (size_t)nr[j] * c->esz > c->lds_max || nr[j] / 8 > QND_PASS_THREADS) {
             delete c;
             return fail(nullptr, QND_EUNSUPPORTED, "NR[%d] = %lld exceeds the single-pass line limit", j,
                         (long long)nr[j]);
@@ -1072,6 +1081,10 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
             c->tw_override[1] = b;
         }
     }
+    if (const char *e = getenv("QND_VPT")) {  // tuning knob: values per thread
+        const int v = atoi(e);
+        if (v == 4 || v == 8) c->vpt_override = v;
+    }
     c->slots.resize(n_slots);
     int rc = 0;
     auto bail = [&](int code) {
@@ -1093,6 +1106,8 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     if (hipMalloc(&c->d_capt, off * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->d_maskt, off * sizeof(double)) != hipSuccess)
         return bail(fail(nullptr, QND_EHIP, "hipMalloc failed"));
+    hipMemset(c->d_capt, 0, off * sizeof(double));   // read unconditionally by the passes
+    hipMemset(c->d_maskt, 0, off * sizeof(double));
     P.capt = c->d_capt;
     P.maskt = c->d_maskt;
     for (int j = 0; j < dim; ++j) {

@@ -21,6 +21,19 @@
 
 namespace qnd {
 
+// Workgroup barrier that no memory access is moved across.  With a bare
+// __syncthreads() the compiler has sunk an LDS read of a Stockham stage
+// below the barrier that separates the stage's reads from its writes (the
+// read is first used after the twiddle block), so another wavefront's write
+// could land first.  The empty asm statements with a memory clobber pin
+// every load and store on its side of the barrier.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("" ::: "memory");
+    __syncthreads();
+    asm volatile("" ::: "memory");
+}
+
 // ---------------------------------------------------------------------
 // complex helpers
 // ---------------------------------------------------------------------
@@ -152,59 +165,105 @@ template <> __device__ __forceinline__ C<float> fac_mul<float>(C<float> x, const
 }
 
 // ---------------------------------------------------------------------
-// Stockham radix stage on one line held in LDS.  The thread owns butterflies
-// j = t + b*T (b < 8/RAD); every thread holds exactly 8 values.
-//   inputs  x[r] = line[j + r*n/RAD]
-//   twiddle x[r] *= W_n^{r k n/(Ns RAD)}, k = j mod Ns
-//   outputs line[(j-k)*RAD + k + r*Ns] = DFT_RAD(x)[r]
+// Register-resident Stockham transform of one line.
+//
+// T = n / VPT threads share a line; thread t holds x[m] = element t + m*T
+// (m < VPT) when the transform starts and again when it ends, so the global
+// load, the pointwise operators and the global store all work on registers.
+// Stage (radix RAD, span Ns) of the Stockham autosort: the thread owns the
+// butterflies j = t + b*T (b < VPT/RAD);
+//   inputs  y[r] = elem[j + r*n/RAD]
+//   twiddle y[r] *= W_n^{r k n/(Ns RAD)},  k = j mod Ns
+//   outputs elem[(j-k)*RAD + k + r*Ns] = DFT_RAD(y)[r]
+// The first stage's inputs (Ns = 1) are t + T*(b + r*VPT/RAD) and the last
+// stage's outputs (Ns = n/RAD, so k = j) are the same set: both map onto
+// x[b + r*VPT/RAD] without LDS.  Interior stages exchange through two LDS
+// buffers used alternately (one barrier per exchange: a buffer is written
+// again only after every thread has passed the barrier that follows its
+// reads).  `wc` counts the exchanges of the kernel and picks the buffer.
 // ---------------------------------------------------------------------
-template <typename R, int RAD, int DIR>
-__device__ __forceinline__ void stockham_stage(C<R> *line, int n, int T, int t, int Ns,
-                                               const typename TwT<R>::T *__restrict__ tw)
+template <typename R, int RAD, int DIR, int VPT>
+__device__ __forceinline__ void reg_stage(C<R> (&x)[VPT], const C<R> *lin, C<R> *lout, int n, int T, int t,
+                                          int Ns, const typename TwT<R>::T *__restrict__ tw)
 {
-    constexpr int NB = 8 / RAD;
-    C<R> x[8];
-    const int nR = n / RAD;
+    if constexpr (RAD <= VPT) {
+        constexpr int NB = VPT / RAD;
+        const int nR = n / RAD;
+        const int stride = n / (Ns * RAD);
+        typename TwT<R>::T w[VPT];
+        if (Ns > 1) {  // twiddles first: their latency overlaps the LDS reads
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int j = t + b * T;
+            for (int b = 0; b < NB; ++b) {
+                const int k = (t + b * T) & (Ns - 1);
 #pragma unroll
-        for (int r = 0; r < RAD; ++r) x[b * RAD + r] = line[j + r * nR];
-    }
-    __syncthreads();
-    const int stride = n / (Ns * RAD);
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int j = t + b * T;
-        const int k = j & (Ns - 1);
-        C<R> *xb = x + b * RAD;
-        if (Ns > 1) {
-#pragma unroll
-            for (int r = 1; r < RAD; ++r) xb[r] = twmul<R, DIR>(xb[r], tw[r * k * stride]);
+                for (int r = 1; r < RAD; ++r) w[b * RAD + r] = tw[r * k * stride];
+            }
         }
-        dftR<R, RAD, DIR>(xb);
-        const int o = (j - k) * RAD + k;
+        C<R> y[VPT];
 #pragma unroll
-        for (int r = 0; r < RAD; ++r) line[o + r * Ns] = xb[r];
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int r = 0; r < RAD; ++r)
+                y[b * RAD + r] = lin ? lin[t + b * T + r * nR] : x[b + r * NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            C<R> *yb = y + b * RAD;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < RAD; ++r) yb[r] = twmul<R, DIR>(yb[r], w[b * RAD + r]);
+            }
+            dftR<R, RAD, DIR>(yb);
+        }
+        if (lout) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int j = t + b * T, k = j & (Ns - 1);
+                const int o = (j - k) * RAD + k;
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) lout[o + r * Ns] = y[b * RAD + r];
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < RAD; ++r) x[b + r * NB] = y[b * RAD + r];
+        }
     }
-    __syncthreads();
 }
 
-// Full transform of one line (n = 2^m >= 8): radix-8 stages, then a final
-// radix-4 or radix-2 stage for the remainder.  Every thread of the block
-// calls it (barriers inside).
-template <typename R, int DIR>
-__device__ __forceinline__ void fft_line(C<R> *line, int n, int log2n, int T, int t,
-                                         const typename TwT<R>::T *__restrict__ tw)
+// Full transform (n = 2^m >= 8) of the line whose two LDS buffers are
+// bufA / bufB.  Eight values per thread: radix-8 stages, then radix 4 or 2
+// for the remainder; four values per thread (twice the threads, for grids
+// too small to fill the chip otherwise): radix-4 stages, then radix 2 if
+// log2 n is odd.  Every thread of the block calls it (barriers inside).
+template <typename R, int DIR, int VPT>
+__device__ __forceinline__ void fft_line(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int &wc, int n, int log2n,
+                                         int T, int t, const typename TwT<R>::T *__restrict__ tw)
 {
-    int Ns = 1;
-    const int n8 = log2n / 3, rem = log2n - 3 * n8;
-    for (int s = 0; s < n8; ++s) {
-        stockham_stage<R, 8, DIR>(line, n, T, t, Ns, tw);
-        Ns *= 8;
+    int rad[12], nst = 0;
+    if (VPT == 8) {
+        for (int s = 0; s < log2n / 3; ++s) rad[nst++] = 8;
+        if (log2n % 3 == 2) rad[nst++] = 4;
+        else if (log2n % 3 == 1) rad[nst++] = 2;
+    } else {
+        for (int s = 0; s < log2n / 2; ++s) rad[nst++] = 4;
+        if (log2n & 1) rad[nst++] = 2;
     }
-    if (rem == 2) stockham_stage<R, 4, DIR>(line, n, T, t, Ns, tw);
-    else if (rem == 1) stockham_stage<R, 2, DIR>(line, n, T, t, Ns, tw);
+    const C<R> *lin = nullptr;
+    int Ns = 1;
+    for (int s = 0; s < nst; ++s) {
+        const bool last = (s == nst - 1);
+        C<R> *lout = last ? nullptr : ((wc & 1) ? bufB : bufA);
+        if (rad[s] == 8) reg_stage<R, 8, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
+        else if (rad[s] == 4) reg_stage<R, 4, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
+        else reg_stage<R, 2, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
+        if (!last) {
+            lds_barrier();
+            lin = lout;
+            ++wc;
+        }
+        Ns *= rad[s];
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -232,8 +291,11 @@ __device__ __forceinline__ double coord_p(const Phys &P, int j, int i)
 {
     return (i < (P.n[j] >> 1)) ? i * P.dp[j] : (i - P.n[j]) * P.dp[j];
 }
-// V(r) and dV/du (u = r^2) for the named potentials (src/meshgrid.c:140-153)
-__device__ __forceinline__ double potential(const Phys &P, double rr, double &dVdu)
+// V(r) and dV/du (u = r^2) for the named potentials (src/meshgrid.c:140-153).
+// complex128 evaluates them in FP64 like the reference; complex64 in FP32
+// (a relative change of V of a few 1e-7, below the FP32 state rounding).
+template <typename R> __device__ double potential(const Phys &P, double rr, double &dVdu);
+template <> __device__ __forceinline__ double potential<double>(const Phys &P, double rr, double &dVdu)
 {
     double rad = sqrt(rr);
     double u = rad * rad;
@@ -246,6 +308,19 @@ __device__ __forceinline__ double potential(const Phys &P, double rr, double &dV
     double g = 2.7459 * exp(-.5 * rad * rad / 1.96);
     dVdu = g * .5 / 1.96;
     return -g;
+}
+template <> __device__ __forceinline__ double potential<float>(const Phys &P, double rr, double &dVdu)
+{
+    const float u = (float)rr;
+    if (P.pot_kind == 0) {
+        const float id = rsqrtf(u + (float)P.pot_c2);
+        const float g = (P.pot_b != 0.0) ? (float)P.pot_b * __expf(-u) : 0.0f;
+        dVdu = (double)(g * id + .5f * (1.f + g) * (id * id * id));
+        return (double)(-(1.f + g) * id);
+    }
+    const float g = 2.7459f * __expf(u * (-.5f / 1.96f));
+    dVdu = (double)(g * (.5f / 1.96f));
+    return (double)(-g);
 }
 
 #define QND_MAX_EXP_ARG 700.0     // src/integrators.h:12
@@ -267,23 +342,77 @@ template <> __device__ __forceinline__ Fac<double> phase_factor<double>(double a
     f.v = mk<double>(e * c, -e * s);
     return f;
 }
+// sin r and cos r for |r| <= pi/4 (+ rounding), FP32 Taylor polynomials to
+// r^9 / r^10: truncation below 3e-8, i.e. within the FP32 rounding.
+__device__ __forceinline__ void sincos_poly_f(float r, float &s, float &c)
+{
+    const float r2 = r * r;
+    float ps = fmaf(r2, 2.75573192e-6f, -1.98412698e-4f);   //  1/9!, -1/7!
+    ps = fmaf(r2, ps, 8.33333333e-3f);                      //  1/5!
+    ps = fmaf(r2, ps, -1.66666667e-1f);                     // -1/3!
+    s = fmaf(r * r2, ps, r);
+    float pc = fmaf(r2, -2.75573192e-7f, 2.48015873e-5f);   // -1/10!, 1/8!
+    pc = fmaf(r2, pc, -1.38888889e-3f);                     // -1/6!
+    pc = fmaf(r2, pc, 4.16666667e-2f);                      //  1/4!
+    pc = fmaf(r2, pc, -0.5f);
+    c = fmaf(r2, pc, 1.0f);
+}
+// 2^y for an FP64 y: integer part exact, fraction through the FP32 exp2
+__device__ __forceinline__ float exp2_split(double y)
+{
+    const double n = floor(y);
+    return ldexpf(exp2f((float)(y - n)), (int)n);
+}
+__device__ __forceinline__ float exp_f32(double arg)
+{
+    return exp2_split(arg * 1.4426950408889634073599246810019);
+}
+// complex64: the angle is reduced in FP64 to an octant (|r| <= pi/4), sin
+// and cos come from FP32 polynomials, and the FP32 pair is corrected on its
+// modulus: e = c^2 + s^2 - 1 is formed exactly (TwoProduct of both squares,
+// Fast2Sum of c^2 - 1, Sterbenz for the rest) and lo = -e/2 * hi makes
+// |hi + lo| = 1 to ~1e-14.  A residual error of the angle (~1e-7 rad) acts
+// like a relative change of the Hamiltonian of that size; an error of the
+// modulus would instead change the norm at every step.
 template <> __device__ __forceinline__ Fac<float> phase_factor<float>(double arg, double ph)
 {
-    const double twopi = 6.283185307179586476925286766559;
-    const double red = ph - twopi * rint(ph * (1.0 / twopi));
-    double s, c;
-    sincos(red, &s, &c);
-    const double e = (arg == 0.0) ? 1.0 : exp(arg > QND_MAX_EXP_ARG_F32 ? QND_MAX_EXP_ARG_F32 : arg);
-    const double fr = e * c, fi = -e * s;
+#pragma clang fp contract(off)  // the error-free products and sums below must not be fused
+    const double j = rint(ph * 0.63661977236758134307553505349006);          // 2/pi
+    double rd = fma(-j, 1.5707963267948965579989817342721, ph);               // pi/2 hi
+    rd = fma(-j, 6.1232339957367660359e-17, rd);                             // pi/2 lo
+    float sr, cr;
+    sincos_poly_f((float)rd, sr, cr);
+    const float pc = cr * cr, ec = fmaf(cr, cr, -pc);
+    const float ps = sr * sr, es = fmaf(sr, sr, -ps);
+    const float t = pc - 1.0f, terr = pc - (t + 1.0f);
+    const float e = ((t + ps) + terr) + (ec + es);
+    const int q = (int)j & 3;
+    float c, s;
+    if (q == 0) { c = cr; s = sr; }
+    else if (q == 1) { c = -sr; s = cr; }
+    else if (q == 2) { c = -cr; s = -sr; }
+    else { c = sr; s = -cr; }
     Fac<float> f;
-    f.hi = mk<float>((float)fr, (float)fi);
-    f.lo = mk<float>((float)(fr - (double)f.hi.x), (float)(fi - (double)f.hi.y));
+    const float he = -0.5f * e;
+    if (arg == 0.0) {
+        f.hi = mk<float>(c, -s);
+        f.lo = mk<float>(he * c, -(he * s));
+    } else {
+        const float g = exp_f32(arg > QND_MAX_EXP_ARG_F32 ? QND_MAX_EXP_ARG_F32 : arg);
+        const float hx = g * c, hy = -(g * s);
+        f.hi = mk<float>(hx, hy);
+        f.lo = mk<float>(fmaf(g, c, -hx) + he * hx, -fmaf(g, s, hy) + he * hy);
+    }
     return f;
 }
-template <typename R> __device__ __forceinline__ R real_exp(double arg)
+template <typename R> __device__ __forceinline__ R real_exp(double arg);
+template <> __device__ __forceinline__ double real_exp<double>(double arg)
 {
-    const double cl = Prec<R>::clamp;
-    return (R)exp(arg > cl ? cl : arg);
+    return exp(arg > QND_MAX_EXP_ARG ? QND_MAX_EXP_ARG : arg);
+}
+template <> __device__ __forceinline__ float real_exp<float>(double arg)
+{
+    return exp_f32(arg > QND_MAX_EXP_ARG_F32 ? QND_MAX_EXP_ARG_F32 : arg);
 }
 
 // ---------------------------------------------------------------------
@@ -306,13 +435,34 @@ __device__ __forceinline__ double block_sum(double v, double *red)
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nw = (blockDim.x + 63) >> 6;
-    __syncthreads();
+    lds_barrier();
     if (lane == 0) red[w] = v;
-    __syncthreads();
+    lds_barrier();
     double s = 0.0;
     if (threadIdx.x == 0)
         for (int i = 0; i < nw; ++i) s += red[i];
     return s;
+}
+// Reduce K values over the block in one exchange and write the K sums to
+// out[0..K) (thread 0).  red = LDS scratch of at least 16*K doubles.  All
+// threads must call.
+template <int K> __device__ __forceinline__ void block_sum_vec(double *v, double *red, double *out)
+{
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = wave_sum(v[q]);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    lds_barrier();
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) red[q * 16 + w] = v[q];
+    }
+    lds_barrier();
+    for (int q = threadIdx.x; q < K; q += blockDim.x) {  // blocks may be smaller than K
+        double s = 0.0;
+        for (int i = 0; i < nw; ++i) s += red[q * 16 + i];
+        out[q] = s;
+    }
 }
 // Sum column q of a [nwg][npart] partials array with the threads of one
 // wavefront (lane-strided, then butterfly): deterministic for a given nwg.
@@ -435,12 +585,12 @@ __device__ __forceinline__ void obs_finalize(const double *part, int nwg, int di
 // the line-pass kernel
 // ---------------------------------------------------------------------
 #define QND_PASS_THREADS 512   // workgroup cap: leaves 256 VGPRs per lane
-template <typename R, int KIND, int DIR1>
+template <typename R, int KIND, int DIR1, int VPT>
 __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ double red[16];
-    C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);
+    __shared__ double red[16 * QND_NOBS];
+    C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
     constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D);
 
     const int tid = threadIdx.x;
@@ -451,73 +601,83 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
     if (KIND == PK_KIN && (fl & PF_FINALIZE) && blockIdx.x == 0)
         obs_finalize(a.fin_part, a.fin_nwg, P.dim, P.dNr, a.fin_spec, a.fin_rec);
 
-    // ---- tile geometry: TW lines, full tiles (TW | S or S == 1) ----
-    const int64_t line0 = (int64_t)blockIdx.x * TW;
+    // ---- tile geometry: TW lines (TW | S or S == 1).  Contiguous lines put
+    // t on the fastest lanes, strided lines the line index l, so that every
+    // wavefront access covers consecutive addresses. ----
     const bool contig = (a.S == 1);
     int log2S = 0;
     while (((int64_t)1 << log2S) < a.S) ++log2S;
-    int64_t base;
-    if (contig) base = line0 << a.log2n;
-    else {
-        const int64_t o = line0 >> log2S;
-        base = ((o << a.log2n) << log2S) + (line0 & (a.S - 1));
-    }
-    const int tot = TW << a.log2n;
+    int l, t;
+    if (contig) { l = tid >> a.log2T; t = tid & (T - 1); }
+    else { l = tid & (TW - 1); t = tid >> a.log2TW; }
+    const int64_t L = (int64_t)blockIdx.x * TW + l;  // this thread's line
+    int64_t lbase, estride;                          // element i at lbase + i * estride
+    if (contig) { lbase = L << a.log2n; estride = 1; }
+    else { lbase = (((L >> log2S) << a.log2n) << log2S) + (L & (a.S - 1)); estride = a.S; }
+    C<R> *bufA = lds + l * ldsS;
+    C<R> *bufB = lds + (TW + l) * ldsS;
+    int wc = 0;
 
-    // ---- load (with the Gram-Schmidt update prologue) ----
-    double gsinv = 1.0;
-    if (POTK && (fl & PF_GS_UPDATE)) gsinv = a.gs_coef[2 * a.nprev];
-    for (int q = tid; q < tot; q += blockDim.x) {
-        int l, i;
-        int64_t g;
-        if (contig) { l = q >> a.log2n; i = q & (n - 1); g = base + q; }
-        else { i = q >> a.log2TW; l = q & (TW - 1); g = base + ((int64_t)i << log2S) + l; }
-        C<R> v = a.data[g];
-        if (POTK && (fl & PF_GS_UPDATE)) {
-            double vr = (double)v.x, vi = (double)v.y;
-            for (int j = 0; j < a.nprev; ++j) {
-                const C<R> pj = a.prev[(int64_t)j * a.slot_stride + g];
-                const double cr = a.gs_coef[2 * j], ci = a.gs_coef[2 * j + 1];
-                vr -= cr * (double)pj.x - ci * (double)pj.y;
-                vi -= cr * (double)pj.y + ci * (double)pj.x;
+    // ---- load x[m] = element t + m T (with the Gram-Schmidt update prologue) ----
+    C<R> x[VPT];
+#pragma unroll
+    for (int m = 0; m < VPT; ++m) x[m] = a.data[lbase + (int64_t)(t + m * T) * estride];
+    if (POTK && (fl & PF_GS_UPDATE)) {
+        const double gsinv = a.gs_coef[2 * a.nprev];
+        double vr[VPT], vi[VPT];
+#pragma unroll
+        for (int m = 0; m < VPT; ++m) { vr[m] = (double)x[m].x; vi[m] = (double)x[m].y; }
+        for (int j = 0; j < a.nprev; ++j) {
+            const double cr = a.gs_coef[2 * j], ci = a.gs_coef[2 * j + 1];
+            const C<R> *pj = a.prev + (int64_t)j * a.slot_stride;
+            C<R> p[VPT];
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) p[m] = pj[lbase + (int64_t)(t + m * T) * estride];
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                vr[m] -= cr * (double)p[m].x - ci * (double)p[m].y;
+                vi[m] -= cr * (double)p[m].y + ci * (double)p[m].x;
             }
-            v = mk<R>((R)(vr * gsinv), (R)(vi * gsinv));
         }
-        lds[l * ldsS + i] = v;
+#pragma unroll
+        for (int m = 0; m < VPT; ++m) x[m] = mk<R>((R)(vr[m] * gsinv), (R)(vi[m] * gsinv));
     }
-    __syncthreads();
 
-    // ---- thread <-> (line, t) ----
-    const int l = tid >> a.log2T, t = tid & (T - 1);
-    C<R> *line = lds + l * ldsS;
     const int ax = a.axis;
-
     if (KIND == PK_PLAIN) {
-        fft_line<R, DIR1>(line, n, a.log2n, T, t, a.tw);
+        fft_line<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw);
         if (a.scale1 != 1.0) {
             const R s1 = (R)a.scale1;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) line[t + e * T] = scl<R>(line[t + e * T], s1);
+            for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], s1);
         }
-        __syncthreads();
     } else {
         int idx[QND_MAXD];
-        line_index(P, ax, a.S, log2S, line0 + l, idx);
+        line_index(P, ax, a.S, log2S, L, idx);
+        // per-axis CAP and mask values of this thread's elements, requested
+        // before the transform so that the loads overlap it (both tables
+        // always exist)
+        double capv[VPT], mskv[VPT];
+        if (POTK) {
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                capv[m] = P.capt[P.off[ax] + t + m * T];
+                mskv[m] = P.maskt[P.off[ax] + t + m * T];
+            }
+        }
 
         if (KIND == PK_POT1D && (fl & PF_KPRE)) {
-#pragma unroll 1
-            for (int e = 0; e < 8; ++e) {
-                const int i = t + e * T;
-                const double p = coord_p(P, ax, i);
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                const double p = coord_p(P, ax, t + m * T);
                 const double K = .5 * p * p;
-                if (a.kmode == KM_IMAG) line[i] = scl<R>(line[i], real_exp<R>(-a.ck_pre * K));
-                else line[i] = fac_mul<R>(line[i], phase_factor<R>(0.0, a.ck_pre * K));
+                if (a.kmode == KM_IMAG) x[m] = scl<R>(x[m], real_exp<R>(-a.ck_pre * K));
+                else x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, a.ck_pre * K));
             }
-            __syncthreads();
         }
 
         // ---- first transform ----
-        fft_line<R, DIR1>(line, n, a.log2n, T, t, a.tw);
+        fft_line<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw);
 
         if (KIND == PK_KIN) {
             // all axes in k-space: kinetic factor (reference order of the sums)
@@ -531,17 +691,15 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                 }
             }
             const double Aax = a.Ak[ax];
-#pragma unroll 1
-            for (int e = 0; e < 8; ++e) {
-                const int i = t + e * T;
-                const double p = coord_p(P, ax, i);
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                const double p = coord_p(P, ax, t + m * T);
                 const double K = Ko + .5 * p * p;
-                if (a.kmode == KM_IMAG) line[i] = scl<R>(line[i], real_exp<R>(-a.ck_mid * K));
+                if (a.kmode == KM_IMAG) x[m] = scl<R>(x[m], real_exp<R>(-a.ck_mid * K));
                 else if (a.kmode == KM_VELOCITY)
-                    line[i] = fac_mul<R>(line[i], phase_factor<R>(0.0, a.ck_mid * (K + (Wk + (p + .5 * Aax) * Aax))));
-                else line[i] = fac_mul<R>(line[i], phase_factor<R>(0.0, a.ck_mid * K));
+                    x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, a.ck_mid * (K + (Wk + (p + .5 * Aax) * Aax))));
+                else x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, a.ck_mid * K));
             }
-            __syncthreads();
         } else {
             // all axes in r-space: potential, CAP, field, mask, observables
             double rro = 0.0, Vco = 0.0, Wv = 0.0, mko = 1.0;
@@ -562,28 +720,27 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
             for (int q = 0; q < QND_NOBS; ++q) obs[q] = 0.0;
             const double s1 = a.scale1;
             const bool need_v = (fl & (PF_V1 | PF_V2 | PF_OBS)) != 0;
-#pragma unroll 1
-            for (int e = 0; e < 8; ++e) {
-                const int i = t + e * T;
-                C<R> v = scl<R>(line[i], (R)s1);
-                const double x = coord_x(P, ax, i);
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                C<R> v = scl<R>(x[m], (R)s1);
+                const double xc = coord_x(P, ax, t + m * T);
                 double dVdu = 0.0, V = 0.0;
-                if (need_v) V = potential(P, x * x + rro, dVdu);
-                const double Vc = P.cap_on ? (P.capt[P.off[ax] + i] + Vco) : 0.0;
+                if (need_v) V = potential<R>(P, xc * xc + rro, dVdu);
+                const double Vc = P.cap_on ? (capv[m] + Vco) : 0.0;
                 if (fl & PF_V1) {
                     if (a.vmode == VM_IMAG) v = scl<R>(v, real_exp<R>(-a.cv1 * V));
                     else if (a.vmode == VM_LENGTH)
-                        v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, a.cv1 * (V + (x * a.Ev[ax] + Wv))));
+                        v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, a.cv1 * (V + (xc * a.Ev[ax] + Wv))));
                     else v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, a.cv1 * V));
                 }
-                if ((fl & PF_MASK) && P.mask_on) v = scl<R>(v, (R)(P.maskt[P.off[ax] + i] * mko));
+                if ((fl & PF_MASK) && P.mask_on) v = scl<R>(v, (R)(mskv[m] * mko));
                 if (fl & PF_OBS) {
                     const double d = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
                     obs[0] += d;
 #pragma unroll
                     for (int j = 0; j < QND_MAXD; ++j) {
                         if (j < P.dim) {
-                            const double xj = (j == ax) ? x : xo[j];
+                            const double xj = (j == ax) ? xc : xo[j];
                             obs[1 + j] += d * xj;
                             obs[1 + QND_MAXD + j] += d * dVdu * 2.0 * xj;
                         }
@@ -593,75 +750,57 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                     if (a.vmode == VM_IMAG) v = scl<R>(v, real_exp<R>(-a.cv2 * V));
                     else v = fac_mul<R>(v, phase_factor<R>(-a.cv2 * Vc, a.cv2 * V));
                 }
-                line[i] = v;
+                x[m] = v;
             }
-            if (fl & PF_OBS) {
-#pragma unroll
-                for (int q = 0; q < QND_NOBS; ++q) {
-                    const double s = block_sum(obs[q], red);
-                    if (tid == 0) a.partials[(int64_t)blockIdx.x * a.npart + q] = s;
-                }
-            }
-            __syncthreads();
+            if (fl & PF_OBS) block_sum_vec<QND_NOBS>(obs, red, a.partials + (int64_t)blockIdx.x * a.npart);
         }
 
         // ---- second transform ----
         if (KIND == PK_KIN || (fl & PF_SECOND)) {
-            fft_line<R, -DIR1>(line, n, a.log2n, T, t, a.tw);
+            fft_line<R, -DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw);
             const R s2 = (R)a.scale2;
             if (KIND == PK_POT1D && (fl & PF_KPOST)) {
-#pragma unroll 1
-                for (int e = 0; e < 8; ++e) {
-                    const int i = t + e * T;
-                    const double p = coord_p(P, ax, i);
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) {
+                    const double p = coord_p(P, ax, t + m * T);
                     const double K = .5 * p * p;
-                    const C<R> v = scl<R>(line[i], s2);
-                    if (a.kmode == KM_IMAG) line[i] = scl<R>(v, real_exp<R>(-a.ck_post * K));
+                    const C<R> v = scl<R>(x[m], s2);
+                    if (a.kmode == KM_IMAG) x[m] = scl<R>(v, real_exp<R>(-a.ck_post * K));
                     else if (a.kmode == KM_VELOCITY)
-                        line[i] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * (K + (p + .5 * a.Ak[ax]) * a.Ak[ax])));
-                    else line[i] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * K));
+                        x[m] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * (K + (p + .5 * a.Ak[ax]) * a.Ak[ax])));
+                    else x[m] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * K));
                 }
             } else if (s2 != (R)1) {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) line[t + e * T] = scl<R>(line[t + e * T], s2);
+                for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], s2);
             }
-            __syncthreads();
         }
     }
 
     // ---- store ----
-    for (int q = tid; q < tot; q += blockDim.x) {
-        int ll, i;
-        int64_t g;
-        if (contig) { ll = q >> a.log2n; i = q & (n - 1); g = base + q; }
-        else { i = q >> a.log2TW; ll = q & (TW - 1); g = base + ((int64_t)i << log2S) + ll; }
-        a.data[g] = lds[ll * ldsS + i];
-    }
+#pragma unroll
+    for (int m = 0; m < VPT; ++m) a.data[lbase + (int64_t)(t + m * T) * estride] = x[m];
 
     // ---- Gram-Schmidt dots on the stored representation ----
     if (POTK && (fl & PF_GS_DOTS)) {
         for (int j = 0; j <= a.nprev; ++j) {
             double sr = 0.0, si = 0.0;
-            for (int q = tid; q < tot; q += blockDim.x) {
-                int ll, i;
-                int64_t g;
-                if (contig) { ll = q >> a.log2n; i = q & (n - 1); g = base + q; }
-                else { i = q >> a.log2TW; ll = q & (TW - 1); g = base + ((int64_t)i << log2S) + ll; }
-                const C<R> v = lds[ll * ldsS + i];
-                if (j < a.nprev) {
-                    const C<R> pj = a.prev[(int64_t)j * a.slot_stride + g];
-                    sr += (double)pj.x * (double)v.x + (double)pj.y * (double)v.y;
-                    si += (double)pj.x * (double)v.y - (double)pj.y * (double)v.x;
-                } else {
-                    sr += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+            if (j < a.nprev) {
+                const C<R> *pj = a.prev + (int64_t)j * a.slot_stride;
+                C<R> p[VPT];
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) p[m] = pj[lbase + (int64_t)(t + m * T) * estride];
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) {
+                    sr += (double)p[m].x * (double)x[m].x + (double)p[m].y * (double)x[m].y;
+                    si += (double)p[m].x * (double)x[m].y - (double)p[m].y * (double)x[m].x;
                 }
+            } else {
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) sr += (double)x[m].x * (double)x[m].x + (double)x[m].y * (double)x[m].y;
             }
-            const double tr = block_sum(sr, red);
-            const double ti = block_sum(si, red);
-            if (tid == 0) {
-                a.partials[(int64_t)blockIdx.x * a.npart + 2 * j] = tr;
-                a.partials[(int64_t)blockIdx.x * a.npart + 2 * j + 1] = ti;
-            }
+            double s2v[2] = {sr, si};
+            block_sum_vec<2>(s2v, red, a.partials + (int64_t)blockIdx.x * a.npart + 2 * j);
         }
     }
 }
@@ -703,7 +842,7 @@ __global__ void __launch_bounds__(256) k_gs_coeffs(const double *__restrict__ pa
         const double s = wave_sum_column(part, nwg, npart, q);
         if ((threadIdx.x & 63) == 0) b[q] = s * scale;
     }
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
         const double nv = b[2 * nprev];
         for (int j = 0; j < nprev; ++j) {
@@ -873,7 +1012,7 @@ __global__ void __launch_bounds__(256) k_red(int mode, const C<R> *a, C<R> *b, c
             }
             if (mode == RM_OBS) {
                 double dVdu;
-                (void)potential(P, rr, dVdu);
+                (void)potential<double>(P, rr, dVdu);
                 const double d = ar * ar + ai * ai;
                 acc[0] += d;
 #pragma unroll
@@ -886,7 +1025,7 @@ __global__ void __launch_bounds__(256) k_red(int mode, const C<R> *a, C<R> *b, c
                 }
             } else {  // RM_HCOMB
                 double dVdu;
-                const double V = potential(P, rr, dVdu);
+                const double V = potential<double>(P, rr, dVdu);
                 const C<R> hk = c[f], hb = b[f];
                 const double hr = (double)hk.x * fs + V * (double)hb.x;
                 const double hi = (double)hk.y * fs + V * (double)hb.y;
@@ -936,7 +1075,7 @@ __global__ void __launch_bounds__(256) k_ped(const C<R> *psi, int64_t N, Phys P,
         dens[q] = inside ? ((double)v.x * (double)v.x + (double)v.y * (double)v.y) : 0.0;
         kin[q] = K;
     }
-    __syncthreads();
+    lds_barrier();
     const double inv2s2 = 1.0 / (sigma * sigma);
     for (int j = threadIdx.x; j < Ne; j += blockDim.x) {
         double s = 0.0;
