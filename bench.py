@@ -115,9 +115,9 @@ def cpu_baseline(psi0, Et, At, Eo, Ao, budget_s=15.0):
                             Ao[done:done + n], names=["dipole_r", "dipole_a", "norm"])
         t_tot += time.perf_counter() - t0
         done += n
-        if t_tot > budget_s or done >= 200:
+        if t_tot > budget_s or done >= 2000:
             break
-        n = max(1, min(200 - done, int(n * 2)))
+        n = max(1, min(2000 - done, int(n * 2)))
     return done, t_tot, threads
 
 
@@ -129,7 +129,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=200)
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for counter runs: a 256 MiB device copy (calibration), then 20 c64 steps; no JSON")
     ap.add_argument("--dump", default="", help="write the observable traces to this .npz")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -146,6 +148,18 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     N = int(np.prod(NR))
+    if args.traffic_probe:
+        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")   # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        mode = F.Mode(intensity=1e14, wavelength=800.0, ellipticity=0.0, cep=0.0, envelope="cos2", cycles=4)
+        Et, At, Eo, Ao = F.tables([mode], "BM4", DT, 100, 2)
+        psi0 = reference_guess(NR, DR)
+        real_run(q, F, "c64", psi0 / np.linalg.norm(psi0), 0, 20, Et, At, Eo, Ao)
+        log("traffic probe done: 3 copies of 256 MiB, 20 c64 steps")
+        return
 
     log("ground state (imaginary time, FP64) ...")
     psi0, E0, n_imag, t_imag = ground_state(q)

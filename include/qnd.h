@@ -132,6 +132,24 @@ int qnd_real_steps(qnd_ctx *ctx, int slot, int gauge, int64_t step0,
                    int64_t nsteps, const int *codes, int ncodes,
                    double *obs_out);
 
+/* Batched real time (no reference counterpart: the reference runs one
+ * field per process; this serves CEP / intensity sweeps).  Instance b of a
+ * batch is slot slot0+b, propagated under its own field:
+ * Et, At [nb][ntime][s][dim] at the stage times and Eobs [nb][ntime][dim] at
+ * t_n, each instance's rows in the layout of qnd_set_field.  Copied to the
+ * device.  nb must not exceed n_slots. */
+int qnd_set_field_batch(qnd_ctx *ctx, int nb, int64_t ntime, const double *Et,
+                        const double *At, const double *Eobs);
+
+/* qnd_real_steps for the nb slots slot0 .. slot0+nb-1 at once: every pass of
+ * a step is one launch over all instances (the batch is an extra,
+ * untransformed leading axis), each instance reading its own field rows.
+ * Recordable codes: dipole_r, dipole_a, norm.  obs_out: [nb][nsteps][width].
+ * With nb == 1 this is qnd_real_steps on the qnd_set_field tables. */
+int qnd_real_steps_batch(qnd_ctx *ctx, int slot0, int nb, int gauge,
+                         int64_t step0, int64_t nsteps, const int *codes,
+                         int ncodes, double *obs_out);
+
 /* Write_final_observables (src/io_functions.c:136-287) numerics: radial
  * sin^2 mask (R_mask), FFT to unitary p-space; PED[Ne] (sigma = de) and the
  * masked p-space state psi_p (host, interleaved, native FFT order) from

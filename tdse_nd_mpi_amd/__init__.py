@@ -27,7 +27,8 @@ EXPORTS = ("qnd_create", "qnd_destroy", "qnd_last_error", "qnd_set_potential", "
            "qnd_set_mask", "qnd_upload_state", "qnd_download_state", "qnd_imag_steps",
            "qnd_gram_schmidt", "qnd_transform", "qnd_energy_moments", "qnd_observables", "qnd_set_field",
            "qnd_real_steps", "qnd_final_observables", "qnd_event_record", "qnd_event_elapsed",
-           "qnd_synchronize", "qnd_profile_enable", "qnd_profile_read", "qnd_version")
+           "qnd_synchronize", "qnd_profile_enable", "qnd_profile_read", "qnd_version",
+           "qnd_set_field_batch", "qnd_real_steps_batch")
 
 _lib = None
 
@@ -66,6 +67,8 @@ def library():
     L.qnd_observables.argtypes = [vp, C.c_int, ip, C.c_int, dp, dp, dp]
     L.qnd_set_field.argtypes = [vp, C.c_int64, dp, dp, dp, dp]
     L.qnd_real_steps.argtypes = [vp, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
+    L.qnd_set_field_batch.argtypes = [vp, C.c_int, C.c_int64, dp, dp, dp]
+    L.qnd_real_steps_batch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
     L.qnd_final_observables.argtypes = [vp, C.c_int, dp, dp, C.c_int, C.c_double, dp, dp]
     L.qnd_event_record.argtypes = [vp, C.c_int]
     L.qnd_event_elapsed.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
@@ -204,6 +207,27 @@ class Grid:
             codes.ctypes.data_as(C.POINTER(C.c_int)) if len(codes) else None, len(codes),
             _d(out) if width else None))
         return out[:nsteps, :width]
+
+    def set_field_batch(self, Et, At, Eobs=None):
+        """Per-instance field tables: Et, At [nb][ntime][s][dim], Eobs [nb][ntime][dim]."""
+        Et = np.ascontiguousarray(Et, dtype=np.float64)
+        At = np.ascontiguousarray(At, dtype=np.float64)
+        nb, nt = Et.shape[0], Et.shape[1]
+        Eo = np.zeros((nb, nt, self.dim)) if Eobs is None else np.ascontiguousarray(Eobs, dtype=np.float64)
+        self._ck(library().qnd_set_field_batch(self.h, nb, nt, _d(Et), _d(At), _d(Eo)))
+        self.batch = nb
+
+    def real_steps_batch(self, slot0, nb, gauge, step0, nsteps, names=()):
+        """real_steps for slots slot0..slot0+nb-1 in one launch per pass, each
+        under its own field; returns [nb][nsteps][width]."""
+        codes = np.array([OBS[n] for n in names], dtype=np.int32)
+        width = sum({5: 1, 3: self.dim, 6: self.dim}.get(int(c), 0) for c in codes)
+        out = np.zeros((nb, max(nsteps, 1), max(width, 1)))
+        self._ck(library().qnd_real_steps_batch(
+            self.h, slot0, nb, GAUGES[gauge], int(step0), int(nsteps),
+            codes.ctypes.data_as(C.POINTER(C.c_int)) if len(codes) else None, len(codes),
+            _d(out) if width else None))
+        return out[:, :nsteps, :width]
 
     def final_observables(self, slot, R_mask, pr_max, Ne, de):
         """Write_final_observables numerics (src/io_functions.c:136-287)."""

@@ -13,6 +13,7 @@
 #include "../../include/qnd.h"
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -76,6 +77,9 @@ struct qnd_ctx {
     AxisPlan plan[QND_MAXD];
     std::vector<double> Et, At, Eobs, Aobs;
     int64_t ntime = 0;
+    double *d_bEt = nullptr, *d_bAt = nullptr, *d_bEo = nullptr;  // batch field tables (device)
+    int bnb = 0;
+    int64_t bntime = 0;
     hipEvent_t ev[16] = {};
     bool prof = false;
     int64_t prof_count[PC_N] = {0};
@@ -183,20 +187,30 @@ int pow2floor(int64_t x)
     return p;
 }
 
-Geo geometry(qnd_ctx *c, int axis)
+// Launch geometry of a pass along `axis` over nb consecutive slots (a batch
+// of instances: an extra, untransformed leading axis).
+Geo geometry(qnd_ctx *c, int axis, int nb = 1)
 {
     Geo g{};
     g.axis = axis;
     g.n = (int)c->nr[axis];
-    g.log2n = ilog2(g.n);
     g.S = 1;
     for (int j = axis + 1; j < c->dim; ++j) g.S *= c->nr[j];
-    g.nlines = c->N / g.n;
-    // values per thread: 8 (radix-8 stages) unless the grid is too small to
-    // give every SIMD a wavefront that way, then 4 (twice the wavefronts)
-    g.vpt = 8;
-    const bool few_waves = c->N / 8 / 64 < 4096;
-    if ((c->vpt_override == 4 || (c->vpt_override == 0 && few_waves)) && g.n / 4 <= QND_PASS_THREADS) g.vpt = 4;
+    const int64_t lpi = c->N / g.n;  // lines per instance
+    g.nlines = lpi * nb;
+    if (g.n % 3 == 0) {
+        // n = 3 * 2^k: twelve values per thread (radix 4/2 stages, radix 3 last)
+        g.vpt = 12;
+        g.log2n = ilog2(g.n / 3);
+    } else {
+        // values per thread: 8 (radix-8 stages) unless the grid is too small
+        // to give every SIMD a wavefront that way, then 4 (twice the wavefronts)
+        g.log2n = ilog2(g.n);
+        g.vpt = 8;
+        const bool few_waves = g.nlines * g.n / 8 / 64 < 4096;
+        if ((c->vpt_override == 4 || (c->vpt_override == 0 && few_waves)) && g.n / 4 <= QND_PASS_THREADS)
+            g.vpt = 4;
+    }
     g.T = g.n / g.vpt;
     const size_t line_bytes = 2 * (size_t)g.n * c->esz;  // two exchange buffers per line
     int twmax_lds = (int)(c->lds_max / line_bytes);
@@ -222,6 +236,7 @@ Geo geometry(qnd_ctx *c, int axis)
     if ((int64_t)tw > g.nlines) tw = (int)g.nlines;
     if (g.S > 1 && (int64_t)tw > g.S) tw = (int)g.S;
     if (tw < 1) tw = 1;
+    while (tw > 1 && lpi % tw != 0) tw /= 2;  // a workgroup never straddles two instances
     // one padding element per line spreads the strided tile rows over the banks
     while (tw > 1 && 2 * (size_t)tw * (g.n + 1) * c->esz > c->lds_max) tw /= 2;
     g.TW = tw;
@@ -254,15 +269,35 @@ template <typename R> PassArgs<R> base_args(qnd_ctx *c, C<R> *data, const Geo &g
     a.scale2 = 1.0;
     a.P = c->P;
     a.slot_stride = c->N;
+    a.B.nb = 1;
+    a.B.lpi = c->N / g.n;
     return a;
+}
+
+// Launch of a pass.  When profiling, the kernel is launched with start/stop
+// events recorded by the dispatch itself (hipExtLaunchKernelGGL), so the
+// measured time is the kernel's own, as a kernel trace reports it.
+template <typename R, int KIND, int DIR, int VPT>
+void launch_vpt(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, hipEvent_t e0, hipEvent_t e1)
+{
+    if (e0)
+        hipExtLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, e0, e1,
+                              0, a);
+    else hipLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, a);
 }
 
 template <typename R, int KIND, int DIR>
 int launch(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, int cls)
 {
-    ProfScope ps(c, cls);
-    if (g.vpt == 4) hipLaunchKernelGGL((k_pass<R, KIND, DIR, 4>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, a);
-    else hipLaunchKernelGGL((k_pass<R, KIND, DIR, 8>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, a);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof) {
+        HIPCK(c, hipEventCreate(&e0));
+        HIPCK(c, hipEventCreate(&e1));
+        c->prof_pending.push_back({cls, {e0, e1}});
+    }
+    if (g.vpt == 4) launch_vpt<R, KIND, DIR, 4>(c, g, a, e0, e1);
+    else if (g.vpt == 12) launch_vpt<R, KIND, DIR, 12>(c, g, a, e0, e1);
+    else launch_vpt<R, KIND, DIR, 8>(c, g, a, e0, e1);
     HIPCK(c, hipGetLastError());
     return 0;
 }
@@ -275,9 +310,9 @@ template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R>
 }
 
 // plain transform of `data` along axis (dir -1 forward, +1 inverse), times scale
-template <typename R> int plain_pass(qnd_ctx *c, C<R> *data, int axis, int dir, double scale)
+template <typename R> int plain_pass(qnd_ctx *c, C<R> *data, int axis, int dir, double scale, int nb = 1)
 {
-    Geo g = geometry(c, axis);
+    Geo g = geometry(c, axis, nb);
     PassArgs<R> a = base_args<R>(c, data, g);
     a.scale1 = scale;
     if (dir < 0) return launch<R, PK_PLAIN, -1>(c, g, a, PC_PLAIN);
@@ -300,45 +335,66 @@ template <typename R> int full_fft(qnd_ctx *c, C<R> *data, int dir)
 // finalized by block 0 of the kinetic pass when fin is given.
 struct Finalize {
     const double *part;
-    int nwg;
-    double *rec;
+    int nwg;         // workgroups per instance of the pass that wrote the partials
+    double *rec;     // record of instance 0
     ObsSpec spec;
+    Batch B;         // nb, per-instance E(t_n) and record stride
+};
+
+int launch_finalize(qnd_ctx *c, const Finalize *fin)
+{
+    ProfScope ps(c, PC_SMALL);
+    hipLaunchKernelGGL(k_obs_finalize, dim3(fin->B.nb), dim3(64), 0, c->stream, fin->part, fin->nwg, c->dim,
+                       c->P.dNr, fin->spec, fin->B, fin->rec);
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
+// A per-instance field row for the passes of a batch: base of instance 0 and
+// the stride between instances (nullptr: the scalar field of the call).
+struct FieldRows {
+    const double *base = nullptr;
+    int64_t stride = 0;
 };
 
 template <typename R>
-int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A, const Finalize *fin = nullptr)
+int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A, const Finalize *fin = nullptr,
+                  int nb = 1, FieldRows Arows = FieldRows())
 {
     const int d = c->dim;
     int rc;
     if (fin && d > 2) {  // a plain pass comes first: finalize separately
-        ProfScope ps(c, PC_SMALL);
-        hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(64), 0, c->stream, fin->part, fin->nwg, d, c->P.dNr,
-                           fin->spec, fin->rec);
-        HIPCK(c, hipGetLastError());
+        if ((rc = launch_finalize(c, fin))) return rc;
         fin = nullptr;
     }
     for (int ax = 1; ax <= d - 2; ++ax) {
-        rc = plain_pass<R>(c, data, ax, -1, 1.0);
+        rc = plain_pass<R>(c, data, ax, -1, 1.0, nb);
         if (rc) return rc;
     }
-    Geo g = geometry(c, d - 1);
+    Geo g = geometry(c, d - 1, nb);
     PassArgs<R> a = base_args<R>(c, data, g);
     a.kmode = kmode;
     a.ck_mid = ck;
     a.scale2 = 1.0 / (double)c->nr[d - 1];
     if (A)
         for (int j = 0; j < d; ++j) a.Ak[j] = A[j];
+    a.B.nb = nb;
+    a.B.fld = Arows.base;
+    a.B.fld_stride = Arows.stride;
     if (fin) {
         a.flags |= PF_FINALIZE;
         a.fin_part = fin->part;
         a.fin_nwg = fin->nwg;
         a.fin_rec = fin->rec;
         a.fin_spec = fin->spec;
+        a.B.finE = fin->B.finE;
+        a.B.finE_stride = fin->B.finE_stride;
+        a.B.rec_stride = fin->B.rec_stride;
     }
     rc = launch<R, PK_KIN, -1>(c, g, a, PC_KIN);
     if (rc) return rc;
     for (int ax = d - 2; ax >= 1; --ax) {
-        rc = plain_pass<R>(c, data, ax, +1, 1.0 / (double)c->nr[ax]);
+        rc = plain_pass<R>(c, data, ax, +1, 1.0 / (double)c->nr[ax], nb);
         if (rc) return rc;
     }
     return 0;
@@ -677,77 +733,114 @@ bool fused_codes(const int *codes, int ncodes)
 }
 
 template <typename R>
-int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, const int *codes,
-               int ncodes, double *obs_out)
+int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int64_t nsteps, const int *codes,
+                    int ncodes, double *obs_out)
 {
     const int d = c->dim, s = c->s;
+    const bool batched = nb > 1;
+    const int64_t ntime = batched ? c->bntime : c->ntime;
     int rc;
-    if (step0 < 0 || step0 + nsteps > c->ntime)
+    if (step0 < 0 || step0 + nsteps > ntime)
         return fail(c, QND_EINVAL, "steps [%lld, %lld) outside the field table of %lld steps",
-                    (long long)step0, (long long)(step0 + nsteps), (long long)c->ntime);
+                    (long long)step0, (long long)(step0 + nsteps), (long long)ntime);
     int width = 0;
     for (int k = 0; k < ncodes; ++k) width += obs_width(codes[k], d);
     const bool fused = fused_codes(codes, ncodes);
     const bool want_obs = width > 0;
+    if (batched && want_obs && !fused)
+        return fail(c, QND_EUNSUPPORTED, "batched runs record dipole_r, dipole_a and norm only");
+    if (ncodes > 8) return fail(c, QND_EINVAL, "at most 8 observables per step");
     if (want_obs && fused) {
-        size_t need = (size_t)nsteps * width;
+        size_t need = (size_t)nb * nsteps * width;
         if (need > c->rec_cap) {
             if (c->d_rec) HIPCK(c, hipFree(c->d_rec));
             HIPCK(c, hipMalloc(&c->d_rec, need * sizeof(double)));
             c->rec_cap = need;
         }
     }
-    SlotState &st = c->slots[slot];
-    C<R> *psi = state_ptr<R>(c, slot);
-    const Geo g0 = geometry(c, 0);
+    // the slots of a batch move together: bring them to one state first
+    for (int b = 1; b < nb; ++b) {
+        const SlotState &s0 = c->slots[slot0], &sb = c->slots[slot0 + b];
+        if (sb.rep != s0.rep || sb.pend_mask != s0.pend_mask || sb.pend_vend != s0.pend_vend) {
+            for (int k = 0; k < nb; ++k)
+                if ((rc = ensure_rep<R>(c, slot0 + k, REP_R))) return rc;
+            break;
+        }
+    }
+    SlotState &st = c->slots[slot0];
+    C<R> *psi = state_ptr<R>(c, slot0);
+    const Geo g0 = geometry(c, 0, nb);
     if ((rc = ensure_part(c, (size_t)g0.grid * QND_NOBS))) return rc;
     const double inv_n0 = 1.0 / (double)c->nr[0];
     ObsSpec spec;
     memset(&spec, 0, sizeof spec);
-    spec.ncodes = ncodes < 8 ? ncodes : 8;
-    for (int k = 0; k < spec.ncodes; ++k) spec.codes[k] = codes[k];
-    if (ncodes > 8) return fail(c, QND_EINVAL, "at most 8 observables per step");
+    spec.ncodes = ncodes;
+    for (int k = 0; k < ncodes; ++k) spec.codes[k] = codes[k];
+    // per-instance field rows: [nb][ntime][s][d] (stage) and [nb][ntime][d] (t_n)
+    const int64_t stage_stride = ntime * s * d, obs_stride = ntime * d;
+    auto stage_rows = [&](const double *tab, int64_t it, int ia) {
+        FieldRows fr;
+        if (batched) {
+            fr.base = tab + (it * s + ia) * d;
+            fr.stride = stage_stride;
+        }
+        return fr;
+    };
 
     for (int64_t it = step0; it < step0 + nsteps; ++it) {
-        const double *Et = &c->Et[(size_t)it * s * d];
-        const double *At = &c->At[(size_t)it * s * d];
-        const double *Eo = &c->Eobs[(size_t)it * d];
-        const double *Ao = &c->Aobs[(size_t)it * d];
+        const double *Et = batched ? nullptr : &c->Et[(size_t)it * s * d];
+        const double *At = batched ? nullptr : &c->At[(size_t)it * s * d];
+        const double *Eo = batched ? nullptr : &c->Eobs[(size_t)it * d];
+        const double *Ao = batched ? nullptr : &c->Aobs[(size_t)it * d];
         // --- observables at the step boundary (unfused codes: position-space evaluation) ---
         if (want_obs && !fused) {
-            if ((rc = observables<R>(c, slot, codes, ncodes, gauge == QND_GAUGE_VELOCITY ? Ao : nullptr, Eo,
+            if ((rc = observables<R>(c, slot0, codes, ncodes, gauge == QND_GAUGE_VELOCITY ? Ao : nullptr, Eo,
                                      obs_out + (it - step0) * width)))
                 return rc;
         }
         // a slot in position space owes nothing; the boundary pass pays the
         // owed mask / half-step of a slot in the axis-0 Fourier representation
         if (st.rep == REP_R) {
-            if ((rc = plain_pass<R>(c, psi, 0, -1, 1.0))) return rc;
+            if ((rc = plain_pass<R>(c, psi, 0, -1, 1.0, nb))) return rc;
             st.rep = REP_M;
         }
         Finalize fin;
+        memset(&fin, 0, sizeof fin);
         const Finalize *finp = nullptr;
         if (want_obs && fused) {
-            for (int j = 0; j < d; ++j) spec.E[j] = Eo[j];
+            if (!batched)
+                for (int j = 0; j < d; ++j) spec.E[j] = Eo[j];
             fin.part = c->d_part;
-            fin.nwg = g0.grid;
+            fin.nwg = g0.grid / nb;
             fin.rec = c->d_rec + (it - step0) * width;
             fin.spec = spec;
+            fin.B.nb = nb;
+            fin.B.rec_stride = nsteps * width;
+            if (batched) {
+                fin.B.finE = c->d_bEo + it * d;
+                fin.B.finE_stride = obs_stride;
+            }
             finp = &fin;
         }
         auto finalize_now = [&]() -> int {
             if (!finp) return 0;
-            ProfScope ps(c, PC_SMALL);
-            hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(64), 0, c->stream, fin.part, fin.nwg, d, c->P.dNr,
-                               fin.spec, fin.rec);
-            HIPCK(c, hipGetLastError());
+            int r = launch_finalize(c, finp);
             finp = nullptr;
-            return 0;
+            return r;
+        };
+        auto pot_args = [&]() {
+            PassArgs<R> a = base_args<R>(c, psi, g0);
+            a.scale1 = inv_n0;
+            a.B.nb = nb;
+            return a;
+        };
+        auto set_rows = [&](PassArgs<R> &a, FieldRows fr) {
+            a.B.fld = fr.base;
+            a.B.fld_stride = fr.stride;
         };
         if (gauge == QND_GAUGE_LENGTH) {
             // boundary pass: owed mask, observables
-            PassArgs<R> a = base_args<R>(c, psi, g0);
-            a.scale1 = inv_n0;
+            PassArgs<R> a = pot_args();
             a.flags = PF_SECOND;
             if (st.pend_mask) a.flags |= PF_MASK;
             if (finp) {
@@ -763,15 +856,16 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
             for (int ia = 0; ia < s; ++ia) {
                 const double a0 = c->as_scaled[2 * ia], a1 = c->as_scaled[2 * ia + 1];
                 if (d > 1 && ia == 0) {
-                    if ((rc = kinetic_block<R>(c, psi, KM_PHASE, a0, nullptr, finp))) return rc;
+                    if ((rc = kinetic_block<R>(c, psi, KM_PHASE, a0, nullptr, finp, nb))) return rc;
                     finp = nullptr;
                 }
-                PassArgs<R> b = base_args<R>(c, psi, g0);
-                b.scale1 = inv_n0;
+                PassArgs<R> b = pot_args();
                 b.flags = PF_V1 | PF_SECOND;
                 b.vmode = VM_LENGTH;
                 b.cv1 = a0 + a1;
-                for (int j = 0; j < d; ++j) b.Ev[j] = Et[ia * d + j];
+                if (batched) set_rows(b, stage_rows(c->d_bEt, it, ia));
+                else
+                    for (int j = 0; j < d; ++j) b.Ev[j] = Et[ia * d + j];
                 if (d == 1) {
                     b.flags |= PF_KPRE | PF_KPOST;
                     b.kmode = KM_PHASE;
@@ -781,7 +875,7 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
                 if ((rc = launch_pot<R>(c, g0, b))) return rc;
                 if (d > 1) {
                     double ck = a1 + (ia < s - 1 ? c->as_scaled[2 * ia + 2] : 0.0);
-                    if ((rc = kinetic_block<R>(c, psi, KM_PHASE, ck, nullptr))) return rc;
+                    if ((rc = kinetic_block<R>(c, psi, KM_PHASE, ck, nullptr, nullptr, nb))) return rc;
                 }
             }
             st.pend_mask = c->P.mask_on != 0;
@@ -789,8 +883,7 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
             // velocity gauge: boundary pass carries the owed last half-step,
             // the owed mask, the observables and the first half-step
             for (int ia = 0; ia < s; ++ia) {
-                PassArgs<R> a = base_args<R>(c, psi, g0);
-                a.scale1 = inv_n0;
+                PassArgs<R> a = pot_args();
                 a.flags = PF_V2 | PF_SECOND;
                 a.vmode = VM_VELOCITY;
                 a.cv2 = c->as_scaled[2 * ia];
@@ -814,7 +907,8 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
                     a.flags |= PF_KPOST;
                     a.kmode = KM_VELOCITY;
                     a.ck_post = ck;
-                    a.Ak[0] = At[ia * d];
+                    if (batched) set_rows(a, stage_rows(c->d_bAt, it, ia));
+                    else a.Ak[0] = At[ia * d];
                 }
                 if ((rc = launch_pot<R>(c, g0, a))) return rc;
                 if (ia == 0) {
@@ -822,7 +916,9 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
                     if (d == 1 && (rc = finalize_now())) return rc;
                 }
                 if (d > 1) {
-                    if ((rc = kinetic_block<R>(c, psi, KM_VELOCITY, ck, &At[ia * d], ia == 0 ? finp : nullptr)))
+                    if ((rc = kinetic_block<R>(c, psi, KM_VELOCITY, ck, batched ? nullptr : &At[ia * d],
+                                               ia == 0 ? finp : nullptr, nb,
+                                               batched ? stage_rows(c->d_bAt, it, ia) : FieldRows())))
                         return rc;
                     if (ia == 0) finp = nullptr;
                 }
@@ -831,13 +927,22 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
             st.pend_mask = c->P.mask_on != 0;
         }
     }
+    for (int b = 1; b < nb; ++b) c->slots[slot0 + b] = st;
     if (want_obs && fused) {
-        HIPCK(c, hipMemcpyAsync(obs_out, c->d_rec, (size_t)nsteps * width * sizeof(double),
+        // device records [nb][nsteps][width] -> obs_out [nb][nsteps][width]
+        HIPCK(c, hipMemcpyAsync(obs_out, c->d_rec, (size_t)nb * nsteps * width * sizeof(double),
                                 hipMemcpyDeviceToHost, c->stream));
     }
     HIPCK(c, hipStreamSynchronize(c->stream));
     prof_flush(c);
     return 0;
+}
+
+template <typename R>
+int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, const int *codes, int ncodes,
+               double *obs_out)
+{
+    return real_steps_impl<R>(c, slot, 1, gauge, step0, nsteps, codes, ncodes, obs_out);
 }
 
 // Write_final_observables numerics (src/io_functions.c:159-199)
@@ -1021,9 +1126,12 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
         return fail(nullptr, QND_EINVAL, "unknown precision %d", precision);
     if (n_slots < 1) return fail(nullptr, QND_EINVAL, "n_slots must be >= 1");
     for (int j = 0; j < dim; ++j) {
-        if (nr[j] < 8 || (nr[j] & (nr[j] - 1)) != 0)
+        const int64_t m = (nr[j] % 3 == 0) ? nr[j] / 3 : nr[j];
+        const bool ok = (nr[j] % 3 == 0) ? (m >= 4 && (m & (m - 1)) == 0) : (m >= 8 && (m & (m - 1)) == 0);
+        if (!ok)
             return fail(nullptr, QND_EUNSUPPORTED,
-                        "NR[%d] = %lld: the GPU path needs a power of two >= 8", j, (long long)nr[j]);
+                        "NR[%d] = %lld: the GPU path needs a power of two >= 8 or 3 * 2^k >= 12", j,
+                        (long long)nr[j]);
         if (!(dr[j] > 0)) return fail(nullptr, QND_EINVAL, "DR[%d] must be positive", j);
     }
     qnd_ctx *c = new qnd_ctx();
@@ -1051,7 +1159,8 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
         c->lds_max = m > 4096 ? m - 4096 : m;
     }
     for (int j = 0; j < dim; ++j) {
-        if (2 * (size_t)nr[j] * c->esz > c->lds_max || nr[j] / 8 > QND_PASS_THREADS) {
+        const int64_t tpl = (nr[j] % 3 == 0) ? nr[j] / 12 : nr[j] / 8;  // threads per line
+        if (2 * (size_t)nr[j] * c->esz > c->lds_max || tpl > QND_PASS_THREADS) {
             delete c;
             return fail(nullptr, QND_EUNSUPPORTED, "NR[%d] = %lld exceeds the single-pass line limit", j,
                         (long long)nr[j]);
@@ -1062,11 +1171,13 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     memset(&P, 0, sizeof P);
     P.dim = dim;
     P.dNr = 1.0;
+    P.pow2 = 1;
     for (int j = 0; j < QND_MAXD; ++j) P.n[j] = 1;
     int off = 0;
     for (int j = 0; j < dim; ++j) {
         P.n[j] = (int)nr[j];
         P.lg[j] = ilog2(nr[j]);
+        if ((nr[j] & (nr[j] - 1)) != 0) P.pow2 = 0;
         P.off[j] = off;
         off += (int)nr[j];
         P.dr[j] = dr[j];
@@ -1146,6 +1257,9 @@ void qnd_destroy(qnd_ctx *c)
     if (c->d_prmax) hipFree(c->d_prmax);
     if (c->d_capt) hipFree(c->d_capt);
     if (c->d_maskt) hipFree(c->d_maskt);
+    if (c->d_bEt) hipFree(c->d_bEt);
+    if (c->d_bAt) hipFree(c->d_bAt);
+    if (c->d_bEo) hipFree(c->d_bEo);
     for (int k = 0; k < 16; ++k)
         if (c->ev[k]) hipEventDestroy(c->ev[k]);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1329,6 +1443,50 @@ int qnd_real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nstep
         return fail(c, QND_EINVAL, "unknown gauge %d", gauge);
     if (ncodes > 0 && (!codes || !obs_out)) return fail(c, QND_EINVAL, "observable codes need an output array");
     return DISPATCH(c, real_steps, c, slot, gauge, step0, nsteps, codes, ncodes, obs_out);
+}
+
+int qnd_set_field_batch(qnd_ctx *c, int nb, int64_t ntime, const double *Et, const double *At,
+                        const double *Eobs)
+{
+    if (!c || nb < 1 || ntime < 0) return QND_EINVAL;
+    if (!c->have_int) return fail(c, QND_ESTATE, "integrator not set (qnd_set_integrator)");
+    if (nb > c->n_slots) return fail(c, QND_EINVAL, "batch of %d exceeds the %d slots", nb, c->n_slots);
+    const size_t ns = (size_t)nb * ntime * c->s * c->dim, no = (size_t)nb * ntime * c->dim;
+    if (c->d_bEt) HIPCK(c, hipFree(c->d_bEt));
+    if (c->d_bAt) HIPCK(c, hipFree(c->d_bAt));
+    if (c->d_bEo) HIPCK(c, hipFree(c->d_bEo));
+    c->d_bEt = c->d_bAt = c->d_bEo = nullptr;
+    HIPCK(c, hipMalloc(&c->d_bEt, (ns ? ns : 1) * sizeof(double)));
+    HIPCK(c, hipMalloc(&c->d_bAt, (ns ? ns : 1) * sizeof(double)));
+    HIPCK(c, hipMalloc(&c->d_bEo, (no ? no : 1) * sizeof(double)));
+    HIPCK(c, hipMemset(c->d_bEt, 0, (ns ? ns : 1) * sizeof(double)));
+    HIPCK(c, hipMemset(c->d_bAt, 0, (ns ? ns : 1) * sizeof(double)));
+    HIPCK(c, hipMemset(c->d_bEo, 0, (no ? no : 1) * sizeof(double)));
+    if (Et && ns) HIPCK(c, hipMemcpy(c->d_bEt, Et, ns * sizeof(double), hipMemcpyHostToDevice));
+    if (At && ns) HIPCK(c, hipMemcpy(c->d_bAt, At, ns * sizeof(double), hipMemcpyHostToDevice));
+    if (Eobs && no) HIPCK(c, hipMemcpy(c->d_bEo, Eobs, no * sizeof(double), hipMemcpyHostToDevice));
+    c->bnb = nb;
+    c->bntime = ntime;
+    return 0;
+}
+
+int qnd_real_steps_batch(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int64_t nsteps,
+                         const int *codes, int ncodes, double *obs_out)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (nb < 1 || slot0 < 0 || slot0 + nb > c->n_slots)
+        return fail(c, QND_EINVAL, "slots [%d, %d) outside the %d slots", slot0, slot0 + nb, c->n_slots);
+    if (nb > 1 && nb != c->bnb)
+        return fail(c, QND_ESTATE, "batch of %d but the field tables hold %d instances (qnd_set_field_batch)", nb,
+                    c->bnb);
+    if (gauge != QND_GAUGE_LENGTH && gauge != QND_GAUGE_VELOCITY)
+        return fail(c, QND_EINVAL, "unknown gauge %d", gauge);
+    if (ncodes > 0 && (!codes || !obs_out)) return fail(c, QND_EINVAL, "observable codes need an output array");
+    if (nb == 1) return DISPATCH(c, real_steps, c, slot0, gauge, step0, nsteps, codes, ncodes, obs_out);
+    return c->prec == QND_C64
+               ? real_steps_impl<float>(c, slot0, nb, gauge, step0, nsteps, codes, ncodes, obs_out)
+               : real_steps_impl<double>(c, slot0, nb, gauge, step0, nsteps, codes, ncodes, obs_out);
 }
 
 int qnd_final_observables(qnd_ctx *c, int slot, const double *R_mask, const double *pr_max, int Ne, double de,

@@ -112,9 +112,33 @@ template <typename R, int DIR> __device__ __forceinline__ void dft8(C<R> *x)
     x[7] = sub<R>(e[3], o3);
 }
 
+// t * (sqrt(3)/2), FP32 as a hi+lo pair for the same reason as mul_rsqrt2
+template <typename R> __device__ __forceinline__ R mul_sqrt3h(R t);
+template <> __device__ __forceinline__ double mul_sqrt3h<double>(double t)
+{
+    return t * 0.86602540378443864676372317075293618347;
+}
+template <> __device__ __forceinline__ float mul_sqrt3h<float>(float t)
+{
+    const float hi = 0.8660254f, lo = 2.3801166e-08f;  // hi + lo = sqrt(3)/2 to ~1e-15
+    return fmaf(t, hi, t * lo);
+}
+
+// radix 3: y_q = sum_r x_r w^{qr}, w = e^{DIR 2 pi i/3} = -1/2 + DIR i sqrt(3)/2
+template <typename R, int DIR> __device__ __forceinline__ void dft3(C<R> *x)
+{
+    const C<R> s = add<R>(x[1], x[2]), d = sub<R>(x[1], x[2]);
+    const C<R> m = mk<R>(x[0].x - (R)0.5 * s.x, x[0].y - (R)0.5 * s.y);
+    const C<R> r = mul_di<R, DIR>(mk<R>(mul_sqrt3h<R>(d.x), mul_sqrt3h<R>(d.y)));
+    x[0] = add<R>(x[0], s);
+    x[1] = add<R>(m, r);
+    x[2] = sub<R>(m, r);
+}
+
 template <typename R, int RAD, int DIR> __device__ __forceinline__ void dftR(C<R> *x)
 {
     if (RAD == 2) dft2<R, DIR>(x);
+    else if (RAD == 3) dft3<R, DIR>(x);
     else if (RAD == 4) dft4<R, DIR>(x);
     else dft8<R, DIR>(x);
 }
@@ -186,7 +210,7 @@ template <typename R, int RAD, int DIR, int VPT>
 __device__ __forceinline__ void reg_stage(C<R> (&x)[VPT], const C<R> *lin, C<R> *lout, int n, int T, int t,
                                           int Ns, const typename TwT<R>::T *__restrict__ tw)
 {
-    if constexpr (RAD <= VPT) {
+    if constexpr (VPT % RAD == 0) {
         constexpr int NB = VPT / RAD;
         const int nR = n / RAD;
         const int stride = n / (Ns * RAD);
@@ -235,7 +259,9 @@ __device__ __forceinline__ void reg_stage(C<R> (&x)[VPT], const C<R> *lin, C<R> 
 // bufA / bufB.  Eight values per thread: radix-8 stages, then radix 4 or 2
 // for the remainder; four values per thread (twice the threads, for grids
 // too small to fill the chip otherwise): radix-4 stages, then radix 2 if
-// log2 n is odd.  Every thread of the block calls it (barriers inside).
+// log2 n is odd; twelve values per thread (n = 3 * 2^k): radix-4 stages,
+// radix 2 if k is odd, radix 3 last.  Every thread of the block calls it
+// (barriers inside).
 template <typename R, int DIR, int VPT>
 __device__ __forceinline__ void fft_line(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int &wc, int n, int log2n,
                                          int T, int t, const typename TwT<R>::T *__restrict__ tw)
@@ -246,8 +272,12 @@ __device__ __forceinline__ void fft_line(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB,
         if (log2n % 3 == 2) rad[nst++] = 4;
         else if (log2n % 3 == 1) rad[nst++] = 2;
     } else {
+        // VPT 4: n = 2^log2n.  VPT 12: n = 3 * 2^log2n (log2n is then the
+        // exponent of the power-of-two part), the radix-3 stage last so that
+        // every span Ns before it is a power of two
         for (int s = 0; s < log2n / 2; ++s) rad[nst++] = 4;
         if (log2n & 1) rad[nst++] = 2;
+        if (VPT == 12) rad[nst++] = 3;
     }
     const C<R> *lin = nullptr;
     int Ns = 1;
@@ -256,6 +286,7 @@ __device__ __forceinline__ void fft_line(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB,
         C<R> *lout = last ? nullptr : ((wc & 1) ? bufB : bufA);
         if (rad[s] == 8) reg_stage<R, 8, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
         else if (rad[s] == 4) reg_stage<R, 4, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
+        else if (rad[s] == 3) reg_stage<R, 3, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
         else reg_stage<R, 2, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
         if (!last) {
             lds_barrier();
@@ -273,7 +304,8 @@ __device__ __forceinline__ void fft_line(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB,
 struct Phys {
     int dim;
     int n[QND_MAXD];
-    int lg[QND_MAXD];        // log2 n
+    int lg[QND_MAXD];        // log2 n (when every extent is a power of two)
+    int pow2;                // every extent a power of two: shift/mask indexing
     int off[QND_MAXD];       // offsets of each axis in the per-axis tables
     double dr[QND_MAXD];
     double L[QND_MAXD];      // n*dr
@@ -503,6 +535,20 @@ struct ObsSpec {
     double E[QND_MAXD];
 };
 
+// Batched instances: nb consecutive slots propagate in one launch, each under
+// its own field.  Field rows are FP64 [dim] vectors; instance b's row is at
+// base + b * stride.  The observable record of instance b is finalized from
+// the partials of its own workgroups (they never straddle instances).
+struct Batch {
+    int nb;                  // instances (1: the scalar fields of PassArgs apply)
+    int64_t lpi;             // lines per instance of this pass
+    const double *fld;       // this stage's E (length) or A (velocity) of instance 0
+    int64_t fld_stride;
+    const double *finE;      // E(t_n) of instance 0 for the record being finalized
+    int64_t finE_stride;
+    int64_t rec_stride;      // doubles between the records of two instances
+};
+
 template <typename R> struct PassArgs {
     C<R> *data;
     const typename TwT<R>::T *tw;
@@ -528,28 +574,51 @@ template <typename R> struct PassArgs {
     int fin_nwg;
     double *fin_rec;
     ObsSpec fin_spec;
+    Batch B;
     Phys P;
 };
 
-// Multi-index of the line L (all axes but `axis`); all extents are 2^lg.
+
+// Multi-index of the line L (all axes but `axis`).  L = o * S + in, with
+// `in` the index over the axes after `axis` and `o` over those before it (a
+// batch index above axis 0 drops out of the modulo).
 __device__ __forceinline__ void line_index(const Phys &P, int axis, int64_t S, int log2S, int64_t L, int *idx)
 {
-    int64_t o = L >> log2S, in = L & (S - 1);
-    int sh = 0;
+    if (P.pow2) {
+        int64_t o = L >> log2S, in = L & (S - 1);
+        int sh = 0;
+#pragma unroll
+        for (int j = QND_MAXD - 1; j >= 0; --j) {
+            idx[j] = 0;
+            if (j < P.dim && j > axis) {
+                idx[j] = (int)((in >> sh) & (P.n[j] - 1));
+                sh += P.lg[j];
+            }
+        }
+        sh = 0;
+#pragma unroll
+        for (int j = QND_MAXD - 1; j >= 0; --j) {
+            if (j < axis) {
+                idx[j] = (int)((o >> sh) & (P.n[j] - 1));
+                sh += P.lg[j];
+            }
+        }
+        return;
+    }
+    int64_t o = L / S, in = L - o * S;
 #pragma unroll
     for (int j = QND_MAXD - 1; j >= 0; --j) {
         idx[j] = 0;
         if (j < P.dim && j > axis) {
-            idx[j] = (int)((in >> sh) & (P.n[j] - 1));
-            sh += P.lg[j];
+            idx[j] = (int)(in % P.n[j]);
+            in /= P.n[j];
         }
     }
-    sh = 0;
 #pragma unroll
     for (int j = QND_MAXD - 1; j >= 0; --j) {
         if (j < axis) {
-            idx[j] = (int)((o >> sh) & (P.n[j] - 1));
-            sh += P.lg[j];
+            idx[j] = (int)(o % P.n[j]);
+            o /= P.n[j];
         }
     }
 }
@@ -557,7 +626,7 @@ __device__ __forceinline__ void line_index(const Phys &P, int axis, int64_t S, i
 // turn per-workgroup [n0, d_j, g_j] sums into the observable record (wave 0
 // of block 0)
 __device__ __forceinline__ void obs_finalize(const double *part, int nwg, int dim, double dNr,
-                                             const ObsSpec &spec, double *rec)
+                                             const ObsSpec &spec, const double *E, double *rec)
 {
     if (threadIdx.x >= 64) return;
     double tot[QND_NOBS];
@@ -575,11 +644,19 @@ __device__ __forceinline__ void obs_finalize(const double *part, int nwg, int di
             } else if (c == 6) {                                    // dipole_a
 #pragma unroll
                 for (int j = 0; j < QND_MAXD; ++j)
-                    if (j < dim) rec[off++] = -tot[1 + QND_MAXD + j] - spec.E[j] * tot[0];
+                    if (j < dim) rec[off++] = -tot[1 + QND_MAXD + j] - E[j] * tot[0];
             }
         }
     }
 }
+// Instance-b view of a finalize request: partials, E(t_n) and record.
+__device__ __forceinline__ void obs_finalize_instance(const double *part, int nwg, int dim, double dNr,
+                                                      const ObsSpec &spec, const Batch &B, int b, double *rec)
+{
+    const double *E = B.finE ? B.finE + b * B.finE_stride : spec.E;
+    obs_finalize(part + (int64_t)b * nwg * QND_NOBS, nwg, dim, dNr, spec, E, rec + b * B.rec_stride);
+}
+
 
 // ---------------------------------------------------------------------
 // the line-pass kernel
@@ -598,8 +675,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
     const Phys &P = a.P;
     const unsigned fl = a.flags;
 
-    if (KIND == PK_KIN && (fl & PF_FINALIZE) && blockIdx.x == 0)
-        obs_finalize(a.fin_part, a.fin_nwg, P.dim, P.dNr, a.fin_spec, a.fin_rec);
+    if (KIND == PK_KIN && (fl & PF_FINALIZE) && (int)blockIdx.x < a.B.nb)
+        obs_finalize_instance(a.fin_part, a.fin_nwg, P.dim, P.dNr, a.fin_spec, a.B, blockIdx.x, a.fin_rec);
 
     // ---- tile geometry: TW lines (TW | S or S == 1).  Contiguous lines put
     // t on the fastest lanes, strided lines the line index l, so that every
@@ -610,10 +687,18 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
     int l, t;
     if (contig) { l = tid >> a.log2T; t = tid & (T - 1); }
     else { l = tid & (TW - 1); t = tid >> a.log2TW; }
-    const int64_t L = (int64_t)blockIdx.x * TW + l;  // this thread's line
+    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
+    // so workgroup w runs on XCD w % 8.  Tile tb = (w % 8) * (grid / 8) + w / 8
+    // gives each XCD a contiguous run of tiles: neighbouring strided tiles,
+    // which share cache lines, then share one L2 instead of fetching the same
+    // lines into several.
+    const int nblk = gridDim.x;
+    const int tb = (nblk % 8 == 0) ? (int)(blockIdx.x % 8) * (nblk / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+    const int64_t L = (int64_t)tb * TW + l;  // this thread's line
     int64_t lbase, estride;                          // element i at lbase + i * estride
-    if (contig) { lbase = L << a.log2n; estride = 1; }
-    else { lbase = (((L >> log2S) << a.log2n) << log2S) + (L & (a.S - 1)); estride = a.S; }
+    if (contig) { lbase = L * n; estride = 1; }
+    else if (P.pow2) { lbase = (((L >> log2S) << a.log2n) << log2S) + (L & (a.S - 1)); estride = a.S; }
+    else { const int64_t o = L / a.S; lbase = o * n * a.S + (L - o * a.S); estride = a.S; }
     C<R> *bufA = lds + l * ldsS;
     C<R> *bufB = lds + (TW + l) * ldsS;
     int wc = 0;
@@ -679,6 +764,15 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
         // ---- first transform ----
         fft_line<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw);
 
+        // this instance's stage field (E in the length gauge, A in the velocity gauge)
+        double fv[QND_MAXD];
+        {
+            const bool use_a = (KIND == PK_KIN) || a.kmode == KM_VELOCITY;
+            const double *src = use_a ? a.Ak : a.Ev;
+            if (a.B.fld) src = a.B.fld + (L / a.B.lpi) * a.B.fld_stride;
+#pragma unroll
+            for (int j = 0; j < QND_MAXD; ++j) fv[j] = (j < P.dim) ? src[j] : 0.0;
+        }
         if (KIND == PK_KIN) {
             // all axes in k-space: kinetic factor (reference order of the sums)
             double Ko = 0.0, Wk = 0.0;
@@ -687,10 +781,10 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                 if (j < P.dim && j != ax) {
                     const double pj = coord_p(P, j, idx[j]);
                     Ko += .5 * pj * pj;
-                    Wk += (pj + .5 * a.Ak[j]) * a.Ak[j];
+                    Wk += (pj + .5 * fv[j]) * fv[j];
                 }
             }
-            const double Aax = a.Ak[ax];
+            const double Aax = fv[ax];
 #pragma unroll
             for (int m = 0; m < VPT; ++m) {
                 const double p = coord_p(P, ax, t + m * T);
@@ -710,7 +804,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                     const double xj = coord_x(P, j, idx[j]);
                     xo[j] = xj;
                     rro += xj * xj;
-                    Wv += xj * a.Ev[j];
+                    Wv += xj * fv[j];
                     if (P.cap_on) Vco += P.capt[P.off[j] + idx[j]];
                     if (P.mask_on) mko *= P.maskt[P.off[j] + idx[j]];
                 }
@@ -730,7 +824,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                 if (fl & PF_V1) {
                     if (a.vmode == VM_IMAG) v = scl<R>(v, real_exp<R>(-a.cv1 * V));
                     else if (a.vmode == VM_LENGTH)
-                        v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, a.cv1 * (V + (xc * a.Ev[ax] + Wv))));
+                        v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, a.cv1 * (V + (xc * fv[ax] + Wv))));
                     else v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, a.cv1 * V));
                 }
                 if ((fl & PF_MASK) && P.mask_on) v = scl<R>(v, (R)(mskv[m] * mko));
@@ -752,7 +846,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                 }
                 x[m] = v;
             }
-            if (fl & PF_OBS) block_sum_vec<QND_NOBS>(obs, red, a.partials + (int64_t)blockIdx.x * a.npart);
+            if (fl & PF_OBS) block_sum_vec<QND_NOBS>(obs, red, a.partials + (int64_t)tb * a.npart);
         }
 
         // ---- second transform ----
@@ -767,7 +861,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                     const C<R> v = scl<R>(x[m], s2);
                     if (a.kmode == KM_IMAG) x[m] = scl<R>(v, real_exp<R>(-a.ck_post * K));
                     else if (a.kmode == KM_VELOCITY)
-                        x[m] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * (K + (p + .5 * a.Ak[ax]) * a.Ak[ax])));
+                        x[m] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * (K + (p + .5 * fv[ax]) * fv[ax])));
                     else x[m] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * K));
                 }
             } else if (s2 != (R)1) {
@@ -800,7 +894,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                 for (int m = 0; m < VPT; ++m) sr += (double)x[m].x * (double)x[m].x + (double)x[m].y * (double)x[m].y;
             }
             double s2v[2] = {sr, si};
-            block_sum_vec<2>(s2v, red, a.partials + (int64_t)blockIdx.x * a.npart + 2 * j);
+            block_sum_vec<2>(s2v, red, a.partials + (int64_t)tb * a.npart + 2 * j);
         }
     }
 }
@@ -820,10 +914,11 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__
     }
 }
 
+// one block per instance
 __global__ void __launch_bounds__(64) k_obs_finalize(const double *__restrict__ part, int nwg, int dim,
-                                                     double dNr, ObsSpec spec, double *rec)
+                                                     double dNr, ObsSpec spec, Batch B, double *rec)
 {
-    obs_finalize(part, nwg, dim, dNr, spec, rec);
+    obs_finalize_instance(part, nwg, dim, dNr, spec, B, blockIdx.x, rec);
 }
 
 // Modified Gram-Schmidt coefficients from partial dots in the stored
@@ -880,13 +975,24 @@ __global__ void __launch_bounds__(256) k_gs_coeffs(const double *__restrict__ pa
 // ---------------------------------------------------------------------
 __device__ __forceinline__ void flat_index(const Phys &P, int64_t f, int *idx)
 {
-    int sh = 0;
+    if (P.pow2) {
+        int sh = 0;
+#pragma unroll
+        for (int j = QND_MAXD - 1; j >= 0; --j) {
+            idx[j] = 0;
+            if (j < P.dim) {
+                idx[j] = (int)((f >> sh) & (P.n[j] - 1));
+                sh += P.lg[j];
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int j = QND_MAXD - 1; j >= 0; --j) {
         idx[j] = 0;
         if (j < P.dim) {
-            idx[j] = (int)((f >> sh) & (P.n[j] - 1));
-            sh += P.lg[j];
+            idx[j] = (int)(f % P.n[j]);
+            f /= P.n[j];
         }
     }
 }

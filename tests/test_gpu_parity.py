@@ -33,7 +33,9 @@ def smooth_state(o, seed, n_states=1):
 
 @pytest.mark.parametrize("prec,tol", [("c128", 1e-14), ("c64", 1e-6)])
 @pytest.mark.parametrize("shape", [(8,), (16,), (32,), (64,), (128,), (256,), (512,), (1024,), (2048,),
-                                   (4096,), (16, 32), (512, 512), (8, 16, 32), (32, 8, 64)])
+                                   (4096,), (16, 32), (512, 512), (8, 16, 32), (32, 8, 64),
+                                   (12,), (24,), (48,), (96,), (192,), (384,), (768,), (1536,), (3072,),
+                                   (48, 96), (384, 384), (12, 32, 24)])
 def test_transform_matches_numpy(gpu, shape, prec, tol):
     if prec == "c128" and shape[-1] > 4096:
         pytest.skip("line too long for c128")
@@ -57,6 +59,8 @@ IMAG_CASES = [
     ((32, 64), 0.4, "Ne_2D", "BM6"),
     ((16, 16, 16), 0.5, "H_3D", "BM4"),
     ((8, 16, 8, 8), 0.8, "H_3D", "Verlet"),
+    ((96,), 0.3, "H_1D", "BM4"),                 # 3 * 2^k lines: radix-3 stage
+    ((48, 24), 0.4, "H_2D", "BM4"),
 ]
 
 
@@ -131,6 +135,8 @@ REAL_CASES = [
     ((32, 64), 0.4, "H_2D", "BM6", "velocity", 0.0, (0.1, 3.0)),
     ((16, 16, 16), 0.5, "H_3D", "BM4", "length", 0.1, (0.05, 1.0)),
     ((16, 16, 16), 0.5, "H_3D", "Verlet", "velocity", 0.1, (0.05, 1.0)),
+    ((384,), 0.2, "H_1D", "BM4", "velocity", 0.0, (0.1, 3.0)),
+    ((48, 96), 0.4, "Ne_2D", "BM4", "length", 0.1, (0.0, 0.0)),
 ]
 
 
@@ -246,3 +252,48 @@ def test_gram_schmidt_collapse_error(gpu):
     with pytest.raises(q.QNDError, match="norm decayed"):
         g.gram_schmidt(1)
     g.close()
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("nr,gauge", [((32, 48), "length"), ((64, 32), "velocity"), ((96,), "length")])
+def test_batched_real_steps(gpu, nr, gauge, prec):
+    """nb instances under different fields in one launch per pass: each equals
+    its own single-instance run, and instance 0 (complex128) the oracle."""
+    print(f"case batch nr={nr} gauge={gauge} prec={prec}", flush=True)
+    dt, nt, nb, dr, pot = 0.05, 10, 3, 0.4, "H_2D" if len(nr) == 2 else "H_1D"
+    dim = len(nr)
+    o = O.Oracle(nr, dr, pot, "BM4", dt)
+    fields = [field_tables(o, nt, dim, I=1e14 * (1 + b), ellip=0.1 * b) for b in range(nb)]
+    psi0 = smooth_state(o, 5)[0]
+    names = ["dipole_r", "dipole_a", "norm"]
+    g = q.Grid(nr, dr, prec, n_slots=nb)
+    g.set_potential(pot)
+    g.set_integrator("BM4", dt)
+    g.set_mask(0.1)
+    g.set_field_batch(np.stack([f[0] for f in fields]), np.stack([f[1] for f in fields]),
+                      np.stack([f[2] for f in fields]))
+    for b in range(nb):
+        g.upload(b, psi0)
+    obs = np.concatenate([g.real_steps_batch(0, nb, gauge, 0, nt // 2, names),
+                          g.real_steps_batch(0, nb, gauge, nt // 2, nt - nt // 2, names)], axis=1)
+    got = [g.download(b) for b in range(nb)]
+    g.close()
+    for b in range(nb):
+        s = q.Grid(nr, dr, prec, n_slots=1)
+        s.set_potential(pot)
+        s.set_integrator("BM4", dt)
+        s.set_mask(0.1)
+        s.set_field(*fields[b])
+        s.upload(0, psi0)
+        one = s.real_steps(0, gauge, 0, nt, names)
+        # the batch may pick another launch geometry (values per thread), so
+        # complex64 agrees to its own rounding, complex128 to FP64 rounding
+        tol = 1e-12 if prec == "c128" else TOL64
+        assert rel(got[b], s.download(0)) < tol, (b, rel(got[b], s.download(0)))
+        assert np.allclose(obs[b], one, rtol=tol, atol=tol * 1e-2), (b, np.abs(obs[b] - one).max())
+        s.close()
+    if prec == "c128":
+        o.set_mask(0.1)
+        ref_psi, ref_obs = o.real_run(psi0, gauge, *fields[0], names)
+        assert rel(got[0], ref_psi) < TOL128, rel(got[0], ref_psi)
+        assert np.allclose(obs[0], ref_obs, rtol=1e-11, atol=1e-12)

@@ -51,7 +51,9 @@ def test_create_fails_loudly_without_gpu():
 
 def test_create_rejects_bad_grids():
     with pytest.raises(q.QNDError, match="power of two"):
-        q.Grid((48,), 0.1)
+        q.Grid((40,), 0.1)
+    with pytest.raises(q.QNDError, match="power of two"):
+        q.Grid((6,), 0.1)          # 3 * 2^k needs k >= 2
     with pytest.raises(q.QNDError, match="dimension"):
         q.Grid((8,) * 5, 0.1)
 
