@@ -95,6 +95,93 @@ def real_run(q, field, prec, psi0, warmup, steps, Et, At, Eo, Ao, profile_steps=
     return np.vstack([obs_w, obs_t]), wall, gpu_ms, prof
 
 
+# ---------------------------------------------------------------------
+# CEP / intensity sweep: 96 H_2D instances (soft-core a = 1) on uneven grids,
+# 32 each at 128^2, 256^2, 384^2, DR 0.25; CEP uniform in [0, 2 pi) and
+# intensity log-uniform in [5e13, 5e14] W/cm^2 from a seed; 800 nm 3-cycle
+# cos2 pulse, BM4, DT 0.05, complex64 with FP64 reductions.  One batched
+# context per grid size (every pass is one launch over its 32 instances),
+# launch geometry autotuned, the three classes running concurrently.
+# ---------------------------------------------------------------------
+SWEEP_SIZES = (128, 256, 384)
+SWEEP_PER = 32
+SWEEP_DR = 0.25
+SWEEP_SEED = 20261203
+
+
+def sweep_fields(F, steps):
+    rng = np.random.default_rng(SWEEP_SEED)
+    nb = SWEEP_PER * len(SWEEP_SIZES)
+    ceps = rng.uniform(0.0, 2 * np.pi, nb)
+    ints = 10.0 ** rng.uniform(np.log10(5e13), np.log10(5e14), nb)
+    tabs = []
+    for i in range(nb):
+        m = F.Mode(intensity=ints[i], wavelength=800.0, ellipticity=0.0, cep=ceps[i], envelope="cos2", cycles=3)
+        tabs.append(F.tables([m], "BM4", DT, steps, 2))
+    ntime_full = int(F.Mode(cycles=3).tau / DT)
+    return ceps, ints, tabs, ntime_full
+
+
+def sweep_ground_state(q, n):
+    g = q.Grid((n, n), SWEEP_DR, "c128", n_slots=1)
+    g.set_potential("H_2D", softcore=1.0)
+    g.set_integrator("BM4", 0.1)
+    g.upload(0, reference_guess((n, n), SWEEP_DR))
+    g.imag_steps(0, q.imag_step_count(100.0, 0.1), gram_schmidt=True)
+    E = g.energy_moments(0, 1)[0]
+    psi = g.download(0)
+    g.close()
+    return psi, E
+
+
+def sweep_run(q, F, steps, warmup):
+    import threading
+    ceps, ints, tabs, ntime_full = sweep_fields(F, steps + warmup + 8)
+    names = ["dipole_r", "dipole_a"]
+    classes = []
+    for k, n in enumerate(SWEEP_SIZES):
+        psi0, E0 = sweep_ground_state(q, n)
+        g = q.Grid((n, n), SWEEP_DR, "c64", n_slots=SWEEP_PER)
+        g.set_potential("H_2D", softcore=1.0)
+        g.set_integrator("BM4", DT)
+        sl = slice(k * SWEEP_PER, (k + 1) * SWEEP_PER)
+        g.set_field_batch(np.stack([t[0] for t in tabs[sl]]), np.stack([t[1] for t in tabs[sl]]),
+                          np.stack([t[2] for t in tabs[sl]]))
+        for b in range(SWEEP_PER):
+            g.upload(b, psi0)
+        from tdse_nd_mpi_amd import tuning
+        best, table = tuning.autotune_batch(g, 0, SWEEP_PER, "length", 0, nsteps=4)
+        g.real_steps_batch(0, SWEEP_PER, "length", 0, warmup, names)
+        classes.append(dict(n=n, grid=g, E0=E0, best=best, table=table))
+    out = [None] * len(classes)
+
+    def work(i):
+        out[i] = classes[i]["grid"].real_steps_batch(0, SWEEP_PER, "length", warmup, steps, names)
+
+    import torch
+    torch.cuda.synchronize()
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(classes))]
+    t0 = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    pts = sum(SWEEP_PER * c["n"] ** 2 for c in classes)
+    value = pts * steps / wall
+    res = {"metric": "grid-point-steps/s (all 96 instances)", "value": value, "wall_s": wall, "steps": steps,
+           "steps_full_pulse": ntime_full, "points_per_step": pts, "seed": SWEEP_SEED,
+           "step_roofline_frac": 14 * 2 * 8 * value / (HBM_PEAK_GBS * 1e9),
+           "classes": [{"grid": [c["n"], c["n"]], "instances": SWEEP_PER, "E0": c["E0"],
+                        "tuned_vpt_twc_tws": list(c["best"]),
+                        "tuning_ms_per_step": {str(list(k)): round(v, 4) for k, v in c["table"]}}
+                       for c in classes],
+           "finite": bool(all(np.isfinite(o).all() for o in out))}
+    for c in classes:
+        c["grid"].close()
+    return res
+
+
 def spectrum(a, dt):
     w = np.hanning(len(a))
     return np.abs(np.fft.rfft((a - a.mean()) * w)) ** 2
@@ -130,6 +217,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--sweep-steps", type=int, default=2000,
+                    help="steps of the 96-instance CEP/intensity sweep (0 skips it)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for counter runs: a 256 MiB device copy (calibration), then 20 c64 steps; no JSON")
     ap.add_argument("--dump", default="", help="write the observable traces to this .npz")
@@ -240,6 +329,10 @@ def main():
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     out["imag_ground_state"] = {"E0": float(E0[0]), "steps": n_imag, "seconds": t_imag,
                                 "grid_point_steps_per_s_fp64": N * n_imag / t_imag}
+    if args.sweep_steps > 0:
+        log(f"CEP/intensity sweep: 96 instances, {args.sweep_steps} steps ...")
+        out["sweep_96"] = sweep_run(q, F, args.sweep_steps, 10)
+        log(f"  sweep: {out['sweep_96']['value']:.3e} grid-point-steps/s")
     print(json.dumps(out))
 
 

@@ -132,6 +132,14 @@ int qnd_real_steps(qnd_ctx *ctx, int slot, int gauge, int64_t step0,
                    int64_t nsteps, const int *codes, int ncodes,
                    double *obs_out);
 
+/* Launch tuning (no reference counterpart: FFTW's planner plays this role,
+ * src/meshgrid.c:166-167 with FFTW_MEASURE).  vpt = values per thread of a
+ * power-of-two line (4 or 8), tw_* = lines per workgroup for contiguous and
+ * strided passes; 0 = automatic.  Any value gives the same results up to
+ * rounding; the library clamps them to what a line and the LDS allow.
+ * tdse_nd_mpi_amd.tuning times candidates on the live state. */
+int qnd_set_tuning(qnd_ctx *ctx, int vpt, int tw_contig, int tw_strided);
+
 /* Batched real time (no reference counterpart: the reference runs one
  * field per process; this serves CEP / intensity sweeps).  Instance b of a
  * batch is slot slot0+b, propagated under its own field:

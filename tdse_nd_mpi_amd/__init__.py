@@ -28,7 +28,7 @@ EXPORTS = ("qnd_create", "qnd_destroy", "qnd_last_error", "qnd_set_potential", "
            "qnd_gram_schmidt", "qnd_transform", "qnd_energy_moments", "qnd_observables", "qnd_set_field",
            "qnd_real_steps", "qnd_final_observables", "qnd_event_record", "qnd_event_elapsed",
            "qnd_synchronize", "qnd_profile_enable", "qnd_profile_read", "qnd_version",
-           "qnd_set_field_batch", "qnd_real_steps_batch")
+           "qnd_set_field_batch", "qnd_real_steps_batch", "qnd_set_tuning")
 
 _lib = None
 
@@ -68,6 +68,7 @@ def library():
     L.qnd_set_field.argtypes = [vp, C.c_int64, dp, dp, dp, dp]
     L.qnd_real_steps.argtypes = [vp, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
     L.qnd_set_field_batch.argtypes = [vp, C.c_int, C.c_int64, dp, dp, dp]
+    L.qnd_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.qnd_real_steps_batch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
     L.qnd_final_observables.argtypes = [vp, C.c_int, dp, dp, C.c_int, C.c_double, dp, dp]
     L.qnd_event_record.argtypes = [vp, C.c_int]
@@ -207,6 +208,11 @@ class Grid:
             codes.ctypes.data_as(C.POINTER(C.c_int)) if len(codes) else None, len(codes),
             _d(out) if width else None))
         return out[:nsteps, :width]
+
+    def set_tuning(self, vpt=0, tw_contig=0, tw_strided=0):
+        """Launch geometry overrides (0 = automatic); see tdse_nd_mpi_amd.tuning."""
+        self._ck(library().qnd_set_tuning(self.h, vpt, tw_contig, tw_strided))
+        self.tuning = (vpt, tw_contig, tw_strided)
 
     def set_field_batch(self, Et, At, Eobs=None):
         """Per-instance field tables: Et, At [nb][ntime][s][dim], Eobs [nb][ntime][dim]."""

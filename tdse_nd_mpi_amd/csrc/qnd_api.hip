@@ -1445,6 +1445,18 @@ int qnd_real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nstep
     return DISPATCH(c, real_steps, c, slot, gauge, step0, nsteps, codes, ncodes, obs_out);
 }
 
+int qnd_set_tuning(qnd_ctx *c, int vpt, int tw_contig, int tw_strided)
+{
+    if (!c) return QND_EINVAL;
+    if (vpt != 0 && vpt != 4 && vpt != 8) return fail(c, QND_EINVAL, "values per thread must be 0, 4 or 8");
+    if (tw_contig < 0 || tw_strided < 0 || tw_contig > 256 || tw_strided > 256)
+        return fail(c, QND_EINVAL, "lines per workgroup must be in [0, 256]");
+    c->vpt_override = vpt;
+    c->tw_override[0] = tw_contig;
+    c->tw_override[1] = tw_strided;
+    return 0;
+}
+
 int qnd_set_field_batch(qnd_ctx *c, int nb, int64_t ntime, const double *Et, const double *At,
                         const double *Eobs)
 {

@@ -703,7 +703,9 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
     C<R> *bufB = lds + (TW + l) * ldsS;
     int wc = 0;
 
-    // ---- load x[m] = element t + m T (with the Gram-Schmidt update prologue) ----
+    // ---- load x[m] = element t + m T (with the Gram-Schmidt update prologue).
+    // Twiddles are read from the table in memory (L1/L2 hits); staging it in
+    // LDS measured slower. ----
     C<R> x[VPT];
 #pragma unroll
     for (int m = 0; m < VPT; ++m) x[m] = a.data[lbase + (int64_t)(t + m * T) * estride];

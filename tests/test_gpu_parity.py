@@ -297,3 +297,33 @@ def test_batched_real_steps(gpu, nr, gauge, prec):
         ref_psi, ref_obs = o.real_run(psi0, gauge, *fields[0], names)
         assert rel(got[0], ref_psi) < TOL128, rel(got[0], ref_psi)
         assert np.allclose(obs[0], ref_obs, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("nr", [(64, 128), (48, 96)])
+def test_tuning_geometries_agree(gpu, nr):
+    """Every launch geometry the autotuner may pick gives the same step up to
+    FP64 rounding (complex128), and the tuner restores the state it timed on."""
+    from tdse_nd_mpi_amd import tuning
+    print(f"case tuning nr={nr}", flush=True)
+    dt, nt, dr = 0.05, 8, 0.4
+    o = O.Oracle(nr, dr, "H_2D", "BM4", dt)
+    E, A, Eo, Ao = field_tables(o, nt + 8, 2)
+    psi0 = smooth_state(o, 2)[0]
+    g = q.Grid(nr, dr, "c128", n_slots=1)
+    g.set_potential("H_2D")
+    g.set_integrator("BM4", dt)
+    g.set_field(E, A, Eo, Ao)
+    ref = None
+    for cand in tuning.candidates(nr, "c128"):
+        g.set_tuning(*cand)
+        g.upload(0, psi0)
+        obs = g.real_steps(0, "length", 0, nt, ["dipole_r", "norm"])
+        got = g.download(0)
+        if ref is None:
+            ref = (got, obs)
+        assert rel(got, ref[0]) < 1e-13, (cand, rel(got, ref[0]))
+        assert np.allclose(obs, ref[1], rtol=1e-12, atol=1e-14), cand
+    g.upload(0, psi0)
+    best, table = tuning.autotune_batch(g, 0, 1, "length", 0, nsteps=2, cands=tuning.candidates(nr, "c128")[:3])
+    assert rel(g.download(0), psi0) < 1e-15 and len(table) == 3
+    g.close()
