@@ -132,6 +132,24 @@ int qnd_real_steps(qnd_ctx *ctx, int slot, int gauge, int64_t step0,
                    int64_t nsteps, const int *codes, int ncodes,
                    double *obs_out);
 
+/* Adjoint gradients (no reference counterpart: the reference has no
+ * differentiation).  qnd_forward_checkpointed runs nsteps length-gauge steps
+ * of slot from step 0 (dim >= 2), recording per step the dipole
+ * acceleration and the norm (obs_out [nsteps][dim+1], may be NULL) and the
+ * norm after the final owed mask; the state is checkpointed on the device
+ * every `every` steps.  qnd_adjoint then back-propagates
+ *     L = f(a_0 .. a_{nsteps-1}) + g(norm_final)
+ * given dL_da [nsteps][dim] and dL_dnorm: the costate runs backwards through
+ * the conjugated passes, each checkpoint segment is recomputed once, and it
+ * returns dL/dEt [nsteps][s][dim] (the stage fields of qnd_set_field),
+ * dL/dEobs [nsteps][dim] (the E(t_n) inside a_n) and dL/d(soft-core length)
+ * (the initial state held fixed). */
+int qnd_forward_checkpointed(qnd_ctx *ctx, int slot, int64_t nsteps,
+                             int64_t every, double *obs_out,
+                             double *norm_final);
+int qnd_adjoint(qnd_ctx *ctx, const double *dL_da, double dL_dnorm,
+                double *dL_dEt, double *dL_dEobs, double *dL_dsoft);
+
 /* Launch tuning (no reference counterpart: FFTW's planner plays this role,
  * src/meshgrid.c:166-167 with FFTW_MEASURE).  vpt = values per thread of a
  * power-of-two line (4 or 8), tw_* = lines per workgroup for contiguous and

@@ -28,7 +28,8 @@ EXPORTS = ("qnd_create", "qnd_destroy", "qnd_last_error", "qnd_set_potential", "
            "qnd_gram_schmidt", "qnd_transform", "qnd_energy_moments", "qnd_observables", "qnd_set_field",
            "qnd_real_steps", "qnd_final_observables", "qnd_event_record", "qnd_event_elapsed",
            "qnd_synchronize", "qnd_profile_enable", "qnd_profile_read", "qnd_version",
-           "qnd_set_field_batch", "qnd_real_steps_batch", "qnd_set_tuning")
+           "qnd_set_field_batch", "qnd_real_steps_batch", "qnd_set_tuning", "qnd_forward_checkpointed",
+           "qnd_adjoint")
 
 _lib = None
 
@@ -69,6 +70,8 @@ def library():
     L.qnd_real_steps.argtypes = [vp, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
     L.qnd_set_field_batch.argtypes = [vp, C.c_int, C.c_int64, dp, dp, dp]
     L.qnd_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.qnd_forward_checkpointed.argtypes = [vp, C.c_int, C.c_int64, C.c_int64, dp, dp]
+    L.qnd_adjoint.argtypes = [vp, dp, C.c_double, dp, dp, dp]
     L.qnd_real_steps_batch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
     L.qnd_final_observables.argtypes = [vp, C.c_int, dp, dp, C.c_int, C.c_double, dp, dp]
     L.qnd_event_record.argtypes = [vp, C.c_int]
@@ -213,6 +216,28 @@ class Grid:
         """Launch geometry overrides (0 = automatic); see tdse_nd_mpi_amd.tuning."""
         self._ck(library().qnd_set_tuning(self.h, vpt, tw_contig, tw_strided))
         self.tuning = (vpt, tw_contig, tw_strided)
+
+    def forward_checkpointed(self, slot, nsteps, every):
+        """Length-gauge steps 0..nsteps-1 with checkpoints every `every` steps;
+        returns (dipole acceleration [nsteps][dim], norms [nsteps], final norm)."""
+        out = np.zeros((nsteps, self.dim + 1))
+        nf = C.c_double(0.0)
+        self._ck(library().qnd_forward_checkpointed(self.h, slot, int(nsteps), int(every), _d(out),
+                                                    C.cast(C.pointer(nf), C.POINTER(C.c_double))))
+        return out[:, :self.dim], out[:, self.dim], nf.value
+
+    def adjoint(self, dL_da, dL_dnorm):
+        """Back-propagate dL/da [nsteps][dim] and dL/dnorm_final through the
+        last forward_checkpointed run: (dL/dEt [nsteps][s][dim],
+        dL/dEobs [nsteps][dim], dL/d soft-core length)."""
+        dL_da = np.ascontiguousarray(dL_da, dtype=np.float64)
+        nsteps = dL_da.shape[0]
+        dEt = np.zeros((nsteps, self.s, self.dim))
+        dEo = np.zeros((nsteps, self.dim))
+        ds = C.c_double(0.0)
+        self._ck(library().qnd_adjoint(self.h, _d(dL_da), float(dL_dnorm), _d(dEt), _d(dEo),
+                                       C.cast(C.pointer(ds), C.POINTER(C.c_double))))
+        return dEt, dEo, ds.value
 
     def set_field_batch(self, Et, At, Eobs=None):
         """Per-instance field tables: Et, At [nb][ntime][s][dim], Eobs [nb][ntime][dim]."""

@@ -78,6 +78,14 @@ struct qnd_ctx {
     std::vector<double> Et, At, Eobs, Aobs;
     int64_t ntime = 0;
     double *d_bEt = nullptr, *d_bAt = nullptr, *d_bEo = nullptr;  // batch field tables (device)
+    // adjoint: checkpoints, saved position-space states, partials
+    void *d_adj_ckpt = nullptr, *d_adj_final = nullptr, *d_adj_save = nullptr;
+    double *d_adj_part = nullptr, *d_adj_gsum = nullptr;
+    size_t adj_ckpt_cap = 0, adj_save_cap = 0, adj_part_cap = 0;
+    std::vector<SlotState> adj_ckpt_state;
+    std::vector<double> adj_obs;
+    int adj_slot = -1, adj_prec = -1;
+    int64_t adj_nsteps = 0, adj_K = 0;
     int bnb = 0;
     int64_t bntime = 0;
     hipEvent_t ev[16] = {};
@@ -945,6 +953,228 @@ int real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nsteps, c
     return real_steps_impl<R>(c, slot, 1, gauge, step0, nsteps, codes, ncodes, obs_out);
 }
 
+// ---------------------------------------------------------------------
+// Adjoint of the length-gauge propagation (dim >= 2)
+//
+// Forward step n, as real_steps runs it on the axis-0 Fourier
+// representation M:  B_n, K(k_0), P_0, K(k_1), ..., P_{s-1}, K(k_s), with
+//   B_n = F0 Mask F0^-1         (the owed mask; observables of y_n = Mask F0^-1 M)
+//   P_i = F0 Phi_i F0^-1,       Phi_i = exp(-c_i Vcap - i c_i (V + x.E_{n,i}))
+//   K(k) = the all-k kinetic factor exp(-i k Kin) between axis-1.. transforms.
+// Every factor's adjoint has its own form with the phase conjugated
+// (F0^dagger = n0 F0^-1), so the costate runs backwards through the same
+// passes with PF_CONJ.  With g = dL/dM* (dL = 2 Re <g, dM>):
+//   dL/dE_{n,i,j}  = 2 n0 c_i sum_r x_j Im(conj(u) phi)      u = F0^-1 g after P_i,
+//                                                           phi = Phi_i F0^-1 M (saved)
+//   B_n^dagger g + source = F0 Mask (F0^-1 g + sum_j s_j dNr/n0 (w_j - E_j) y_n)
+// for the dipole acceleration a_{n,j} = dNr sum (w_j - E_j)|y_n|^2, w = -grad V,
+// s = dL/da; the final norm enters as a source dNr g_norm / n0 * y_N.
+// Forward states are checkpointed every K steps; each segment is recomputed
+// once, saving y_n and phi_{n,i} in position space (s+1 states per step).
+// ---------------------------------------------------------------------
+template <typename R> int adj_forward(qnd_ctx *c, int slot, int64_t nsteps, int64_t K, double *obs_out,
+                                      double *norm_final)
+{
+    const int d = c->dim;
+    int rc;
+    const int64_t nseg = (nsteps + K - 1) / K;
+    const size_t sbytes = (size_t)c->N * sizeof(C<R>);
+    if (c->adj_ckpt_cap < (size_t)nseg) {
+        if (c->d_adj_ckpt) HIPCK(c, hipFree(c->d_adj_ckpt));
+        c->d_adj_ckpt = nullptr;
+        HIPCK(c, hipMalloc(&c->d_adj_ckpt, sbytes * nseg));
+        c->adj_ckpt_cap = nseg;
+    }
+    if (!c->d_adj_final) HIPCK(c, hipMalloc(&c->d_adj_final, sbytes));
+    c->adj_ckpt_state.assign(nseg, SlotState());
+    c->adj_obs.assign((size_t)nsteps * (d + 1), 0.0);
+    const int codes[2] = {QND_OBS_DIPOLE_A, QND_OBS_NORM};
+    C<R> *psi = state_ptr<R>(c, slot);
+    for (int64_t sg = 0; sg < nseg; ++sg) {
+        const int64_t n0 = sg * K, len = (n0 + K <= nsteps) ? K : nsteps - n0;
+        HIPCK(c, hipMemcpyAsync(reinterpret_cast<C<R> *>(c->d_adj_ckpt) + sg * c->N, psi, sbytes,
+                                hipMemcpyDeviceToDevice, c->stream));
+        c->adj_ckpt_state[sg] = c->slots[slot];
+        if ((rc = real_steps<R>(c, slot, QND_GAUGE_LENGTH, n0, len, codes, 2, c->adj_obs.data() + n0 * (d + 1))))
+            return rc;
+    }
+    // the final boundary: owed mask, then the norm of y_N (kept for the costate)
+    double nf = 0.0;
+    const int cn = QND_OBS_NORM;
+    if ((rc = observables<R>(c, slot, &cn, 1, nullptr, nullptr, &nf))) return rc;
+    HIPCK(c, hipMemcpyAsync(c->d_adj_final, psi, sbytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    c->adj_slot = slot;
+    c->adj_nsteps = nsteps;
+    c->adj_K = K;
+    c->adj_prec = c->prec;
+    if (obs_out) memcpy(obs_out, c->adj_obs.data(), c->adj_obs.size() * sizeof(double));
+    if (norm_final) *norm_final = nf;
+    return 0;
+}
+
+template <typename R>
+int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dEt, double *dL_dEobs, double *dL_dsoft)
+{
+    const int d = c->dim, s = c->s;
+    const int64_t nsteps = c->adj_nsteps, K = c->adj_K, nseg = (nsteps + K - 1) / K;
+    const double n0d = (double)c->nr[0], dNr = c->P.dNr, inv_n0 = 1.0 / n0d;
+    int rc;
+    const size_t sbytes = (size_t)c->N * sizeof(C<R>);
+    const Geo g0 = geometry(c, 0);
+    const int nwg = g0.grid, per_step = s + 1;
+    // buffers: costate, work state, saved position-space states, partials
+    const size_t nsave = (size_t)K * per_step;
+    if (c->adj_save_cap < nsave) {
+        if (c->d_adj_save) HIPCK(c, hipFree(c->d_adj_save));
+        c->d_adj_save = nullptr;
+        HIPCK(c, hipMalloc(&c->d_adj_save, sbytes * nsave));
+        c->adj_save_cap = nsave;
+    }
+    const size_t npart = nsave * nwg * QND_NOBS;
+    if (c->adj_part_cap < npart) {
+        if (c->d_adj_part) HIPCK(c, hipFree(c->d_adj_part));
+        if (c->d_adj_gsum) HIPCK(c, hipFree(c->d_adj_gsum));
+        c->d_adj_part = c->d_adj_gsum = nullptr;
+        HIPCK(c, hipMalloc(&c->d_adj_part, npart * sizeof(double)));
+        HIPCK(c, hipMalloc(&c->d_adj_gsum, nsave * QND_NOBS * sizeof(double)));
+        c->adj_part_cap = npart;
+    }
+    C<R> *lam = reinterpret_cast<C<R> *>(c->d_scr[0]);
+    C<R> *W = reinterpret_cast<C<R> *>(c->d_scr[1]);
+    C<R> *saves = reinterpret_cast<C<R> *>(c->d_adj_save);
+    std::vector<double> gsum(nsave * QND_NOBS);
+    for (int64_t k = 0; k < nsteps * s * d; ++k) dL_dEt[k] = 0.0;
+    for (int64_t k = 0; k < nsteps * d; ++k) dL_dEobs[k] = 0.0;
+    double dsoft = 0.0;
+
+    auto pot = [&](C<R> *data) {
+        PassArgs<R> a = base_args<R>(c, data, g0);
+        a.scale1 = inv_n0;
+        a.flags = PF_SECOND;
+        return a;
+    };
+    auto kin_adj = [&](C<R> *data, double ck) -> int {
+        Geo g = geometry(c, d - 1);
+        for (int ax = 1; ax <= d - 2; ++ax)
+            if (int r = plain_pass<R>(c, data, ax, -1, 1.0)) return r;
+        PassArgs<R> a = base_args<R>(c, data, g);
+        a.kmode = KM_PHASE;
+        a.ck_mid = ck;
+        a.scale2 = 1.0 / (double)c->nr[d - 1];
+        a.flags = PF_CONJ;
+        if (int r = launch<R, PK_KIN, -1>(c, g, a, PC_KIN)) return r;
+        for (int ax = d - 2; ax >= 1; --ax)
+            if (int r = plain_pass<R>(c, data, ax, +1, 1.0 / (double)c->nr[ax])) return r;
+        return 0;
+    };
+    const double *as = c->as_scaled;
+    auto kcoef = [&](int i) { return i == 0 ? as[0] : (i == s ? as[2 * s - 1] : as[2 * i - 1] + as[2 * i]); };
+
+    // costate after the final boundary: F0 Mask (dNr g_norm / n0 y_N)
+    HIPCK(c, hipMemsetAsync(lam, 0, sbytes, c->stream));
+    {
+        PassArgs<R> a = pot(lam);
+        a.flags |= PF_SRC | (c->P.mask_on ? PF_MASK : 0u);
+        a.save = reinterpret_cast<C<R> *>(c->d_adj_final);
+        a.src_norm = dL_dnorm * dNr * inv_n0;
+        a.partials = c->d_adj_part;
+        a.npart = QND_NOBS;
+        if ((rc = launch<R, PK_POT, +1>(c, g0, a, PC_POT))) return rc;
+    }
+    for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+        const int64_t nA = sg * K, len = (nA + K <= nsteps) ? K : nsteps - nA;
+        // ---- recompute the segment from its checkpoint, saving y_n and phi_{n,i} ----
+        HIPCK(c, hipMemcpyAsync(W, reinterpret_cast<C<R> *>(c->d_adj_ckpt) + sg * c->N, sbytes,
+                                hipMemcpyDeviceToDevice, c->stream));
+        SlotState st = c->adj_ckpt_state[sg];
+        if (st.rep == REP_R) {
+            if ((rc = plain_pass<R>(c, W, 0, -1, 1.0))) return rc;
+            st.rep = REP_M;
+        }
+        std::vector<char> masked(len, 0);
+        for (int64_t k = 0; k < len; ++k) {
+            const int64_t n = nA + k;
+            const double *Et = &c->Et[(size_t)n * s * d];
+            PassArgs<R> b = pot(W);
+            b.flags |= PF_SAVE | PF_OBS | (st.pend_mask ? PF_MASK : 0u);
+            b.save = saves + (k * per_step) * c->N;
+            b.partials = c->d_part;
+            b.npart = QND_NOBS;
+            masked[k] = st.pend_mask;
+            if ((rc = launch<R, PK_POT, +1>(c, g0, b, PC_POT))) return rc;
+            if ((rc = kinetic_block<R>(c, W, KM_PHASE, kcoef(0), nullptr))) return rc;
+            for (int ia = 0; ia < s; ++ia) {
+                PassArgs<R> p = pot(W);
+                p.flags |= PF_V1 | PF_SAVE;
+                p.vmode = VM_LENGTH;
+                p.cv1 = as[2 * ia] + as[2 * ia + 1];
+                for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
+                p.save = saves + (k * per_step + 1 + ia) * c->N;
+                if ((rc = launch<R, PK_POT, +1>(c, g0, p, PC_POT))) return rc;
+                if ((rc = kinetic_block<R>(c, W, KM_PHASE, kcoef(ia + 1), nullptr))) return rc;
+            }
+            st.pend_mask = c->P.mask_on != 0;
+        }
+        // ---- adjoint sweep over the segment ----
+        for (int64_t k = len - 1; k >= 0; --k) {
+            const int64_t n = nA + k;
+            const double *Et = &c->Et[(size_t)n * s * d];
+            const double *Eo = &c->Eobs[(size_t)n * d];
+            if ((rc = kin_adj(lam, kcoef(s)))) return rc;
+            for (int ia = s - 1; ia >= 0; --ia) {
+                PassArgs<R> p = pot(lam);
+                p.flags |= PF_V1 | PF_CONJ | PF_GRAD;
+                p.vmode = VM_LENGTH;
+                p.cv1 = as[2 * ia] + as[2 * ia + 1];
+                for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
+                p.save = saves + (k * per_step + 1 + ia) * c->N;
+                p.partials = c->d_adj_part + (size_t)(k * per_step + 1 + ia) * nwg * QND_NOBS;
+                p.npart = QND_NOBS;
+                if ((rc = launch<R, PK_POT, +1>(c, g0, p, PC_POT))) return rc;
+                if ((rc = kin_adj(lam, kcoef(ia)))) return rc;
+            }
+            PassArgs<R> b = pot(lam);
+            b.flags |= PF_SRC | (masked[k] ? PF_MASK : 0u);
+            for (int j = 0; j < d; ++j) {
+                b.src[j] = dL_da[n * d + j] * dNr * inv_n0;
+                b.Ev[j] = Eo[j];
+            }
+            b.save = saves + (k * per_step) * c->N;
+            b.partials = c->d_adj_part + (size_t)(k * per_step) * nwg * QND_NOBS;
+            b.npart = QND_NOBS;
+            if ((rc = launch<R, PK_POT, +1>(c, g0, b, PC_POT))) return rc;
+        }
+        // ---- sum the partials of the segment's passes ----
+        const int npass = (int)(len * per_step);
+        {
+            ProfScope ps(c, PC_SMALL);
+            hipLaunchKernelGGL(k_sum_blocks, dim3((npass * QND_NOBS + 3) / 4), dim3(256), 0, c->stream,
+                               (const double *)c->d_adj_part, npass, nwg, QND_NOBS, (double *)c->d_adj_gsum);
+            HIPCK(c, hipGetLastError());
+        }
+        HIPCK(c, hipMemcpyAsync(gsum.data(), c->d_adj_gsum, (size_t)npass * QND_NOBS * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        for (int64_t k = 0; k < len; ++k) {
+            const int64_t n = nA + k;
+            const double *bsum = &gsum[(size_t)(k * per_step) * QND_NOBS];
+            dsoft += n0d * bsum[0];  // explicit dependence of a_n on the soft-core length
+            const double nrm = c->adj_obs[n * (d + 1) + d];
+            for (int j = 0; j < d; ++j) dL_dEobs[n * d + j] = -dL_da[n * d + j] * nrm;
+            for (int ia = 0; ia < s; ++ia) {
+                const double *ps = &gsum[(size_t)(k * per_step + 1 + ia) * QND_NOBS];
+                const double cc = as[2 * ia] + as[2 * ia + 1];
+                dsoft += 2.0 * n0d * cc * ps[0];
+                for (int j = 0; j < d; ++j) dL_dEt[(n * s + ia) * d + j] = 2.0 * n0d * cc * ps[1 + j];
+            }
+        }
+    }
+    if (dL_dsoft) *dL_dsoft = dsoft;
+    prof_flush(c);
+    return 0;
+}
+
 // Write_final_observables numerics (src/io_functions.c:159-199)
 template <typename R>
 int final_observables(qnd_ctx *c, int slot, const double *R_mask, const double *pr_max, int Ne, double de,
@@ -1260,6 +1490,8 @@ void qnd_destroy(qnd_ctx *c)
     if (c->d_bEt) hipFree(c->d_bEt);
     if (c->d_bAt) hipFree(c->d_bAt);
     if (c->d_bEo) hipFree(c->d_bEo);
+    for (void *p : {c->d_adj_ckpt, c->d_adj_final, c->d_adj_save, (void *)c->d_adj_part, (void *)c->d_adj_gsum})
+        if (p) hipFree(p);
     for (int k = 0; k < 16; ++k)
         if (c->ev[k]) hipEventDestroy(c->ev[k]);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -1443,6 +1675,27 @@ int qnd_real_steps(qnd_ctx *c, int slot, int gauge, int64_t step0, int64_t nstep
         return fail(c, QND_EINVAL, "unknown gauge %d", gauge);
     if (ncodes > 0 && (!codes || !obs_out)) return fail(c, QND_EINVAL, "observable codes need an output array");
     return DISPATCH(c, real_steps, c, slot, gauge, step0, nsteps, codes, ncodes, obs_out);
+}
+
+int qnd_forward_checkpointed(qnd_ctx *c, int slot, int64_t nsteps, int64_t every, double *obs_out,
+                             double *norm_final)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if ((rc = check_slot(c, slot))) return rc;
+    if (c->dim < 2) return fail(c, QND_EUNSUPPORTED, "the adjoint needs dim >= 2");
+    if (nsteps < 1 || every < 1) return fail(c, QND_EINVAL, "nsteps and the checkpoint interval must be >= 1");
+    if (nsteps > c->ntime) return fail(c, QND_EINVAL, "%lld steps exceed the field table", (long long)nsteps);
+    return DISPATCH(c, adj_forward, c, slot, nsteps, every, obs_out, norm_final);
+}
+
+int qnd_adjoint(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dEt, double *dL_dEobs,
+                double *dL_dsoft)
+{
+    if (!c || !dL_da || !dL_dEt || !dL_dEobs) return QND_EINVAL;
+    if (c->adj_nsteps < 1 || c->adj_prec != c->prec)
+        return fail(c, QND_ESTATE, "no checkpointed forward run (qnd_forward_checkpointed)");
+    return DISPATCH(c, adj_backward, c, dL_da, dL_dnorm, dL_dEt, dL_dEobs, dL_dsoft);
 }
 
 int qnd_set_tuning(qnd_ctx *c, int vpt, int tw_contig, int tw_strided)

@@ -355,6 +355,21 @@ template <> __device__ __forceinline__ double potential<float>(const Phys &P, do
     return (double)(-g);
 }
 
+// Derivatives with respect to the soft-core length a (c2 = a^2) of V and of
+// dV/du, for the soft-core family V = -(1 + b e^{-u}) / sqrt(u + c2): the
+// adjoint's parameter gradient.  Zero for the Gaussian well.
+__device__ __forceinline__ void softcore_derivs(const Phys &P, double u, double &dV_da, double &dVdu_da)
+{
+    dV_da = 0.0;
+    dVdu_da = 0.0;
+    if (P.pot_kind != 0) return;
+    const double a = sqrt(P.pot_c2), w = u + P.pot_c2;
+    const double g = (P.pot_b != 0.0) ? P.pot_b * exp(-u) : 0.0;
+    const double i1 = 1.0 / sqrt(w), i3 = i1 / w, i5 = i3 / w;
+    dV_da = 2.0 * a * (.5 * (1.0 + g) * i3);
+    dVdu_da = 2.0 * a * (-.5 * g * i3 - .75 * (1.0 + g) * i5);
+}
+
 #define QND_MAX_EXP_ARG 700.0     // src/integrators.h:12
 #define QND_MAX_EXP_ARG_F32 80.0  // FP32 path: e^80 stays finite in float
 
@@ -522,6 +537,11 @@ enum PassFlags : unsigned {
     PF_KPOST = 1u << 7,      // (dim = 1) kinetic factor after the second transform
     PF_GS_DOTS = 1u << 8,    // Gram-Schmidt partial dots after the second transform
     PF_FINALIZE = 1u << 9,   // block 0 first turns the previous pass's observable partials into a record
+    // adjoint (costate) passes
+    PF_SAVE = 1u << 10,      // store the position-space state after the r-space operators into `save`
+    PF_CONJ = 1u << 11,      // conjugate phase factors (damping unchanged): the adjoint of the pass
+    PF_GRAD = 1u << 12,      // partials of x_j Im(conj(u) phi) and dV/da Im(conj(u) phi), phi from `save`
+    PF_SRC = 1u << 13,       // u += sum_j src_j (w_j - E_j) y + src_norm y (y from `save`), w = -grad V
 };
 enum KMode { KM_PHASE = 0, KM_VELOCITY = 1, KM_IMAG = 2 };
 enum VMode { VM_LENGTH = 0, VM_VELOCITY = 1, VM_IMAG = 2 };
@@ -562,6 +582,10 @@ template <typename R> struct PassArgs {
     int vmode;
     double cv1, cv2;
     double Ev[QND_MAXD];
+    // adjoint passes: forward states saved in position space, source weights
+    C<R> *save;
+    double src[QND_MAXD];
+    double src_norm;
     // Gram-Schmidt
     const C<R> *prev;
     int64_t slot_stride;
@@ -731,6 +755,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
     }
 
     const int ax = a.axis;
+    const double cs = (fl & PF_CONJ) ? -1.0 : 1.0;  // adjoint: conjugated phases
     if (KIND == PK_PLAIN) {
         fft_line<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw);
         if (a.scale1 != 1.0) {
@@ -759,7 +784,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                 const double p = coord_p(P, ax, t + m * T);
                 const double K = .5 * p * p;
                 if (a.kmode == KM_IMAG) x[m] = scl<R>(x[m], real_exp<R>(-a.ck_pre * K));
-                else x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, a.ck_pre * K));
+                else x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_pre * K));
             }
         }
 
@@ -793,8 +818,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                 const double K = Ko + .5 * p * p;
                 if (a.kmode == KM_IMAG) x[m] = scl<R>(x[m], real_exp<R>(-a.ck_mid * K));
                 else if (a.kmode == KM_VELOCITY)
-                    x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, a.ck_mid * (K + (Wk + (p + .5 * Aax) * Aax))));
-                else x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, a.ck_mid * K));
+                    x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_mid * (K + (Wk + (p + .5 * Aax) * Aax))));
+                else x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_mid * K));
             }
         } else {
             // all axes in r-space: potential, CAP, field, mask, observables
@@ -815,19 +840,45 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
 #pragma unroll
             for (int q = 0; q < QND_NOBS; ++q) obs[q] = 0.0;
             const double s1 = a.scale1;
-            const bool need_v = (fl & (PF_V1 | PF_V2 | PF_OBS)) != 0;
+            const bool need_v = (fl & (PF_V1 | PF_V2 | PF_OBS | PF_SRC)) != 0;
 #pragma unroll
             for (int m = 0; m < VPT; ++m) {
                 C<R> v = scl<R>(x[m], (R)s1);
+                const int64_t gi = lbase + (int64_t)(t + m * T) * estride;
                 const double xc = coord_x(P, ax, t + m * T);
                 double dVdu = 0.0, V = 0.0;
                 if (need_v) V = potential<R>(P, xc * xc + rro, dVdu);
                 const double Vc = P.cap_on ? (capv[m] + Vco) : 0.0;
+                if (fl & (PF_GRAD | PF_SRC)) {
+                    // adjoint: u = costate here; phi / y = forward state saved here
+                    const C<R> f = a.save[gi];
+                    double dV_da, dVdu_da;
+                    softcore_derivs(P, xc * xc + rro, dV_da, dVdu_da);
+                    if (fl & PF_GRAD) {
+                        const double im = (double)v.x * (double)f.y - (double)v.y * (double)f.x;
+                        obs[0] += dV_da * im;
+#pragma unroll
+                        for (int j = 0; j < QND_MAXD; ++j)
+                            if (j < P.dim) obs[1 + j] += ((j == ax) ? xc : xo[j]) * im;
+                    } else {
+                        double wsum = a.src_norm, dwsum = 0.0;
+#pragma unroll
+                        for (int j = 0; j < QND_MAXD; ++j) {
+                            if (j < P.dim) {
+                                const double xj = (j == ax) ? xc : xo[j];
+                                wsum += a.src[j] * (-2.0 * xj * dVdu - fv[j]);
+                                dwsum += a.src[j] * (-2.0 * xj * dVdu_da);
+                            }
+                        }
+                        v = mk<R>((R)((double)v.x + wsum * (double)f.x), (R)((double)v.y + wsum * (double)f.y));
+                        obs[0] += dwsum * ((double)f.x * (double)f.x + (double)f.y * (double)f.y);
+                    }
+                }
                 if (fl & PF_V1) {
                     if (a.vmode == VM_IMAG) v = scl<R>(v, real_exp<R>(-a.cv1 * V));
                     else if (a.vmode == VM_LENGTH)
-                        v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, a.cv1 * (V + (xc * fv[ax] + Wv))));
-                    else v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, a.cv1 * V));
+                        v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, cs * a.cv1 * (V + (xc * fv[ax] + Wv))));
+                    else v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, cs * a.cv1 * V));
                 }
                 if ((fl & PF_MASK) && P.mask_on) v = scl<R>(v, (R)(mskv[m] * mko));
                 if (fl & PF_OBS) {
@@ -844,11 +895,13 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                 }
                 if (fl & PF_V2) {
                     if (a.vmode == VM_IMAG) v = scl<R>(v, real_exp<R>(-a.cv2 * V));
-                    else v = fac_mul<R>(v, phase_factor<R>(-a.cv2 * Vc, a.cv2 * V));
+                    else v = fac_mul<R>(v, phase_factor<R>(-a.cv2 * Vc, cs * a.cv2 * V));
                 }
+                if (fl & PF_SAVE) a.save[gi] = v;
                 x[m] = v;
             }
-            if (fl & PF_OBS) block_sum_vec<QND_NOBS>(obs, red, a.partials + (int64_t)tb * a.npart);
+            if (fl & (PF_OBS | PF_GRAD | PF_SRC))
+                block_sum_vec<QND_NOBS>(obs, red, a.partials + (int64_t)tb * a.npart);
         }
 
         // ---- second transform ----
@@ -863,8 +916,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                     const C<R> v = scl<R>(x[m], s2);
                     if (a.kmode == KM_IMAG) x[m] = scl<R>(v, real_exp<R>(-a.ck_post * K));
                     else if (a.kmode == KM_VELOCITY)
-                        x[m] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * (K + (p + .5 * fv[ax]) * fv[ax])));
-                    else x[m] = fac_mul<R>(v, phase_factor<R>(0.0, a.ck_post * K));
+                        x[m] = fac_mul<R>(v, phase_factor<R>(0.0, cs * a.ck_post * (K + (p + .5 * fv[ax]) * fv[ax])));
+                    else x[m] = fac_mul<R>(v, phase_factor<R>(0.0, cs * a.ck_post * K));
                 }
             } else if (s2 != (R)1) {
 #pragma unroll
@@ -913,6 +966,19 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__
     for (int q = blockIdx.x * nwv + wv; q < nq; q += gridDim.x * nwv) {
         const double s = wave_sum_column(part, nwg, npart, q);
         if ((threadIdx.x & 63) == 0) out[q] = s * scale;
+    }
+}
+
+// out[p][q] = sum_w part[p][w][q] for npass blocks of [nwg][npart] partials,
+// one wavefront per (p, q), fixed order
+__global__ void __launch_bounds__(256) k_sum_blocks(const double *__restrict__ part, int npass, int nwg, int npart,
+                                                    double *out)
+{
+    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    for (int pq = blockIdx.x * nwv + wv; pq < npass * npart; pq += gridDim.x * nwv) {
+        const int p = pq / npart, q = pq - p * npart;
+        const double s = wave_sum_column(part + (int64_t)p * nwg * npart, nwg, npart, q);
+        if ((threadIdx.x & 63) == 0) out[pq] = s;
     }
 }
 

@@ -327,3 +327,84 @@ def test_tuning_geometries_agree(gpu, nr):
     best, table = tuning.autotune_batch(g, 0, 1, "length", 0, nsteps=2, cands=tuning.candidates(nr, "c128")[:3])
     assert rel(g.download(0), psi0) < 1e-15 and len(table) == 3
     g.close()
+
+
+def _adjoint_problem(prec, nr=(64, 64), nsteps=40, every=8):
+    from tdse_nd_mpi_amd import adjoint as adj
+    rng = np.random.default_rng(7)
+    o = O.Oracle(nr, 0.4, "H_2D", "BM4", 0.05)
+    psi0 = smooth_state(o, 4)[0]
+    psi0 = psi0 / np.sqrt(np.sum(np.abs(psi0) ** 2) * 0.16)
+    t = np.arange(nsteps) * 0.05
+    E = 0.05 * np.sin(0.5 * t) * np.exp(-((t - 1.0) / 1.2) ** 2) + 0.01 * rng.standard_normal(nsteps)
+    spec = adj.Spec(nr, 0.4, "H_2D", "BM4", 0.05, psi0, precision=prec, mask=0.1, every=every)
+    wa = rng.standard_normal((nsteps, 2))
+    return adj, spec, E, wa
+
+
+def _linear_loss(adj, spec, E, soft, wa, gnorm=0.7):
+    a, _, nf = adj.forward_numpy(spec, E, soft)
+    return float(np.sum(wa * a) + gnorm * nf)
+
+
+def test_adjoint_matches_finite_differences(gpu):
+    """dL/dE(t_n) and dL/da (soft-core) of L = sum w.a + 0.7 norm_final,
+    complex128, against central differences of the forward run."""
+    print("case adjoint fd", flush=True)
+    adj, spec, E, wa = _adjoint_problem("c128")
+    soft = 1.0
+    adj.forward_numpy(spec, E, soft)
+    dEt, dEo, ds = spec.grid.adjoint(wa, 0.7)
+    dE = dEt[:, :, 0].sum(axis=1) + dEo[:, 0]
+    h = 1e-4
+    for k in [0, 3, 11, 20, 27, 39]:
+        Ep, Em = E.copy(), E.copy()
+        Ep[k] += h
+        Em[k] -= h
+        fd = (_linear_loss(adj, spec, Ep, soft, wa) - _linear_loss(adj, spec, Em, soft, wa)) / (2 * h)
+        assert abs(fd - dE[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, dE[k])
+    hs = 1e-4
+    fd = (_linear_loss(adj, spec, E, soft + hs, wa) - _linear_loss(adj, spec, E, soft - hs, wa)) / (2 * hs)
+    assert abs(fd - ds) <= 1e-6 * max(1.0, abs(fd)), (fd, ds)
+    spec.close()
+
+
+def test_adjoint_complex64_tracks_complex128(gpu):
+    print("case adjoint c64", flush=True)
+    grads = {}
+    for prec in ("c128", "c64"):
+        adj, spec, E, wa = _adjoint_problem(prec)
+        adj.forward_numpy(spec, E, 1.0)
+        dEt, dEo, ds = spec.grid.adjoint(wa, 0.7)
+        grads[prec] = (dEt[:, :, 0].sum(axis=1) + dEo[:, 0], ds)
+        spec.close()
+    assert rel(grads["c64"][0], grads["c128"][0]) < 1e-4, rel(grads["c64"][0], grads["c128"][0])
+    assert abs(grads["c64"][1] - grads["c128"][1]) < 1e-4 * max(1.0, abs(grads["c128"][1]))
+
+
+def test_autograd_function_harmonic_loss(gpu):
+    """torch.autograd through Propagate with the harmonic-window loss equals
+    central differences of the same loss (complex128)."""
+    import torch
+    print("case adjoint autograd", flush=True)
+    adj, spec, E, _ = _adjoint_problem("c128", nsteps=48, every=16)
+    Et = torch.tensor(E, requires_grad=True)
+    soft = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    a, nf = adj.Propagate.apply(Et, soft, spec)
+    L = adj.harmonic_loss(a, nf, 0.05, 0.8, orders=(2, 5))
+    L.backward()
+
+    def loss(Ev, sv):
+        a2, nf2 = adj.Propagate.apply(torch.tensor(Ev), torch.tensor(sv, dtype=torch.float64), spec)
+        return float(adj.harmonic_loss(a2, nf2, 0.05, 0.8, orders=(2, 5)))
+
+    h = 1e-4
+    for k in [5, 24, 40]:
+        Ep, Em = E.copy(), E.copy()
+        Ep[k] += h
+        Em[k] -= h
+        fd = (loss(Ep, 1.0) - loss(Em, 1.0)) / (2 * h)
+        assert abs(fd - Et.grad[k].item()) <= 1e-6 * max(1e-3, abs(fd)), (k, fd, Et.grad[k].item())
+    fd = (loss(E, 1.0 + h) - loss(E, 1.0 - h)) / (2 * h)
+    assert abs(fd - soft.grad.item()) <= 1e-6 * max(1e-3, abs(fd)), (fd, soft.grad.item())
+    spec.close()
