@@ -182,6 +182,180 @@ def sweep_run(q, F, steps, warmup):
     return res
 
 
+# ---------------------------------------------------------------------
+# Adjoint gradient: H_2D (a = 1) on 256^2, DR 0.3, 4000 BM4 steps at DT 0.05
+# under a seeded smooth random pulse (8 Gaussian-windowed cosines, peak 0.05
+# a.u., x-polarized, held over the stages of a step); loss = power of the
+# dipole-acceleration spectrum in windows around harmonics 11..21 of 800 nm
+# plus the norm lost to the absorber (cos^(1/8) mask); dLoss/dE(t_n) and
+# dLoss/d(soft-core a) in complex64 and complex128 with checkpoints every 64
+# steps, checked against central differences on 16 random E samples (FP64).
+# ---------------------------------------------------------------------
+ADJ_NR = (256, 256)
+ADJ_DR = 0.3
+ADJ_STEPS = 4000
+ADJ_SEED = 4000
+
+
+def adjoint_field(nt):
+    rng = np.random.default_rng(ADJ_SEED)
+    t = np.arange(nt) * DT
+    E = np.zeros(nt)
+    for _ in range(8):
+        A, tk, sk = rng.uniform(0.5, 1.0), rng.uniform(0.2, 0.8) * t[-1], rng.uniform(10.0, 40.0)
+        wk, pk = rng.uniform(0.04, 0.08), rng.uniform(0.0, 2 * np.pi)
+        E += A * np.exp(-(t - tk) ** 2 / (2 * sk * sk)) * np.cos(wk * t + pk)
+    return 0.05 * E / np.abs(E).max()
+
+
+def adjoint_run(q, steps, fd_samples):
+    import torch
+    from tdse_nd_mpi_amd import adjoint as adj
+    g = q.Grid(ADJ_NR, ADJ_DR, "c128", n_slots=1)
+    g.set_potential("H_2D", softcore=1.0)
+    g.set_integrator("BM4", 0.1)
+    g.upload(0, reference_guess(ADJ_NR, ADJ_DR))
+    g.imag_steps(0, q.imag_step_count(100.0, 0.1), gram_schmidt=True)
+    psi0 = g.download(0)
+    g.close()
+    E = adjoint_field(steps)
+    w0 = 45.5637639947958 / 800.0
+    res = {"grid": list(ADJ_NR), "steps": steps, "checkpoint_every": 64, "seed": ADJ_SEED}
+    grads = {}
+    for prec in ("c64", "c128"):
+        spec = adj.Spec(ADJ_NR, ADJ_DR, "H_2D", "BM4", DT, psi0, precision=prec, mask=0.1, every=64)
+        gr = spec.ensure(1.0)
+        gr.set_field(*adj.field_tables(E, gr.s, 2))
+        gr.upload(0, psi0)
+        gr.real_steps(0, "length", 0, 50, ["dipole_a", "norm"])          # warm-up
+        gr.upload(0, psi0)
+        gr.synchronize()
+        t0 = time.perf_counter()
+        gr.real_steps(0, "length", 0, steps, ["dipole_a", "norm"])
+        t_fwd = time.perf_counter() - t0
+        Et = torch.tensor(E, requires_grad=True)
+        soft = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        a, nf = adj.Propagate.apply(Et, soft, spec)
+        L = adj.harmonic_loss(a, nf, DT, w0, orders=(11, 21))
+        L.backward()
+        t_tot = time.perf_counter() - t0
+        grads[prec] = (Et.grad.numpy().copy(), float(soft.grad), float(L.detach()))
+        res[prec] = {"forward_s": t_fwd, "forward_checkpointed_plus_adjoint_s": t_tot,
+                     "adjoint_over_forward": t_tot / t_fwd, "loss": float(L.detach()),
+                     "dL_dsoftcore": float(soft.grad)}
+        if prec == "c128" and fd_samples:
+            rng = np.random.default_rng(ADJ_SEED + 1)
+            ks = np.sort(rng.choice(steps, fd_samples, replace=False))
+            h = 1e-5
+            errs = []
+
+            def loss(Ev):
+                a2, _, nf2 = adj.forward_numpy(spec, Ev, 1.0)
+                return float(adj.harmonic_loss(torch.from_numpy(a2), torch.tensor(nf2), DT, w0, orders=(11, 21)))
+            gscale = np.abs(grads["c128"][0]).max()
+            for k in ks:
+                Ep, Em = E.copy(), E.copy()
+                Ep[k] += h
+                Em[k] -= h
+                fd = (loss(Ep) - loss(Em)) / (2 * h)
+                errs.append(abs(fd - grads["c128"][0][k]) / gscale)
+            res["fd_check"] = {"samples": [int(k) for k in ks], "h": h,
+                               "max_abs_err_over_max_grad": float(max(errs))}
+        spec.close()
+    g64, g128 = grads["c64"][0], grads["c128"][0]
+    res["c64_vs_c128_grad_rel_l2"] = float(np.linalg.norm(g64 - g128) / np.linalg.norm(g128))
+    res["metric"] = "adjoint/forward time ratio (forward with checkpoints + recompute + backward over plain forward)"
+    res["value"] = res["c64"]["adjoint_over_forward"]
+    return res
+
+
+# ---------------------------------------------------------------------
+# 1-D eigenstates: H_1D, NR 4096, DR 0.1, BM4 imaginary time DT 0.1, T 100
+# (1001 steps per state), 4 states with Gram-Schmidt after every step,
+# complex128; guesses: seeded complex Gaussian-random vectors times
+# exp(-x^2/20), and the reference's deterministic guess.  Eigenenergies are
+# compared with the FP64 oracle run on the same guesses.
+# ---------------------------------------------------------------------
+def eig1d_run(q):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    nr, dr, nst, dt = (4096,), 0.1, 4, 0.1
+    nsteps = q.imag_step_count(100.0, dt)
+    res = {"grid": [4096], "states": nst, "steps_per_state": nsteps}
+    o = O.Oracle(nr, dr, "H_1D", "BM4", dt)
+    for mode, name in ((1, "seeded_gaussian"), (0, "reference_guess")):
+        guess = o.initial_guess(nst, guess_mode=mode, seed=1234)
+        g = q.Grid(nr, dr, "c128", n_slots=nst)
+        g.set_potential("H_1D")
+        g.set_integrator("BM4", dt)
+        t0 = time.perf_counter()
+        E = []
+        for j in range(nst):
+            g.upload(j, guess[j])
+            g.imag_steps(j, nsteps, gram_schmidt=True)
+            E.append(g.energy_moments(j, 1)[0])
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        g.close()
+        t0 = time.perf_counter()
+        Eo = o.imag_run(nst, 100.0, guess_mode=mode, seed=1234)[1][:, 0]
+        t_cpu = time.perf_counter() - t0
+        res[name] = {"E_gpu": [float(e) for e in E], "E_oracle": [float(e) for e in Eo],
+                     "max_abs_diff_Ha": float(np.max(np.abs(np.array(E) - np.array(Eo)))),
+                     "gpu_seconds": wall, "grid_point_steps_per_s": 4096 * nst * nsteps / wall,
+                     "oracle_seconds": t_cpu}
+    return res
+
+
+# ---------------------------------------------------------------------
+# Large 3-D: H_3D with soft-core length a = 0.05 (V = -1/sqrt(r^2 + a^2)) on
+# 512^3, DR 0.2, complex64 (1 GiB per state).  Timed: imaginary-time BM6
+# steps (DT 0.05) of state 7 with FP64 Gram-Schmidt against 7 states, then
+# real-time BM4 steps (DT 0.02) under 800 nm 1e14 W/cm^2 plus 400 nm 1e13
+# W/cm^2 at 20 degrees, seeded relative phase, with dipole, acceleration and
+# the cos^(1/8) mask every step.  Few steps (the per-step cost is stationary).
+# ---------------------------------------------------------------------
+def big3d_run(q, F, imag_steps, real_steps):
+    nr, dr = (512, 512, 512), 0.2
+    N = 512 ** 3
+    g = q.Grid(nr, dr, "c64", n_slots=8)
+    g.set_potential("H_3D", softcore=0.05)
+    g.set_integrator("BM6", 0.05)
+    x = -0.5 * 512 * dr + np.arange(512) * dr
+    gauss = np.exp(-(x * x) / 20.0)
+    psi = (gauss[:, None, None] * gauss[None, :, None] * gauss[None, None, :]).astype(np.complex128)
+    for j in range(8):
+        g.upload(j, psi * (1.0 + 0.1 * j * (x[:, None, None] / 10.0)))
+    g.imag_steps(7, 1, gram_schmidt=True)
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.imag_steps(7, imag_steps, gram_schmidt=True)
+    g.synchronize()
+    t_imag = (time.perf_counter() - t0) / imag_steps
+    rng = np.random.default_rng(512)
+    ph = rng.uniform(0, 2 * np.pi)
+    modes = [F.Mode(intensity=1e14, wavelength=800.0, cep=0.0, envelope="cos2", cycles=4),
+             F.Mode(intensity=1e13, wavelength=400.0, cep=ph, envelope="cos2", cycles=8, angle=np.deg2rad(20.0))]
+    g.set_integrator("BM4", 0.02)
+    g.set_mask(0.1)
+    nt = real_steps + 2
+    g.set_field(*F.tables(modes, "BM4", 0.02, nt, 3))
+    g.real_steps(0, "length", 0, 1, ["dipole_r", "dipole_a", "norm"])
+    g.synchronize()
+    t0 = time.perf_counter()
+    obs = g.real_steps(0, "length", 1, real_steps, ["dipole_r", "dipole_a", "norm"])
+    g.synchronize()
+    t_real = (time.perf_counter() - t0) / real_steps
+    g.close()
+    return {"grid": list(nr), "softcore_a": 0.05, "imag_BM6_ms_per_step_state7": t_imag * 1e3,
+            "imag_grid_point_steps_per_s": N / t_imag,
+            "real_BM4_ms_per_step": t_real * 1e3, "real_grid_point_steps_per_s": N / t_real,
+            "real_step_roofline_frac": 14 * 2 * 8 * (N / t_real) / (HBM_PEAK_GBS * 1e9),
+            "second_mode_phase": ph, "finite": bool(np.isfinite(obs).all()),
+            "steps_timed": {"imag": imag_steps, "real": real_steps}}
+
+
 def spectrum(a, dt):
     w = np.hanning(len(a))
     return np.abs(np.fft.rfft((a - a.mean()) * w)) ** 2
@@ -219,6 +393,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--sweep-steps", type=int, default=2000,
                     help="steps of the 96-instance CEP/intensity sweep (0 skips it)")
+    ap.add_argument("--adjoint-steps", type=int, default=ADJ_STEPS,
+                    help="steps of the adjoint-gradient workload (0 skips it)")
+    ap.add_argument("--fd-samples", type=int, default=16)
+    ap.add_argument("--skip-1d", action="store_true", help="skip the 1-D eigenstate workload")
+    ap.add_argument("--big3d-steps", type=int, default=3, help="timed steps of the 512^3 workload (0 skips it)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for counter runs: a 256 MiB device copy (calibration), then 20 c64 steps; no JSON")
     ap.add_argument("--dump", default="", help="write the observable traces to this .npz")
@@ -329,6 +508,18 @@ def main():
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     out["imag_ground_state"] = {"E0": float(E0[0]), "steps": n_imag, "seconds": t_imag,
                                 "grid_point_steps_per_s_fp64": N * n_imag / t_imag}
+    if not args.skip_1d:
+        log("1-D eigenstates: 4096 points, 4 states x 1001 steps, GPU and oracle ...")
+        out["eig1d_4096"] = eig1d_run(q)
+        log(f"  max |E_gpu - E_oracle| = {out['eig1d_4096']['seeded_gaussian']['max_abs_diff_Ha']:.2e} Ha")
+    if args.big3d_steps > 0:
+        log(f"3-D 512^3: {args.big3d_steps} imaginary and real steps ...")
+        out["big3d_512"] = big3d_run(q, F, args.big3d_steps, args.big3d_steps)
+        log(f"  real {out['big3d_512']['real_BM4_ms_per_step']:.2f} ms/step")
+    if args.adjoint_steps > 0:
+        log(f"adjoint gradient: 256^2, {args.adjoint_steps} steps, FD on {args.fd_samples} samples ...")
+        out["adjoint_256"] = adjoint_run(q, args.adjoint_steps, args.fd_samples)
+        log(f"  adjoint/forward = {out['adjoint_256']['value']:.2f}")
     if args.sweep_steps > 0:
         log(f"CEP/intensity sweep: 96 instances, {args.sweep_steps} steps ...")
         out["sweep_96"] = sweep_run(q, F, args.sweep_steps, 10)

@@ -26,14 +26,19 @@ def coefficients(name, dt):
 
 
 class Mode:
+    """One laser mode.  `angle` (radians, extension: the reference fixes the
+    major axis on x, src/parameters.c:942-947) rotates the mode's (x, y)
+    field vector in the xy plane."""
+
     def __init__(self, intensity=1e14, wavelength=800.0, ellipticity=0.0, cep=0.0, delay=0.0,
-                 envelope="cos2", duration=None, cycles=None):
+                 envelope="cos2", duration=None, cycles=None, angle=0.0):
         self.w = AU_TIME_NM / wavelength
         self.F0 = AU_FIELD * np.sqrt(intensity)
         self.xi = ellipticity
         self.phi = cep
         self.delay = delay
         self.env = envelope
+        self.angle = angle
         if duration is None:
             duration = (cycles or 4) * 2 * np.pi / self.w
         self.tau = duration
@@ -70,11 +75,16 @@ def field_vectors(modes, t, dim):
         ts = t - m.delay
         f, df = m.envelope(ts)
         arg = m.w * ts + m.phi
-        E[..., 0] += -m.F0 * (df * np.cos(arg) - f * m.w * np.sin(arg)) / m.w
-        A[..., 0] += m.F0 * f * np.cos(arg) / m.w
+        ex = -m.F0 * (df * np.cos(arg) - f * m.w * np.sin(arg)) / m.w
+        ax = m.F0 * f * np.cos(arg) / m.w
+        ey = -m.F0 * m.xi * (df * np.sin(arg) + f * m.w * np.cos(arg)) / m.w
+        ay = m.F0 * m.xi * f * np.sin(arg) / m.w
+        ca, sa = np.cos(m.angle), np.sin(m.angle)
+        E[..., 0] += ca * ex - sa * ey
+        A[..., 0] += ca * ax - sa * ay
         if dim >= 2:
-            E[..., 1] += -m.F0 * m.xi * (df * np.sin(arg) + f * m.w * np.cos(arg)) / m.w
-            A[..., 1] += m.F0 * m.xi * f * np.sin(arg) / m.w
+            E[..., 1] += sa * ex + ca * ey
+            A[..., 1] += sa * ax + ca * ay
     return A, E
 
 
