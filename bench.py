@@ -63,13 +63,16 @@ def ground_state(q):
     return psi, E, nsteps, t_imag
 
 
-def real_run(q, field, prec, psi0, warmup, steps, Et, At, Eo, Ao, profile_steps=0):
+def real_run(q, field, prec, psi0, warmup, steps, Et, At, Eo, Ao, profile_steps=0, tune=True):
     g = q.Grid(NR, DR, prec, n_slots=1)
     g.set_potential(POT)
     g.set_integrator("BM4", DT)
     g.set_mask(MASK)
     g.set_field(Et, At, Eo, Ao)
     g.upload(0, psi0)
+    from tdse_nd_mpi_amd import tuning
+    best, table = (tuning.autotune_batch(g, 0, 1, "length", 0, nsteps=8)   # restores psi0
+                   if tune else ((0, 0, 0), []))
     names = ["dipole_r", "dipole_a", "norm"]
     obs_w = g.real_steps(0, "length", 0, warmup, names) if warmup else np.zeros((0, 5))
     import torch
@@ -92,7 +95,7 @@ def real_run(q, field, prec, psi0, warmup, steps, Et, At, Eo, Ao, profile_steps=
         g.profile(False)
         prof = (cnt, ms)
     g.close()
-    return np.vstack([obs_w, obs_t]), wall, gpu_ms, prof
+    return np.vstack([obs_w, obs_t]), wall, gpu_ms, prof, (best, table)
 
 
 # ---------------------------------------------------------------------
@@ -252,7 +255,8 @@ def adjoint_run(q, steps, fd_samples):
 
             def loss(Ev):
                 a2, _, nf2 = adj.forward_numpy(spec, Ev, 1.0)
-                return float(adj.harmonic_loss(torch.from_numpy(a2), torch.tensor(nf2), DT, w0, orders=(11, 21)))
+                return float(adj.harmonic_loss(torch.from_numpy(a2), torch.tensor(nf2, dtype=torch.float64), DT,
+                                                 w0, orders=(11, 21)))
             gscale = np.abs(grads["c128"][0]).max()
             for k in ks:
                 Ep, Em = E.copy(), E.copy()
@@ -401,6 +405,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for counter runs: a 256 MiB device copy (calibration), then 20 c64 steps; no JSON")
     ap.add_argument("--dump", default="", help="write the observable traces to this .npz")
+    ap.add_argument("--only", choices=["eig1d", "big3d", "adjoint", "sweep"], default=None,
+                    help="run one secondary workload alone and print its JSON (for kernel-trace profiles)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build holds the whole grid in one MI355X's HBM "
@@ -416,17 +422,25 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     N = int(np.prod(NR))
+    if args.only:
+        runs = {"eig1d": lambda: eig1d_run(q),
+                "big3d": lambda: big3d_run(q, F, max(args.big3d_steps, 1), max(args.big3d_steps, 1)),
+                "adjoint": lambda: adjoint_run(q, args.adjoint_steps, args.fd_samples),
+                "sweep": lambda: sweep_run(q, F, args.sweep_steps, 10)}
+        print(json.dumps({"workload": args.only, "result": runs[args.only]()}))
+        return
     if args.traffic_probe:
-        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")   # 256 MiB
-        dst = torch.empty_like(src)
+        # calibration: an FP64 -> FP32 conversion copy reads 256 MiB and writes 128 MiB
+        src = torch.empty(32 << 20, dtype=torch.float64, device="cuda").uniform_()
+        dst = torch.empty(32 << 20, dtype=torch.float32, device="cuda")
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
         mode = F.Mode(intensity=1e14, wavelength=800.0, ellipticity=0.0, cep=0.0, envelope="cos2", cycles=4)
         Et, At, Eo, Ao = F.tables([mode], "BM4", DT, 100, 2)
         psi0 = reference_guess(NR, DR)
-        real_run(q, F, "c64", psi0 / np.linalg.norm(psi0), 0, 20, Et, At, Eo, Ao)
-        log("traffic probe done: 3 copies of 256 MiB, 20 c64 steps")
+        real_run(q, F, "c64", psi0 / np.linalg.norm(psi0), 0, 20, Et, At, Eo, Ao, tune=False)
+        log("traffic probe done: 3 conversion copies (256 MiB read, 128 MiB written), 20 c64 steps")
         return
 
     log("ground state (imaginary time, FP64) ...")
@@ -441,9 +455,9 @@ def main():
     res = {}
     for prec in (["c64"] if args.skip_fp64 else ["c64", "c128"]):
         log(f"real time {prec}: warmup {args.warmup}, timed {args.steps} ...")
-        obs, wall, gpu_ms, prof = real_run(q, F, prec, psi0, args.warmup, args.steps, Et, At, Eo, Ao,
-                                           profile_steps=args.profile_steps)
-        res[prec] = dict(obs=obs, wall=wall, gpu_ms=gpu_ms, prof=prof)
+        obs, wall, gpu_ms, prof, tuned = real_run(q, F, prec, psi0, args.warmup, args.steps, Et, At, Eo, Ao,
+                                                  profile_steps=args.profile_steps)
+        res[prec] = dict(obs=obs, wall=wall, gpu_ms=gpu_ms, prof=prof, tuned=tuned)
         log(f"  {prec}: {wall * 1e3 / args.steps:.4f} ms/step wall, {gpu_ms / args.steps:.4f} ms/step gpu")
 
     if args.dump:
@@ -472,12 +486,14 @@ def main():
                                "dipole + acceleration every step",
                    "grid": list(NR), "steps_full_pulse": ntime_full, "parallelism": "single GPU"},
         "step_roofline_frac": b_alg_c64 * value / (HBM_PEAK_GBS * 1e9),
+        "tuned_vpt_twc_tws": list(mixed["tuned"][0]),
     }
     if "c128" in res:
         fp64 = res["c128"]
         v64 = N * args.steps / fp64["wall"]
         out["fp64"] = {"value": v64, "ms_per_step": fp64["wall"] * 1e3 / args.steps,
-                       "step_roofline_frac": n_fft * 2 * 16 * v64 / (HBM_PEAK_GBS * 1e9)}
+                       "step_roofline_frac": n_fft * 2 * 16 * v64 / (HBM_PEAK_GBS * 1e9),
+                       "tuned_vpt_twc_tws": list(fp64["tuned"][0])}
         out["mixed_over_fp64"] = value / v64
         a64 = mixed["obs"][:, 2]
         a128 = fp64["obs"][:, 2]

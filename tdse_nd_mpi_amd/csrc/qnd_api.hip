@@ -73,6 +73,7 @@ struct qnd_ctx {
     double *d_maskt = nullptr;   // per-axis mask table (product over axes)
     int tw_override[2] = {0, 0}; // lines per workgroup (contiguous, strided); QND_TW env
     int vpt_override = 0;        // values per thread (4 or 8); QND_VPT env
+    int wave_lines = 1;          // wavefront-local exchanges for 64-thread lines; QND_WAVE_LINES=0 disables
     std::vector<SlotState> slots;
     AxisPlan plan[QND_MAXD];
     std::vector<double> Et, At, Eobs, Aobs;
@@ -271,6 +272,7 @@ template <typename R> PassArgs<R> base_args(qnd_ctx *c, C<R> *data, const Geo &g
     a.TW = g.TW;
     a.log2TW = g.log2TW;
     a.ldsStride = g.ldsStride;
+    a.wave_lines = c->wave_lines;
     a.S = g.S;
     a.nlines = g.nlines;
     a.scale1 = 1.0;
@@ -1422,6 +1424,7 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
             c->tw_override[1] = b;
         }
     }
+    if (const char *e = getenv("QND_WAVE_LINES")) c->wave_lines = atoi(e) != 0;
     if (const char *e = getenv("QND_VPT")) {  // tuning knob: values per thread
         const int v = atoi(e);
         if (v == 4 || v == 8) c->vpt_override = v;

@@ -34,6 +34,17 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("" ::: "memory");
 }
 
+// Exchange fence for a line held by exactly one wavefront: the LDS operations
+// of one wavefront complete in program order, so no workgroup barrier is
+// needed; the fence keeps the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void wave_fence()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // ---------------------------------------------------------------------
 // complex helpers
 // ---------------------------------------------------------------------
@@ -264,7 +275,8 @@ __device__ __forceinline__ void reg_stage(C<R> (&x)[VPT], const C<R> *lin, C<R> 
 // (barriers inside).
 template <typename R, int DIR, int VPT>
 __device__ __forceinline__ void fft_line(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int &wc, int n, int log2n,
-                                         int T, int t, const typename TwT<R>::T *__restrict__ tw)
+                                         int T, int t, const typename TwT<R>::T *__restrict__ tw,
+                                         bool wave_line = false)
 {
     int rad[12], nst = 0;
     if (VPT == 8) {
@@ -289,7 +301,8 @@ __device__ __forceinline__ void fft_line(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB,
         else if (rad[s] == 3) reg_stage<R, 3, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
         else reg_stage<R, 2, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
         if (!last) {
-            lds_barrier();
+            if (wave_line) wave_fence();
+            else lds_barrier();
             lin = lout;
             ++wc;
         }
@@ -573,6 +586,7 @@ template <typename R> struct PassArgs {
     C<R> *data;
     const typename TwT<R>::T *tw;
     int axis, n, log2n, T, log2T, TW, log2TW, ldsStride;
+    int wave_lines;          // allow wavefront-local exchanges for 64-thread contiguous lines
     int64_t S, nlines;
     unsigned flags;
     double scale1, scale2;   // after the first / second transform
@@ -756,8 +770,11 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
 
     const int ax = a.axis;
     const double cs = (fl & PF_CONJ) ? -1.0 : 1.0;  // adjoint: conjugated phases
+    // a contiguous line of exactly 64 threads is one wavefront: its exchanges
+    // need no workgroup barrier
+    const bool wave_line = contig && T == 64 && a.wave_lines;
     if (KIND == PK_PLAIN) {
-        fft_line<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw);
+        fft_line<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, wave_line);
         if (a.scale1 != 1.0) {
             const R s1 = (R)a.scale1;
 #pragma unroll
@@ -789,7 +806,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
         }
 
         // ---- first transform ----
-        fft_line<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw);
+        fft_line<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, wave_line);
 
         // this instance's stage field (E in the length gauge, A in the velocity gauge)
         double fv[QND_MAXD];
@@ -906,7 +923,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
 
         // ---- second transform ----
         if (KIND == PK_KIN || (fl & PF_SECOND)) {
-            fft_line<R, -DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw);
+            fft_line<R, -DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, wave_line);
             const R s2 = (R)a.scale2;
             if (KIND == PK_POT1D && (fl & PF_KPOST)) {
 #pragma unroll

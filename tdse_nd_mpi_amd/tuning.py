@@ -19,7 +19,7 @@ def candidates(nr, precision):
     pow2 = all(n & (n - 1) == 0 for n in nr)
     vpts = [4, 8] if pow2 else [0]
     tw_c = [0, 1, 2, 4]
-    tw_s = [0, 2, 4, 8, 16] if precision in ("c64", 1) else [0, 2, 4, 8]
+    tw_s = [0, 1, 2, 4, 8, 16] if precision in ("c64", 1) else [0, 1, 2, 4, 8]
     if len(nr) == 1:
         tw_s = [0]
     return list(itertools.product(vpts, tw_c, tw_s))
