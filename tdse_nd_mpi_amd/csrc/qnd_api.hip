@@ -55,6 +55,7 @@ struct qnd_ctx {
     int device = 0;
     size_t lds_max = 65536;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // second stream (default priority): adjoint recompute overlap
     Phys P{};
     bool have_pot = false, have_int = false;
     int s = 0;
@@ -81,7 +82,7 @@ struct qnd_ctx {
     double *d_bEt = nullptr, *d_bAt = nullptr, *d_bEo = nullptr;  // batch field tables (device)
     // adjoint: checkpoints, saved position-space states, partials
     void *d_adj_ckpt = nullptr, *d_adj_final = nullptr, *d_adj_save = nullptr;
-    double *d_adj_part = nullptr, *d_adj_gsum = nullptr;
+    double *d_adj_part = nullptr;
     size_t adj_ckpt_cap = 0, adj_save_cap = 0, adj_part_cap = 0;
     std::vector<SlotState> adj_ckpt_state;
     std::vector<double> adj_obs;
@@ -354,7 +355,7 @@ struct Finalize {
 int launch_finalize(qnd_ctx *c, const Finalize *fin)
 {
     ProfScope ps(c, PC_SMALL);
-    hipLaunchKernelGGL(k_obs_finalize, dim3(fin->B.nb), dim3(64), 0, c->stream, fin->part, fin->nwg, c->dim,
+    hipLaunchKernelGGL(k_obs_finalize, dim3(fin->B.nb), dim3(1024), 0, c->stream, fin->part, fin->nwg, c->dim,
                        c->P.dNr, fin->spec, fin->B, fin->rec);
     HIPCK(c, hipGetLastError());
     return 0;
@@ -670,7 +671,7 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
         if ((rc = launch_pot<R>(c, g0, a))) return rc;
         if (gs) {
             ProfScope ps(c, PC_SMALL);
-            hipLaunchKernelGGL(k_gs_coeffs, dim3(1), dim3(256), 0, c->stream, c->d_part, g0.grid, npart,
+            hipLaunchKernelGGL(k_gs_coeffs, dim3(1), dim3(1024), 0, c->stream, c->d_part, g0.grid, npart,
                                nprev, c->P.dNr * inv_n0, c->d_small + SMALL_GRAM, coef);
             HIPCK(c, hipGetLastError());
             pending = true;
@@ -1025,31 +1026,41 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
     const size_t sbytes = (size_t)c->N * sizeof(C<R>);
     const Geo g0 = geometry(c, 0);
     const int nwg = g0.grid, per_step = s + 1;
-    // buffers: costate, work state, saved position-space states, partials
+    // double-buffered saves and partials: the recompute of segment k-1 (second
+    // stream) overlaps the adjoint sweep of segment k (first stream)
     const size_t nsave = (size_t)K * per_step;
-    if (c->adj_save_cap < nsave) {
+    if (c->adj_save_cap < 2 * nsave) {
         if (c->d_adj_save) HIPCK(c, hipFree(c->d_adj_save));
         c->d_adj_save = nullptr;
-        HIPCK(c, hipMalloc(&c->d_adj_save, sbytes * nsave));
-        c->adj_save_cap = nsave;
+        HIPCK(c, hipMalloc(&c->d_adj_save, sbytes * 2 * nsave));
+        c->adj_save_cap = 2 * nsave;
     }
-    const size_t npart = nsave * nwg * QND_NOBS;
-    if (c->adj_part_cap < npart) {
+    const size_t npart = 2 * nsave * nwg * QND_NOBS, ngsum = (size_t)nsteps * per_step * QND_NOBS;
+    if (c->adj_part_cap < npart + ngsum) {
         if (c->d_adj_part) HIPCK(c, hipFree(c->d_adj_part));
-        if (c->d_adj_gsum) HIPCK(c, hipFree(c->d_adj_gsum));
-        c->d_adj_part = c->d_adj_gsum = nullptr;
-        HIPCK(c, hipMalloc(&c->d_adj_part, npart * sizeof(double)));
-        HIPCK(c, hipMalloc(&c->d_adj_gsum, nsave * QND_NOBS * sizeof(double)));
-        c->adj_part_cap = npart;
+        c->d_adj_part = nullptr;
+        HIPCK(c, hipMalloc(&c->d_adj_part, (npart + ngsum) * sizeof(double)));
+        c->adj_part_cap = npart + ngsum;
     }
+    double *gsum_dev = c->d_adj_part + npart;
     C<R> *lam = reinterpret_cast<C<R> *>(c->d_scr[0]);
     C<R> *W = reinterpret_cast<C<R> *>(c->d_scr[1]);
-    C<R> *saves = reinterpret_cast<C<R> *>(c->d_adj_save);
-    std::vector<double> gsum(nsave * QND_NOBS);
-    for (int64_t k = 0; k < nsteps * s * d; ++k) dL_dEt[k] = 0.0;
-    for (int64_t k = 0; k < nsteps * d; ++k) dL_dEobs[k] = 0.0;
-    double dsoft = 0.0;
-
+    auto saves_of = [&](int64_t sg) { return reinterpret_cast<C<R> *>(c->d_adj_save) + (sg & 1) * nsave * c->N; };
+    auto part_of = [&](int64_t sg) { return c->d_adj_part + (sg & 1) * nsave * nwg * QND_NOBS; };
+    std::vector<hipEvent_t> ev_rec(nseg, nullptr), ev_adj(nseg, nullptr);
+    for (int64_t k = 0; k < nseg; ++k) {
+        HIPCK(c, hipEventCreateWithFlags(&ev_rec[k], hipEventDisableTiming));
+        HIPCK(c, hipEventCreateWithFlags(&ev_adj[k], hipEventDisableTiming));
+    }
+    auto cleanup = [&]() {
+        for (auto e : ev_rec)
+            if (e) hipEventDestroy(e);
+        for (auto e : ev_adj)
+            if (e) hipEventDestroy(e);
+    };
+    std::vector<std::vector<char>> masked(nseg);
+    const double *as = c->as_scaled;
+    auto kcoef = [&](int i) { return i == 0 ? as[0] : (i == s ? as[2 * s - 1] : as[2 * i - 1] + as[2 * i]); };
     auto pot = [&](C<R> *data) {
         PassArgs<R> a = base_args<R>(c, data, g0);
         a.scale1 = inv_n0;
@@ -1070,8 +1081,95 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
             if (int r = plain_pass<R>(c, data, ax, +1, 1.0 / (double)c->nr[ax])) return r;
         return 0;
     };
-    const double *as = c->as_scaled;
-    auto kcoef = [&](int i) { return i == 0 ? as[0] : (i == s ? as[2 * s - 1] : as[2 * i - 1] + as[2 * i]); };
+    // recompute segment sg from its checkpoint on the second stream, saving y_n
+    // and phi_{n,i}; it may start once the adjoint that last read its buffer is done
+    auto recompute = [&](int64_t sg) -> int {
+        std::swap(c->stream, c->stream2);
+        struct Restore {
+            qnd_ctx *c;
+            ~Restore() { std::swap(c->stream, c->stream2); }
+        } restore{c};
+        if (sg + 2 < nseg) HIPCK(c, hipStreamWaitEvent(c->stream, ev_adj[sg + 2], 0));
+        const int64_t nA = sg * K, len = (nA + K <= nsteps) ? K : nsteps - nA;
+        C<R> *saves = saves_of(sg);
+        HIPCK(c, hipMemcpyAsync(W, reinterpret_cast<C<R> *>(c->d_adj_ckpt) + sg * c->N, sbytes,
+                                hipMemcpyDeviceToDevice, c->stream));
+        SlotState st = c->adj_ckpt_state[sg];
+        if (st.rep == REP_R) {
+            if (int r = plain_pass<R>(c, W, 0, -1, 1.0)) return r;
+            st.rep = REP_M;
+        }
+        masked[sg].assign(len, 0);
+        for (int64_t k = 0; k < len; ++k) {
+            const int64_t n = nA + k;
+            const double *Et = &c->Et[(size_t)n * s * d];
+            PassArgs<R> b = pot(W);
+            b.flags |= PF_SAVE | PF_OBS | (st.pend_mask ? PF_MASK : 0u);
+            b.save = saves + (k * per_step) * c->N;
+            b.partials = c->d_part;
+            b.npart = QND_NOBS;
+            masked[sg][k] = st.pend_mask;
+            if (int r = launch<R, PK_POT, +1>(c, g0, b, PC_POT)) return r;
+            if (int r = kinetic_block<R>(c, W, KM_PHASE, kcoef(0), nullptr)) return r;
+            for (int ia = 0; ia < s; ++ia) {
+                PassArgs<R> p = pot(W);
+                p.flags |= PF_V1 | PF_SAVE;
+                p.vmode = VM_LENGTH;
+                p.cv1 = as[2 * ia] + as[2 * ia + 1];
+                for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
+                p.save = saves + (k * per_step + 1 + ia) * c->N;
+                if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
+                if (int r = kinetic_block<R>(c, W, KM_PHASE, kcoef(ia + 1), nullptr)) return r;
+            }
+            st.pend_mask = c->P.mask_on != 0;
+        }
+        HIPCK(c, hipEventRecord(ev_rec[sg], c->stream));
+        return 0;
+    };
+    // adjoint sweep over segment sg on the first stream
+    auto sweep = [&](int64_t sg) -> int {
+        HIPCK(c, hipStreamWaitEvent(c->stream, ev_rec[sg], 0));
+        const int64_t nA = sg * K, len = (nA + K <= nsteps) ? K : nsteps - nA;
+        C<R> *saves = saves_of(sg);
+        double *part = part_of(sg);
+        for (int64_t k = len - 1; k >= 0; --k) {
+            const int64_t n = nA + k;
+            const double *Et = &c->Et[(size_t)n * s * d];
+            const double *Eo = &c->Eobs[(size_t)n * d];
+            if (int r = kin_adj(lam, kcoef(s))) return r;
+            for (int ia = s - 1; ia >= 0; --ia) {
+                PassArgs<R> p = pot(lam);
+                p.flags |= PF_V1 | PF_CONJ | PF_GRAD;
+                p.vmode = VM_LENGTH;
+                p.cv1 = as[2 * ia] + as[2 * ia + 1];
+                for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
+                p.save = saves + (k * per_step + 1 + ia) * c->N;
+                p.partials = part + (size_t)(k * per_step + 1 + ia) * nwg * QND_NOBS;
+                p.npart = QND_NOBS;
+                if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
+                if (int r = kin_adj(lam, kcoef(ia))) return r;
+            }
+            PassArgs<R> b = pot(lam);
+            b.flags |= PF_SRC | (masked[sg][k] ? PF_MASK : 0u);
+            for (int j = 0; j < d; ++j) {
+                b.src[j] = dL_da[n * d + j] * dNr * inv_n0;
+                b.Ev[j] = Eo[j];
+            }
+            b.save = saves + (k * per_step) * c->N;
+            b.partials = part + (size_t)(k * per_step) * nwg * QND_NOBS;
+            b.npart = QND_NOBS;
+            if (int r = launch<R, PK_POT, +1>(c, g0, b, PC_POT)) return r;
+        }
+        {
+            ProfScope ps(c, PC_SMALL);
+            const int npass = (int)(len * per_step);
+            hipLaunchKernelGGL(k_sum_blocks, dim3((npass * QND_NOBS + 3) / 4), dim3(256), 0, c->stream,
+                               (const double *)part, npass, nwg, QND_NOBS, gsum_dev + nA * per_step * QND_NOBS);
+            HIPCK(c, hipGetLastError());
+        }
+        HIPCK(c, hipEventRecord(ev_adj[sg], c->stream));
+        return 0;
+    };
 
     // costate after the final boundary: F0 Mask (dNr g_norm / n0 y_N)
     HIPCK(c, hipMemsetAsync(lam, 0, sbytes, c->stream));
@@ -1082,94 +1180,32 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
         a.src_norm = dL_dnorm * dNr * inv_n0;
         a.partials = c->d_adj_part;
         a.npart = QND_NOBS;
-        if ((rc = launch<R, PK_POT, +1>(c, g0, a, PC_POT))) return rc;
+        if ((rc = launch<R, PK_POT, +1>(c, g0, a, PC_POT))) return cleanup(), rc;
     }
+    // the second stream must not start before the checkpoints are written
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if ((rc = recompute(nseg - 1))) return cleanup(), rc;
     for (int64_t sg = nseg - 1; sg >= 0; --sg) {
-        const int64_t nA = sg * K, len = (nA + K <= nsteps) ? K : nsteps - nA;
-        // ---- recompute the segment from its checkpoint, saving y_n and phi_{n,i} ----
-        HIPCK(c, hipMemcpyAsync(W, reinterpret_cast<C<R> *>(c->d_adj_ckpt) + sg * c->N, sbytes,
-                                hipMemcpyDeviceToDevice, c->stream));
-        SlotState st = c->adj_ckpt_state[sg];
-        if (st.rep == REP_R) {
-            if ((rc = plain_pass<R>(c, W, 0, -1, 1.0))) return rc;
-            st.rep = REP_M;
-        }
-        std::vector<char> masked(len, 0);
-        for (int64_t k = 0; k < len; ++k) {
-            const int64_t n = nA + k;
-            const double *Et = &c->Et[(size_t)n * s * d];
-            PassArgs<R> b = pot(W);
-            b.flags |= PF_SAVE | PF_OBS | (st.pend_mask ? PF_MASK : 0u);
-            b.save = saves + (k * per_step) * c->N;
-            b.partials = c->d_part;
-            b.npart = QND_NOBS;
-            masked[k] = st.pend_mask;
-            if ((rc = launch<R, PK_POT, +1>(c, g0, b, PC_POT))) return rc;
-            if ((rc = kinetic_block<R>(c, W, KM_PHASE, kcoef(0), nullptr))) return rc;
-            for (int ia = 0; ia < s; ++ia) {
-                PassArgs<R> p = pot(W);
-                p.flags |= PF_V1 | PF_SAVE;
-                p.vmode = VM_LENGTH;
-                p.cv1 = as[2 * ia] + as[2 * ia + 1];
-                for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
-                p.save = saves + (k * per_step + 1 + ia) * c->N;
-                if ((rc = launch<R, PK_POT, +1>(c, g0, p, PC_POT))) return rc;
-                if ((rc = kinetic_block<R>(c, W, KM_PHASE, kcoef(ia + 1), nullptr))) return rc;
-            }
-            st.pend_mask = c->P.mask_on != 0;
-        }
-        // ---- adjoint sweep over the segment ----
-        for (int64_t k = len - 1; k >= 0; --k) {
-            const int64_t n = nA + k;
-            const double *Et = &c->Et[(size_t)n * s * d];
-            const double *Eo = &c->Eobs[(size_t)n * d];
-            if ((rc = kin_adj(lam, kcoef(s)))) return rc;
-            for (int ia = s - 1; ia >= 0; --ia) {
-                PassArgs<R> p = pot(lam);
-                p.flags |= PF_V1 | PF_CONJ | PF_GRAD;
-                p.vmode = VM_LENGTH;
-                p.cv1 = as[2 * ia] + as[2 * ia + 1];
-                for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
-                p.save = saves + (k * per_step + 1 + ia) * c->N;
-                p.partials = c->d_adj_part + (size_t)(k * per_step + 1 + ia) * nwg * QND_NOBS;
-                p.npart = QND_NOBS;
-                if ((rc = launch<R, PK_POT, +1>(c, g0, p, PC_POT))) return rc;
-                if ((rc = kin_adj(lam, kcoef(ia)))) return rc;
-            }
-            PassArgs<R> b = pot(lam);
-            b.flags |= PF_SRC | (masked[k] ? PF_MASK : 0u);
-            for (int j = 0; j < d; ++j) {
-                b.src[j] = dL_da[n * d + j] * dNr * inv_n0;
-                b.Ev[j] = Eo[j];
-            }
-            b.save = saves + (k * per_step) * c->N;
-            b.partials = c->d_adj_part + (size_t)(k * per_step) * nwg * QND_NOBS;
-            b.npart = QND_NOBS;
-            if ((rc = launch<R, PK_POT, +1>(c, g0, b, PC_POT))) return rc;
-        }
-        // ---- sum the partials of the segment's passes ----
-        const int npass = (int)(len * per_step);
-        {
-            ProfScope ps(c, PC_SMALL);
-            hipLaunchKernelGGL(k_sum_blocks, dim3((npass * QND_NOBS + 3) / 4), dim3(256), 0, c->stream,
-                               (const double *)c->d_adj_part, npass, nwg, QND_NOBS, (double *)c->d_adj_gsum);
-            HIPCK(c, hipGetLastError());
-        }
-        HIPCK(c, hipMemcpyAsync(gsum.data(), c->d_adj_gsum, (size_t)npass * QND_NOBS * sizeof(double),
-                                hipMemcpyDeviceToHost, c->stream));
-        HIPCK(c, hipStreamSynchronize(c->stream));
-        for (int64_t k = 0; k < len; ++k) {
-            const int64_t n = nA + k;
-            const double *bsum = &gsum[(size_t)(k * per_step) * QND_NOBS];
-            dsoft += n0d * bsum[0];  // explicit dependence of a_n on the soft-core length
-            const double nrm = c->adj_obs[n * (d + 1) + d];
-            for (int j = 0; j < d; ++j) dL_dEobs[n * d + j] = -dL_da[n * d + j] * nrm;
-            for (int ia = 0; ia < s; ++ia) {
-                const double *ps = &gsum[(size_t)(k * per_step + 1 + ia) * QND_NOBS];
-                const double cc = as[2 * ia] + as[2 * ia + 1];
-                dsoft += 2.0 * n0d * cc * ps[0];
-                for (int j = 0; j < d; ++j) dL_dEt[(n * s + ia) * d + j] = 2.0 * n0d * cc * ps[1 + j];
-            }
+        if (sg > 0 && (rc = recompute(sg - 1))) return cleanup(), rc;
+        if ((rc = sweep(sg))) return cleanup(), rc;
+    }
+    std::vector<double> gsum(ngsum);
+    HIPCK(c, hipMemcpyAsync(gsum.data(), gsum_dev, ngsum * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream2));
+    cleanup();
+    for (int64_t k = 0; k < nsteps * s * d; ++k) dL_dEt[k] = 0.0;
+    double dsoft = 0.0;
+    for (int64_t n = 0; n < nsteps; ++n) {
+        const double *bsum = &gsum[(size_t)(n * per_step) * QND_NOBS];
+        dsoft += n0d * bsum[0];  // explicit dependence of a_n on the soft-core length
+        const double nrm = c->adj_obs[n * (d + 1) + d];
+        for (int j = 0; j < d; ++j) dL_dEobs[n * d + j] = -dL_da[n * d + j] * nrm;
+        for (int ia = 0; ia < s; ++ia) {
+            const double *ps = &gsum[(size_t)(n * per_step + 1 + ia) * QND_NOBS];
+            const double cc = as[2 * ia] + as[2 * ia + 1];
+            dsoft += 2.0 * n0d * cc * ps[0];
+            for (int j = 0; j < d; ++j) dL_dEt[(n * s + ia) * d + j] = 2.0 * n0d * cc * ps[1 + j];
         }
     }
     if (dL_dsoft) *dL_dsoft = dsoft;
@@ -1435,7 +1471,8 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
         qnd_destroy(c);
         return code;
     };
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(nullptr, QND_EHIP, "hipStreamCreate failed"));
     size_t sbytes = (size_t)c->N * c->esz;
     if (hipMalloc(&c->d_states, sbytes * n_slots) != hipSuccess)
@@ -1493,10 +1530,14 @@ void qnd_destroy(qnd_ctx *c)
     if (c->d_bEt) hipFree(c->d_bEt);
     if (c->d_bAt) hipFree(c->d_bAt);
     if (c->d_bEo) hipFree(c->d_bEo);
-    for (void *p : {c->d_adj_ckpt, c->d_adj_final, c->d_adj_save, (void *)c->d_adj_part, (void *)c->d_adj_gsum})
+    for (void *p : {c->d_adj_ckpt, c->d_adj_final, c->d_adj_save, (void *)c->d_adj_part})
         if (p) hipFree(p);
     for (int k = 0; k < 16; ++k)
         if (c->ev[k]) hipEventDestroy(c->ev[k]);
+    if (c->stream2) {
+        hipStreamSynchronize(c->stream2);
+        hipStreamDestroy(c->stream2);
+    }
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }

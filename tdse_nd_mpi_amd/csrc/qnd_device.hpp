@@ -661,15 +661,56 @@ __device__ __forceinline__ void line_index(const Phys &P, int axis, int64_t S, i
     }
 }
 
-// turn per-workgroup [n0, d_j, g_j] sums into the observable record (wave 0
-// of block 0)
-__device__ __forceinline__ void obs_finalize(const double *part, int nwg, int dim, double dNr,
-                                             const ObsSpec &spec, const double *E, double *rec)
+// Column sums of a [nwg][npart] partials array, columns q0 .. q0+K-1, with
+// every thread of the block: thread-strided partial sums, then the fixed
+// tree of block_reduce_lds.  Deterministic for a given block size.  All
+// threads must call; every thread gets the sums in tot[].
+__device__ __forceinline__ void block_reduce_lds(double *v, int K, double *red);
+template <int K>
+__device__ __forceinline__ void block_column_sums(const double *part, int nwg, int npart, int q0, double *tot,
+                                                  double *red)
 {
-    if (threadIdx.x >= 64) return;
-    double tot[QND_NOBS];
+    double acc[K];
 #pragma unroll
-    for (int q = 0; q < QND_NOBS; ++q) tot[q] = wave_sum_column(part, nwg, QND_NOBS, q) * dNr;
+    for (int q = 0; q < K; ++q) acc[q] = 0.0;
+    for (int w = threadIdx.x; w < nwg; w += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+            if (q0 + q < npart) acc[q] += part[(int64_t)w * npart + q0 + q];
+    }
+    block_reduce_lds(acc, K, red);
+#pragma unroll
+    for (int q = 0; q < K; ++q) tot[q] = acc[q];
+}
+
+// Sums v[0..K) over the block (wave butterflies, then the waves in order);
+// every thread receives the result in v.  red: LDS of >= 16*K doubles.
+__device__ __forceinline__ void block_reduce_lds(double *v, int K, double *red)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int q = 0; q < K; ++q) v[q] = wave_sum(v[q]);
+    lds_barrier();
+    if (lane == 0)
+        for (int q = 0; q < K; ++q) red[q * 16 + w] = v[q];
+    lds_barrier();
+    for (int q = 0; q < K; ++q) {
+        double s = 0.0;
+        for (int i = 0; i < nw; ++i) s += red[q * 16 + i];
+        v[q] = s;
+    }
+    lds_barrier();
+}
+
+// Turn per-workgroup [n0, d_j, g_j] sums into the observable record.  Every
+// thread of the calling block must call (block-wide reduction).
+__device__ __forceinline__ void obs_finalize(const double *part, int nwg, int dim, double dNr,
+                                             const ObsSpec &spec, const double *E, double *rec, double *red)
+{
+    double tot[QND_NOBS];
+    block_column_sums<QND_NOBS>(part, nwg, QND_NOBS, 0, tot, red);
+#pragma unroll
+    for (int q = 0; q < QND_NOBS; ++q) tot[q] *= dNr;
     if (threadIdx.x == 0) {
         int off = 0;
         for (int k = 0; k < spec.ncodes; ++k) {
@@ -689,10 +730,11 @@ __device__ __forceinline__ void obs_finalize(const double *part, int nwg, int di
 }
 // Instance-b view of a finalize request: partials, E(t_n) and record.
 __device__ __forceinline__ void obs_finalize_instance(const double *part, int nwg, int dim, double dNr,
-                                                      const ObsSpec &spec, const Batch &B, int b, double *rec)
+                                                      const ObsSpec &spec, const Batch &B, int b, double *rec,
+                                                      double *red)
 {
     const double *E = B.finE ? B.finE + b * B.finE_stride : spec.E;
-    obs_finalize(part + (int64_t)b * nwg * QND_NOBS, nwg, dim, dNr, spec, E, rec + b * B.rec_stride);
+    obs_finalize(part + (int64_t)b * nwg * QND_NOBS, nwg, dim, dNr, spec, E, rec + b * B.rec_stride, red);
 }
 
 
@@ -714,7 +756,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
     const unsigned fl = a.flags;
 
     if (KIND == PK_KIN && (fl & PF_FINALIZE) && (int)blockIdx.x < a.B.nb)
-        obs_finalize_instance(a.fin_part, a.fin_nwg, P.dim, P.dNr, a.fin_spec, a.B, blockIdx.x, a.fin_rec);
+        obs_finalize_instance(a.fin_part, a.fin_nwg, P.dim, P.dNr, a.fin_spec, a.B, blockIdx.x, a.fin_rec, red);
 
     // ---- tile geometry: TW lines (TW | S or S == 1).  Contiguous lines put
     // t on the fastest lanes, strided lines the line index l, so that every
@@ -1000,10 +1042,11 @@ __global__ void __launch_bounds__(256) k_sum_blocks(const double *__restrict__ p
 }
 
 // one block per instance
-__global__ void __launch_bounds__(64) k_obs_finalize(const double *__restrict__ part, int nwg, int dim,
-                                                     double dNr, ObsSpec spec, Batch B, double *rec)
+__global__ void __launch_bounds__(1024) k_obs_finalize(const double *__restrict__ part, int nwg, int dim,
+                                                       double dNr, ObsSpec spec, Batch B, double *rec)
 {
-    obs_finalize_instance(part, nwg, dim, dNr, spec, B, blockIdx.x, rec);
+    __shared__ double red[16 * QND_NOBS];
+    obs_finalize_instance(part, nwg, dim, dNr, spec, B, blockIdx.x, rec, red);
 }
 
 // Modified Gram-Schmidt coefficients from partial dots in the stored
@@ -1011,16 +1054,20 @@ __global__ void __launch_bounds__(64) k_obs_finalize(const double *__restrict__ 
 // b_j = <prev_j|v>, nv = <v|v>; c_j = b_j - sum_{i<j} c_i G_{j i};
 // |v'|^2 = nv - 2 Re sum_j c_j conj(b_j) + sum_{ij} conj(c_i) c_j G_{ij}.
 // coef = [c_0 .. c_{N-1} (re,im), invnorm, flag, norm]
-__global__ void __launch_bounds__(256) k_gs_coeffs(const double *__restrict__ part, int nwg, int npart,
-                                                   int nprev, double scale, const double *__restrict__ gram,
-                                                   double *coef)
+__global__ void __launch_bounds__(1024) k_gs_coeffs(const double *__restrict__ part, int nwg, int npart,
+                                                    int nprev, double scale, const double *__restrict__ gram,
+                                                    double *coef)
 {
-    __shared__ double b[2 * 65];
+    __shared__ double b[2 * 65 + 8];
     __shared__ double cre[64], cim[64];
-    const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
-    for (int q = wv; q < 2 * (nprev + 1); q += nwv) {
-        const double s = wave_sum_column(part, nwg, npart, q);
-        if ((threadIdx.x & 63) == 0) b[q] = s * scale;
+    __shared__ double red[16 * 8];
+    // column sums of the dot partials, eight columns at a time over the whole block
+    const int ncol = 2 * (nprev + 1);
+    for (int q0 = 0; q0 < ncol; q0 += 8) {
+        double tot[8];
+        block_column_sums<8>(part, nwg, npart, q0, tot, red);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 8 && q0 + q < ncol; ++q) b[q0 + q] = tot[q] * scale;
     }
     lds_barrier();
     if (threadIdx.x == 0) {
