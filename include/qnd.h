@@ -12,7 +12,8 @@
  * Conventions
  *  - Every entry returns 0 on success and a non-zero QND_E* code on error;
  *    qnd_last_error() gives the message.  The library never exits.
- *  - One host thread per context; each context owns one HIP stream.  Calls
+ *  - One host thread per context; each context owns two HIP streams (the
+ *    second overlaps the adjoint's recompute with its costate sweep).  Calls
  *    are stream-ordered; host-visible outputs are complete on return.
  *  - Complex host arrays are interleaved FP64 (re, im), row-major, axis 0
  *    slowest: the reference layout (src/meshgrid.c:205-211).
@@ -143,7 +144,8 @@ int qnd_real_steps(qnd_ctx *ctx, int slot, int gauge, int64_t step0,
  * the conjugated passes, each checkpoint segment is recomputed once, and it
  * returns dL/dEt [nsteps][s][dim] (the stage fields of qnd_set_field),
  * dL/dEobs [nsteps][dim] (the E(t_n) inside a_n) and dL/d(soft-core length)
- * (the initial state held fixed). */
+ * (the initial state held fixed).  Changing the potential, integrator, mask
+ * or field between the two calls makes qnd_adjoint fail with QND_ESTATE. */
 int qnd_forward_checkpointed(qnd_ctx *ctx, int slot, int64_t nsteps,
                              int64_t every, double *obs_out,
                              double *norm_final);

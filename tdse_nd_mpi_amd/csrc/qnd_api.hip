@@ -87,6 +87,7 @@ struct qnd_ctx {
     std::vector<SlotState> adj_ckpt_state;
     std::vector<double> adj_obs;
     int adj_slot = -1, adj_prec = -1;
+    int64_t setup_gen = 0, adj_gen = -1;  // potential/integrator/mask/field changes; the forward's value
     int64_t adj_nsteps = 0, adj_K = 0;
     int bnb = 0;
     int64_t bntime = 0;
@@ -1011,6 +1012,7 @@ template <typename R> int adj_forward(qnd_ctx *c, int slot, int64_t nsteps, int6
     c->adj_nsteps = nsteps;
     c->adj_K = K;
     c->adj_prec = c->prec;
+    c->adj_gen = c->setup_gen;
     if (obs_out) memcpy(obs_out, c->adj_obs.data(), c->adj_obs.size() * sizeof(double));
     if (norm_final) *norm_final = nf;
     return 0;
@@ -1545,6 +1547,7 @@ void qnd_destroy(qnd_ctx *c)
 int qnd_set_potential(qnd_ctx *c, const char *name, double softcore, const double *cap)
 {
     if (!c || !name) return QND_EINVAL;
+    ++c->setup_gen;
     int kind;
     double b, c2;
     if (!potential_params(name, &kind, &b, &c2))
@@ -1573,6 +1576,7 @@ int qnd_set_potential(qnd_ctx *c, const char *name, double softcore, const doubl
 int qnd_set_integrator(qnd_ctx *c, const char *name, double dt)
 {
     if (!c || !name) return QND_EINVAL;
+    ++c->setup_gen;
     static const double bm4[6] = {.0792036964311957, .1303114101821663, .2228614958676077,
                                   -.3667132690474257, .3246481886897062, .1096884778767498};
     static const double bm6[10] = {.050262764400392, .098553683500650, .314960616927694, -.447346482695478,
@@ -1601,6 +1605,7 @@ int qnd_set_integrator(qnd_ctx *c, const char *name, double dt)
 int qnd_set_mask(qnd_ctx *c, double frac)
 {
     if (!c) return QND_EINVAL;
+    ++c->setup_gen;
     // m_j(x) = cos(pi/2 (|x| - x0)/(L - x0))^(1/8) for |x| > x0 = (1-frac) L,
     // L = n_j dr_j / 2; the mask is the product over axes
     c->mask_frac = frac;
@@ -1706,6 +1711,7 @@ int qnd_set_field(qnd_ctx *c, int64_t ntime, const double *Et, const double *At,
     if (Eobs) memcpy(c->Eobs.data(), Eobs, no * sizeof(double));
     if (Aobs) memcpy(c->Aobs.data(), Aobs, no * sizeof(double));
     c->ntime = ntime;
+    ++c->setup_gen;
     return 0;
 }
 
@@ -1739,6 +1745,8 @@ int qnd_adjoint(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dEt
     if (!c || !dL_da || !dL_dEt || !dL_dEobs) return QND_EINVAL;
     if (c->adj_nsteps < 1 || c->adj_prec != c->prec)
         return fail(c, QND_ESTATE, "no checkpointed forward run (qnd_forward_checkpointed)");
+    if (c->adj_gen != c->setup_gen)
+        return fail(c, QND_ESTATE, "potential, integrator, mask or field changed since the checkpointed forward run");
     return DISPATCH(c, adj_backward, c, dL_da, dL_dnorm, dL_dEt, dL_dEobs, dL_dsoft);
 }
 

@@ -442,3 +442,18 @@ def test_wavefront_local_exchanges(gpu, wave_lines, monkeypatch):
     assert rel(g.download(0), ref_psi) < TOL128
     assert np.allclose(obs, ref_obs, rtol=1e-11, atol=1e-12)
     g.close()
+
+
+def test_adjoint_rejects_changed_setup(gpu):
+    """The costate needs the forward run's operators: a field or mask change in
+    between is an error, not a silently wrong gradient."""
+    print("case adjoint setup change", flush=True)
+    adj, spec, E, wa = _adjoint_problem("c128", nr=(32, 32), nsteps=8, every=4)
+    adj.forward_numpy(spec, E, 1.0)
+    spec.grid.set_mask(0.2)
+    with pytest.raises(q.QNDError, match="changed since"):
+        spec.grid.adjoint(wa, 0.7)
+    adj.forward_numpy(spec, E, 1.0)
+    dEt, dEo, ds = spec.grid.adjoint(wa, 0.7)
+    assert np.isfinite(dEt).all() and np.isfinite(ds)
+    spec.close()
