@@ -353,10 +353,22 @@ struct Finalize {
     Batch B;         // nb, per-instance E(t_n) and record stride
 };
 
+// Block size of the one-block column reductions over nwg partial rows: at
+// least eight rows per thread, so small grids reduce within one wavefront
+// and large ones use 1024 threads (the sums' order, hence their rounding, is
+// fixed for a given nwg).
+int reduce_threads(int64_t nwg)
+{
+    int thr = 64;
+    while (thr < 1024 && (int64_t)thr * 8 < nwg) thr *= 2;
+    return thr;
+}
+
 int launch_finalize(qnd_ctx *c, const Finalize *fin)
 {
     ProfScope ps(c, PC_SMALL);
-    hipLaunchKernelGGL(k_obs_finalize, dim3(fin->B.nb), dim3(1024), 0, c->stream, fin->part, fin->nwg, c->dim,
+    hipLaunchKernelGGL(k_obs_finalize, dim3(fin->B.nb), dim3(reduce_threads(fin->nwg)), 0, c->stream, fin->part,
+                       fin->nwg, c->dim,
                        c->P.dNr, fin->spec, fin->B, fin->rec);
     HIPCK(c, hipGetLastError());
     return 0;
@@ -672,7 +684,8 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
         if ((rc = launch_pot<R>(c, g0, a))) return rc;
         if (gs) {
             ProfScope ps(c, PC_SMALL);
-            hipLaunchKernelGGL(k_gs_coeffs, dim3(1), dim3(1024), 0, c->stream, c->d_part, g0.grid, npart,
+            hipLaunchKernelGGL(k_gs_coeffs, dim3(1), dim3(reduce_threads(g0.grid)), 0, c->stream, c->d_part,
+                               g0.grid, npart,
                                nprev, c->P.dNr * inv_n0, c->d_small + SMALL_GRAM, coef);
             HIPCK(c, hipGetLastError());
             pending = true;

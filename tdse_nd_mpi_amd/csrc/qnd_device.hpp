@@ -217,23 +217,15 @@ template <> __device__ __forceinline__ C<float> fac_mul<float>(C<float> x, const
 // again only after every thread has passed the barrier that follows its
 // reads).  `wc` counts the exchanges of the kernel and picks the buffer.
 // ---------------------------------------------------------------------
+// Butterflies of one stage given its twiddles w[b*RAD + r] (r >= 1; unused
+// when Ns = 1).
 template <typename R, int RAD, int DIR, int VPT>
-__device__ __forceinline__ void reg_stage(C<R> (&x)[VPT], const C<R> *lin, C<R> *lout, int n, int T, int t,
-                                          int Ns, const typename TwT<R>::T *__restrict__ tw)
+__device__ __forceinline__ void stage_core(C<R> (&x)[VPT], const C<R> *lin, C<R> *lout, int n, int T, int t,
+                                           int Ns, const typename TwT<R>::T (&w)[VPT])
 {
     if constexpr (VPT % RAD == 0) {
         constexpr int NB = VPT / RAD;
         const int nR = n / RAD;
-        const int stride = n / (Ns * RAD);
-        typename TwT<R>::T w[VPT];
-        if (Ns > 1) {  // twiddles first: their latency overlaps the LDS reads
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int k = (t + b * T) & (Ns - 1);
-#pragma unroll
-                for (int r = 1; r < RAD; ++r) w[b * RAD + r] = tw[r * k * stride];
-            }
-        }
         C<R> y[VPT];
 #pragma unroll
         for (int b = 0; b < NB; ++b)
@@ -263,6 +255,27 @@ __device__ __forceinline__ void reg_stage(C<R> (&x)[VPT], const C<R> *lin, C<R> 
 #pragma unroll
                 for (int r = 0; r < RAD; ++r) x[b + r * NB] = y[b * RAD + r];
         }
+    }
+}
+
+// One stage with its twiddles read from the table tw (n entries).
+template <typename R, int RAD, int DIR, int VPT>
+__device__ __forceinline__ void reg_stage(C<R> (&x)[VPT], const C<R> *lin, C<R> *lout, int n, int T, int t,
+                                          int Ns, const typename TwT<R>::T *__restrict__ tw)
+{
+    if constexpr (VPT % RAD == 0) {
+        constexpr int NB = VPT / RAD;
+        const int stride = n / (Ns * RAD);
+        typename TwT<R>::T w[VPT];
+        if (Ns > 1) {  // twiddles first: their latency overlaps the LDS reads
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int k = (t + b * T) & (Ns - 1);
+#pragma unroll
+                for (int r = 1; r < RAD; ++r) w[b * RAD + r] = tw[r * k * stride];
+            }
+        }
+        stage_core<R, RAD, DIR, VPT>(x, lin, lout, n, T, t, Ns, w);
     }
 }
 
@@ -310,6 +323,80 @@ __device__ __forceinline__ void fft_line(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB,
     }
 }
 
+// Four values per thread (n = 2^log2n <= 2048, T = n/4 threads): the stage
+// schedule is fixed at compile time (radix-4 stage s has span 4^s; radix 2
+// last when log2 n is odd) and every twiddle a thread needs is read once, at
+// kernel entry, into registers.  Both transforms of a pass share them (the
+// direction only conjugates), so no stage waits on a table load: these
+// grids run one wavefront per SIMD, where each such wait is exposed.
+#define QND_TW4_STAGES 4   // radix-4 stages with twiddles: s = 1..4 (n <= 2048)
+template <typename R> struct Tw4 {
+    typename TwT<R>::T w4[QND_TW4_STAGES][3];  // stage s: W^{r k n/(4 Ns)}, k = t mod Ns, r = 1..3
+    typename TwT<R>::T w2[2];                   // last radix-2 stage (odd log2 n): W^{t + b T}
+};
+template <typename R>
+__device__ __forceinline__ void tw4_load(Tw4<R> &P, const typename TwT<R>::T *__restrict__ tw, int log2n, int T,
+                                         int t)
+{
+    const int ns4 = log2n >> 1;
+#pragma unroll
+    for (int s = 1; s <= QND_TW4_STAGES; ++s) {
+        if (s < ns4) {
+            const int k = t & ((1 << (2 * s)) - 1);
+            const int stride = 1 << (log2n - 2 - 2 * s);
+#pragma unroll
+            for (int r = 1; r < 4; ++r) P.w4[s - 1][r - 1] = tw[r * k * stride];
+        }
+    }
+    if (log2n & 1) {
+        P.w2[0] = tw[t];
+        P.w2[1] = tw[t + T];
+    }
+}
+template <typename R, int DIR>
+__device__ __forceinline__ void fft_line4(C<R> (&x)[4], C<R> *bufA, C<R> *bufB, int &wc, int n, int log2n, int T,
+                                          int t, const Tw4<R> &P, bool wave_line)
+{
+    const int ns4 = log2n >> 1;
+    const bool odd = (log2n & 1) != 0;
+    typename TwT<R>::T w[4];
+    const C<R> *lin = nullptr;
+#pragma unroll
+    for (int s = 0; s <= QND_TW4_STAGES; ++s) {
+        if (s < ns4) {
+            const bool last = !odd && s == ns4 - 1;
+            C<R> *lout = last ? nullptr : ((wc & 1) ? bufB : bufA);
+            if (s > 0) {
+#pragma unroll
+                for (int r = 1; r < 4; ++r) w[r] = P.w4[s - 1][r - 1];
+            }
+            stage_core<R, 4, DIR, 4>(x, s ? lin : nullptr, lout, n, T, t, 1 << (2 * s), w);
+            if (!last) {
+                if (wave_line) wave_fence();
+                else lds_barrier();
+                lin = lout;
+                ++wc;
+            }
+        }
+    }
+    if (odd) {
+        w[1] = P.w2[0];
+        w[3] = P.w2[1];
+        stage_core<R, 2, DIR, 4>(x, lin, nullptr, n, T, t, n >> 1, w);
+    }
+}
+
+// The transform of a pass: the preloaded-twiddle schedule for four values
+// per thread, the table-driven one otherwise.
+template <typename R, int DIR, int VPT>
+__device__ __forceinline__ void transform(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int &wc, int n, int log2n, int T,
+                                          int t, const typename TwT<R>::T *__restrict__ tw, const Tw4<R> &P,
+                                          bool wave_line)
+{
+    if constexpr (VPT == 4) fft_line4<R, DIR>(x, bufA, bufB, wc, n, log2n, T, t, P, wave_line);
+    else fft_line<R, DIR, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, wave_line);
+}
+
 // ---------------------------------------------------------------------
 // physics parameters (by value in every launch)
 // ---------------------------------------------------------------------
@@ -339,33 +426,64 @@ __device__ __forceinline__ double coord_p(const Phys &P, int j, int i)
 // V(r) and dV/du (u = r^2) for the named potentials (src/meshgrid.c:140-153).
 // complex128 evaluates them in FP64 like the reference; complex64 in FP32
 // (a relative change of V of a few 1e-7, below the FP32 state rounding).
-template <typename R> __device__ double potential(const Phys &P, double rr, double &dVdu);
-template <> __device__ __forceinline__ double potential<double>(const Phys &P, double rr, double &dVdu)
-{
-    double rad = sqrt(rr);
-    double u = rad * rad;
-    if (P.pot_kind == 0) {
-        double den = sqrt(u + P.pot_c2);
-        double g = (P.pot_b != 0.0) ? P.pot_b * exp(-u) : 0.0;
-        dVdu = g / den + .5 * (1. + g) / (den * (u + P.pot_c2));
-        return -(1. + g) / den;
+// One point of a potential family known at compile time: SOFT (soft-core,
+// PK = 0) with or without the Gaussian term (HASB), else the Gaussian well.
+template <typename R, bool SOFT, bool HASB> struct PotPoint;
+template <bool SOFT, bool HASB> struct PotPoint<double, SOFT, HASB> {
+    static __device__ __forceinline__ double eval(const Phys &P, double rr, double &dVdu)
+    {
+        double rad = sqrt(rr);
+        double u = rad * rad;
+        if constexpr (SOFT) {
+            double den = sqrt(u + P.pot_c2);
+            double g = HASB ? P.pot_b * exp(-u) : 0.0;
+            dVdu = g / den + .5 * (1. + g) / (den * (u + P.pot_c2));
+            return -(1. + g) / den;
+        } else {
+            double g = 2.7459 * exp(-.5 * rad * rad / 1.96);
+            dVdu = g * .5 / 1.96;
+            return -g;
+        }
     }
-    double g = 2.7459 * exp(-.5 * rad * rad / 1.96);
-    dVdu = g * .5 / 1.96;
-    return -g;
+};
+template <bool SOFT, bool HASB> struct PotPoint<float, SOFT, HASB> {
+    static __device__ __forceinline__ double eval(const Phys &P, double rr, double &dVdu)
+    {
+        const float u = (float)rr;
+        if constexpr (SOFT) {
+            const float id = rsqrtf(u + (float)P.pot_c2);
+            const float g = HASB ? (float)P.pot_b * __expf(-u) : 0.0f;
+            dVdu = (double)(g * id + .5f * (1.f + g) * (id * id * id));
+            return (double)(-(1.f + g) * id);
+        } else {
+            const float g = 2.7459f * __expf(u * (-.5f / 1.96f));
+            dVdu = (double)(g * (.5f / 1.96f));
+            return (double)(-g);
+        }
+    }
+};
+template <typename R> __device__ __forceinline__ double potential(const Phys &P, double rr, double &dVdu)
+{
+    if (P.pot_kind != 0) return PotPoint<R, false, false>::eval(P, rr, dVdu);
+    if (P.pot_b != 0.0) return PotPoint<R, true, true>::eval(P, rr, dVdu);
+    return PotPoint<R, true, false>::eval(P, rr, dVdu);
 }
-template <> __device__ __forceinline__ double potential<float>(const Phys &P, double rr, double &dVdu)
+// V and dV/du at the VPT points of a thread, rr[m] = r^2: the family is
+// chosen once, outside the loop, so the points form independent chains.
+template <typename R, int VPT>
+__device__ __forceinline__ void potential_pts(const Phys &P, const double (&rr)[VPT], double (&V)[VPT],
+                                              double (&dVdu)[VPT])
 {
-    const float u = (float)rr;
-    if (P.pot_kind == 0) {
-        const float id = rsqrtf(u + (float)P.pot_c2);
-        const float g = (P.pot_b != 0.0) ? (float)P.pot_b * __expf(-u) : 0.0f;
-        dVdu = (double)(g * id + .5f * (1.f + g) * (id * id * id));
-        return (double)(-(1.f + g) * id);
+    if (P.pot_kind != 0) {
+#pragma unroll
+        for (int m = 0; m < VPT; ++m) V[m] = PotPoint<R, false, false>::eval(P, rr[m], dVdu[m]);
+    } else if (P.pot_b != 0.0) {
+#pragma unroll
+        for (int m = 0; m < VPT; ++m) V[m] = PotPoint<R, true, true>::eval(P, rr[m], dVdu[m]);
+    } else {
+#pragma unroll
+        for (int m = 0; m < VPT; ++m) V[m] = PotPoint<R, true, false>::eval(P, rr[m], dVdu[m]);
     }
-    const float g = 2.7459f * __expf(u * (-.5f / 1.96f));
-    dVdu = (double)(g * (.5f / 1.96f));
-    return (double)(-g);
 }
 
 // Derivatives with respect to the soft-core length a (c2 = a^2) of V and of
@@ -397,7 +515,9 @@ template <> __device__ __forceinline__ Fac<double> phase_factor<double>(double a
 {
     double s, c;
     sincos(ph, &s, &c);
-    const double e = (arg == 0.0) ? 1.0 : exp(arg > QND_MAX_EXP_ARG ? QND_MAX_EXP_ARG : arg);
+    // computed for every point and selected (no branch): e^0 = 1 exactly
+    const double ex = exp(arg > QND_MAX_EXP_ARG ? QND_MAX_EXP_ARG : arg);
+    const double e = (arg == 0.0) ? 1.0 : ex;
     Fac<double> f;
     f.v = mk<double>(e * c, -e * s);
     return f;
@@ -454,15 +574,14 @@ template <> __device__ __forceinline__ Fac<float> phase_factor<float>(double arg
     else { c = sr; s = -cr; }
     Fac<float> f;
     const float he = -0.5f * e;
-    if (arg == 0.0) {
-        f.hi = mk<float>(c, -s);
-        f.lo = mk<float>(he * c, -(he * s));
-    } else {
-        const float g = exp_f32(arg > QND_MAX_EXP_ARG_F32 ? QND_MAX_EXP_ARG_F32 : arg);
-        const float hx = g * c, hy = -(g * s);
-        f.hi = mk<float>(hx, hy);
-        f.lo = mk<float>(fmaf(g, c, -hx) + he * hx, -fmaf(g, s, hy) + he * hy);
-    }
+    // the damping factor is computed for every point and selected (no
+    // branch); with g = 1 the products below are exact and reduce to the
+    // undamped pair
+    const float gx = exp_f32(arg > QND_MAX_EXP_ARG_F32 ? QND_MAX_EXP_ARG_F32 : arg);
+    const float g = (arg == 0.0) ? 1.0f : gx;
+    const float hx = g * c, hy = -(g * s);
+    f.hi = mk<float>(hx, hy);
+    f.lo = mk<float>(fmaf(g, c, -hx) + he * hx, -fmaf(g, s, hy) + he * hy);
     return f;
 }
 template <typename R> __device__ __forceinline__ R real_exp(double arg);
@@ -782,6 +901,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
     C<R> *bufA = lds + l * ldsS;
     C<R> *bufB = lds + (TW + l) * ldsS;
     int wc = 0;
+    Tw4<R> tw4;  // four values per thread: all twiddles requested before the state
+    if constexpr (VPT == 4) tw4_load<R>(tw4, a.tw, a.log2n, T, t);
 
     // ---- load x[m] = element t + m T (with the Gram-Schmidt update prologue).
     // Twiddles are read from the table in memory (L1/L2 hits); staging it in
@@ -816,7 +937,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
     // need no workgroup barrier
     const bool wave_line = contig && T == 64 && a.wave_lines;
     if (KIND == PK_PLAIN) {
-        fft_line<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, wave_line);
+        transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, tw4, wave_line);
         if (a.scale1 != 1.0) {
             const R s1 = (R)a.scale1;
 #pragma unroll
@@ -829,11 +950,34 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
         // before the transform so that the loads overlap it (both tables
         // always exist)
         double capv[VPT], mskv[VPT];
+        double Vco = 0.0, mko = 1.0;  // CAP and mask of the other axes (same order of terms)
         if (POTK) {
 #pragma unroll
             for (int m = 0; m < VPT; ++m) {
                 capv[m] = P.capt[P.off[ax] + t + m * T];
                 mskv[m] = P.maskt[P.off[ax] + t + m * T];
+            }
+#pragma unroll
+            for (int j = 0; j < QND_MAXD; ++j) {
+                if (j < P.dim && j != ax) {
+                    if (P.cap_on) Vco += P.capt[P.off[j] + idx[j]];
+                    if (P.mask_on) mko *= P.maskt[P.off[j] + idx[j]];
+                }
+            }
+        }
+        // this instance's stage field (E in the length gauge, A in the
+        // velocity gauge): kernel arguments, or a row of the batch tables
+        // requested here so that the load overlaps the transform
+        double fv[QND_MAXD];
+        {
+            const bool use_a = (KIND == PK_KIN) || a.kmode == KM_VELOCITY;
+            if (a.B.fld) {
+                const double *src = a.B.fld + (L / a.B.lpi) * a.B.fld_stride;
+#pragma unroll
+                for (int j = 0; j < QND_MAXD; ++j) fv[j] = (j < P.dim) ? src[j] : 0.0;
+            } else {
+#pragma unroll
+                for (int j = 0; j < QND_MAXD; ++j) fv[j] = (j < P.dim) ? (use_a ? a.Ak[j] : a.Ev[j]) : 0.0;
             }
         }
 
@@ -848,17 +992,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
         }
 
         // ---- first transform ----
-        fft_line<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, wave_line);
+        transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, tw4, wave_line);
 
-        // this instance's stage field (E in the length gauge, A in the velocity gauge)
-        double fv[QND_MAXD];
-        {
-            const bool use_a = (KIND == PK_KIN) || a.kmode == KM_VELOCITY;
-            const double *src = use_a ? a.Ak : a.Ev;
-            if (a.B.fld) src = a.B.fld + (L / a.B.lpi) * a.B.fld_stride;
-#pragma unroll
-            for (int j = 0; j < QND_MAXD; ++j) fv[j] = (j < P.dim) ? src[j] : 0.0;
-        }
         if (KIND == PK_KIN) {
             // all axes in k-space: kinetic factor (reference order of the sums)
             double Ko = 0.0, Wk = 0.0;
@@ -871,18 +1006,32 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                 }
             }
             const double Aax = fv[ax];
+            // one loop per mode (mode tests outside the loop over the points)
+            if (a.kmode == KM_IMAG) {
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) {
-                const double p = coord_p(P, ax, t + m * T);
-                const double K = Ko + .5 * p * p;
-                if (a.kmode == KM_IMAG) x[m] = scl<R>(x[m], real_exp<R>(-a.ck_mid * K));
-                else if (a.kmode == KM_VELOCITY)
+                for (int m = 0; m < VPT; ++m) {
+                    const double p = coord_p(P, ax, t + m * T);
+                    const double K = Ko + .5 * p * p;
+                    x[m] = scl<R>(x[m], real_exp<R>(-a.ck_mid * K));
+                }
+            } else if (a.kmode == KM_VELOCITY) {
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) {
+                    const double p = coord_p(P, ax, t + m * T);
+                    const double K = Ko + .5 * p * p;
                     x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_mid * (K + (Wk + (p + .5 * Aax) * Aax))));
-                else x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_mid * K));
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) {
+                    const double p = coord_p(P, ax, t + m * T);
+                    const double K = Ko + .5 * p * p;
+                    x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_mid * K));
+                }
             }
         } else {
             // all axes in r-space: potential, CAP, field, mask, observables
-            double rro = 0.0, Vco = 0.0, Wv = 0.0, mko = 1.0;
+            double rro = 0.0, Wv = 0.0;
             double xo[QND_MAXD] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int j = 0; j < QND_MAXD; ++j) {
@@ -891,73 +1040,130 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                     xo[j] = xj;
                     rro += xj * xj;
                     Wv += xj * fv[j];
-                    if (P.cap_on) Vco += P.capt[P.off[j] + idx[j]];
-                    if (P.mask_on) mko *= P.maskt[P.off[j] + idx[j]];
                 }
             }
             double obs[QND_NOBS];
 #pragma unroll
             for (int q = 0; q < QND_NOBS; ++q) obs[q] = 0.0;
-            const double s1 = a.scale1;
             const bool need_v = (fl & (PF_V1 | PF_V2 | PF_OBS | PF_SRC)) != 0;
+            // Groups of four points: each operator is one unrolled loop over the
+            // group with its mode tests outside, so the points are independent
+            // chains the scheduler can interleave (a group bounds the registers
+            // this takes).  Every point sees the operators in the same order
+            // (potential, adjoint terms, V1, mask, observables, V2), and the
+            // partial sums take the points in order m = 0 .. VPT-1.
+            constexpr int G = 4;
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) {
-                C<R> v = scl<R>(x[m], (R)s1);
-                const int64_t gi = lbase + (int64_t)(t + m * T) * estride;
-                const double xc = coord_x(P, ax, t + m * T);
-                double dVdu = 0.0, V = 0.0;
-                if (need_v) V = potential<R>(P, xc * xc + rro, dVdu);
-                const double Vc = P.cap_on ? (capv[m] + Vco) : 0.0;
+            for (int m0 = 0; m0 < VPT; m0 += G) {
+                C<R> v[G];
+                double xc[G], rr[G], V[G], dVdu[G], Vc[G];
+#pragma unroll
+                for (int i = 0; i < G; ++i) {
+                    const int m = m0 + i;
+                    v[i] = scl<R>(x[m], (R)a.scale1);
+                    xc[i] = coord_x(P, ax, t + m * T);
+                    rr[i] = xc[i] * xc[i] + rro;
+                    V[i] = 0.0;
+                    dVdu[i] = 0.0;
+                    Vc[i] = P.cap_on ? (capv[m] + Vco) : 0.0;
+                }
+                if (need_v) potential_pts<R, G>(P, rr, V, dVdu);
                 if (fl & (PF_GRAD | PF_SRC)) {
-                    // adjoint: u = costate here; phi / y = forward state saved here
-                    const C<R> f = a.save[gi];
-                    double dV_da, dVdu_da;
-                    softcore_derivs(P, xc * xc + rro, dV_da, dVdu_da);
-                    if (fl & PF_GRAD) {
-                        const double im = (double)v.x * (double)f.y - (double)v.y * (double)f.x;
-                        obs[0] += dV_da * im;
 #pragma unroll
-                        for (int j = 0; j < QND_MAXD; ++j)
-                            if (j < P.dim) obs[1 + j] += ((j == ax) ? xc : xo[j]) * im;
-                    } else {
-                        double wsum = a.src_norm, dwsum = 0.0;
+                    for (int i = 0; i < G; ++i) {
+                        const int m = m0 + i;
+                        // adjoint: u = costate here; phi / y = forward state saved here
+                        const C<R> f = a.save[lbase + (int64_t)(t + m * T) * estride];
+                        double dV_da, dVdu_da;
+                        softcore_derivs(P, rr[i], dV_da, dVdu_da);
+                        if (fl & PF_GRAD) {
+                            const double im = (double)v[i].x * (double)f.y - (double)v[i].y * (double)f.x;
+                            obs[0] += dV_da * im;
 #pragma unroll
-                        for (int j = 0; j < QND_MAXD; ++j) {
-                            if (j < P.dim) {
-                                const double xj = (j == ax) ? xc : xo[j];
-                                wsum += a.src[j] * (-2.0 * xj * dVdu - fv[j]);
-                                dwsum += a.src[j] * (-2.0 * xj * dVdu_da);
+                            for (int j = 0; j < QND_MAXD; ++j)
+                                if (j < P.dim) obs[1 + j] += ((j == ax) ? xc[i] : xo[j]) * im;
+                        } else {
+                            double wsum = a.src_norm, dwsum = 0.0;
+#pragma unroll
+                            for (int j = 0; j < QND_MAXD; ++j) {
+                                if (j < P.dim) {
+                                    const double xj = (j == ax) ? xc[i] : xo[j];
+                                    wsum += a.src[j] * (-2.0 * xj * dVdu[i] - fv[j]);
+                                    dwsum += a.src[j] * (-2.0 * xj * dVdu_da);
+                                }
                             }
+                            v[i] = mk<R>((R)((double)v[i].x + wsum * (double)f.x),
+                                         (R)((double)v[i].y + wsum * (double)f.y));
+                            obs[0] += dwsum * ((double)f.x * (double)f.x + (double)f.y * (double)f.y);
                         }
-                        v = mk<R>((R)((double)v.x + wsum * (double)f.x), (R)((double)v.y + wsum * (double)f.y));
-                        obs[0] += dwsum * ((double)f.x * (double)f.x + (double)f.y * (double)f.y);
                     }
                 }
                 if (fl & PF_V1) {
-                    if (a.vmode == VM_IMAG) v = scl<R>(v, real_exp<R>(-a.cv1 * V));
-                    else if (a.vmode == VM_LENGTH)
-                        v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, cs * a.cv1 * (V + (xc * fv[ax] + Wv))));
-                    else v = fac_mul<R>(v, phase_factor<R>(-a.cv1 * Vc, cs * a.cv1 * V));
-                }
-                if ((fl & PF_MASK) && P.mask_on) v = scl<R>(v, (R)(mskv[m] * mko));
-                if (fl & PF_OBS) {
-                    const double d = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-                    obs[0] += d;
+                    if (a.vmode == VM_IMAG) {
 #pragma unroll
-                    for (int j = 0; j < QND_MAXD; ++j) {
-                        if (j < P.dim) {
-                            const double xj = (j == ax) ? xc : xo[j];
-                            obs[1 + j] += d * xj;
-                            obs[1 + QND_MAXD + j] += d * dVdu * 2.0 * xj;
+                        for (int i = 0; i < G; ++i) {
+                            v[i] = scl<R>(v[i], real_exp<R>(-a.cv1 * V[i]));
+                        }
+                    } else if (a.vmode == VM_LENGTH) {
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv1 * Vc[i],
+                                                                    cs * a.cv1 * (V[i] + (xc[i] * fv[ax] + Wv))));
+                        }
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv1 * Vc[i], cs * a.cv1 * V[i]));
+                        }
+                    }
+                }
+                if ((fl & PF_MASK) && P.mask_on) {
+#pragma unroll
+                    for (int i = 0; i < G; ++i) {
+                        const int m = m0 + i;
+                        v[i] = scl<R>(v[i], (R)(mskv[m] * mko));
+                    }
+                }
+                if (fl & PF_OBS) {
+#pragma unroll
+                    for (int i = 0; i < G; ++i) {
+                        const double d = (double)v[i].x * (double)v[i].x + (double)v[i].y * (double)v[i].y;
+                        obs[0] += d;
+#pragma unroll
+                        for (int j = 0; j < QND_MAXD; ++j) {
+                            if (j < P.dim) {
+                                const double xj = (j == ax) ? xc[i] : xo[j];
+                                obs[1 + j] += d * xj;
+                                obs[1 + QND_MAXD + j] += d * dVdu[i] * 2.0 * xj;
+                            }
                         }
                     }
                 }
                 if (fl & PF_V2) {
-                    if (a.vmode == VM_IMAG) v = scl<R>(v, real_exp<R>(-a.cv2 * V));
-                    else v = fac_mul<R>(v, phase_factor<R>(-a.cv2 * Vc, cs * a.cv2 * V));
+                    if (a.vmode == VM_IMAG) {
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            v[i] = scl<R>(v[i], real_exp<R>(-a.cv2 * V[i]));
+                        }
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv2 * Vc[i], cs * a.cv2 * V[i]));
+                        }
+                    }
                 }
-                if (fl & PF_SAVE) a.save[gi] = v;
-                x[m] = v;
+                if (fl & PF_SAVE) {
+#pragma unroll
+                    for (int i = 0; i < G; ++i) {
+                        const int m = m0 + i;
+                        a.save[lbase + (int64_t)(t + m * T) * estride] = v[i];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < G; ++i) {
+                    const int m = m0 + i;
+                    x[m] = v[i];
+                }
             }
             if (fl & (PF_OBS | PF_GRAD | PF_SRC))
                 block_sum_vec<QND_NOBS>(obs, red, a.partials + (int64_t)tb * a.npart);
@@ -965,7 +1171,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
 
         // ---- second transform ----
         if (KIND == PK_KIN || (fl & PF_SECOND)) {
-            fft_line<R, -DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, wave_line);
+            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, tw4, wave_line);
             const R s2 = (R)a.scale2;
             if (KIND == PK_POT1D && (fl & PF_KPOST)) {
 #pragma unroll
