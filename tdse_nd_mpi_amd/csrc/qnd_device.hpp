@@ -217,21 +217,32 @@ template <> __device__ __forceinline__ C<float> fac_mul<float>(C<float> x, const
 // again only after every thread has passed the barrier that follows its
 // reads).  `wc` counts the exchanges of the kernel and picks the buffer.
 // ---------------------------------------------------------------------
-// Butterflies of one stage given its twiddles w[b*RAD + r] (r >= 1; unused
-// when Ns = 1).
-template <typename R, int RAD, int DIR, int VPT>
+// Butterflies of one stage; twf(b, r) gives the twiddle of input r (>= 1) of
+// butterfly b (not called when Ns = 1).  IN: inputs from the LDS buffer lin (else from registers, the
+// first stage); OUT: outputs to the LDS buffer lout (else to registers, the
+// last stage).  Both are compile-time, so the element moves carry no tests.
+template <typename R, int RAD, int DIR, int VPT, bool IN, bool OUT, class TWF>
 __device__ __forceinline__ void stage_core(C<R> (&x)[VPT], const C<R> *lin, C<R> *lout, int n, int T, int t,
-                                           int Ns, const typename TwT<R>::T (&w)[VPT])
+                                           int Ns, const TWF &twf)
 {
     if constexpr (VPT % RAD == 0) {
         constexpr int NB = VPT / RAD;
         const int nR = n / RAD;
+        typename TwT<R>::T w[VPT];
+        if (Ns > 1) {  // twiddles first: a table's load latency overlaps the LDS reads
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 1; r < RAD; ++r) w[b * RAD + r] = twf(b, r);
+        }
         C<R> y[VPT];
 #pragma unroll
         for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int r = 0; r < RAD; ++r)
-                y[b * RAD + r] = lin ? lin[t + b * T + r * nR] : x[b + r * NB];
+            for (int r = 0; r < RAD; ++r) {
+                if constexpr (IN) y[b * RAD + r] = lin[t + b * T + r * nR];
+                else y[b * RAD + r] = x[b + r * NB];
+            }
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             C<R> *yb = y + b * RAD;
@@ -241,7 +252,7 @@ __device__ __forceinline__ void stage_core(C<R> (&x)[VPT], const C<R> *lin, C<R>
             }
             dftR<R, RAD, DIR>(yb);
         }
-        if (lout) {
+        if constexpr (OUT) {
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int j = t + b * T, k = j & (Ns - 1);
@@ -258,77 +269,11 @@ __device__ __forceinline__ void stage_core(C<R> (&x)[VPT], const C<R> *lin, C<R>
     }
 }
 
-// One stage with its twiddles read from the table tw (n entries).
-template <typename R, int RAD, int DIR, int VPT>
-__device__ __forceinline__ void reg_stage(C<R> (&x)[VPT], const C<R> *lin, C<R> *lout, int n, int T, int t,
-                                          int Ns, const typename TwT<R>::T *__restrict__ tw)
-{
-    if constexpr (VPT % RAD == 0) {
-        constexpr int NB = VPT / RAD;
-        const int stride = n / (Ns * RAD);
-        typename TwT<R>::T w[VPT];
-        if (Ns > 1) {  // twiddles first: their latency overlaps the LDS reads
-#pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int k = (t + b * T) & (Ns - 1);
-#pragma unroll
-                for (int r = 1; r < RAD; ++r) w[b * RAD + r] = tw[r * k * stride];
-            }
-        }
-        stage_core<R, RAD, DIR, VPT>(x, lin, lout, n, T, t, Ns, w);
-    }
-}
-
-// Full transform (n = 2^m >= 8) of the line whose two LDS buffers are
-// bufA / bufB.  Eight values per thread: radix-8 stages, then radix 4 or 2
-// for the remainder; four values per thread (twice the threads, for grids
-// too small to fill the chip otherwise): radix-4 stages, then radix 2 if
-// log2 n is odd; twelve values per thread (n = 3 * 2^k): radix-4 stages,
-// radix 2 if k is odd, radix 3 last.  Every thread of the block calls it
-// (barriers inside).
-template <typename R, int DIR, int VPT>
-__device__ __forceinline__ void fft_line(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int &wc, int n, int log2n,
-                                         int T, int t, const typename TwT<R>::T *__restrict__ tw,
-                                         bool wave_line = false)
-{
-    int rad[12], nst = 0;
-    if (VPT == 8) {
-        for (int s = 0; s < log2n / 3; ++s) rad[nst++] = 8;
-        if (log2n % 3 == 2) rad[nst++] = 4;
-        else if (log2n % 3 == 1) rad[nst++] = 2;
-    } else {
-        // VPT 4: n = 2^log2n.  VPT 12: n = 3 * 2^log2n (log2n is then the
-        // exponent of the power-of-two part), the radix-3 stage last so that
-        // every span Ns before it is a power of two
-        for (int s = 0; s < log2n / 2; ++s) rad[nst++] = 4;
-        if (log2n & 1) rad[nst++] = 2;
-        if (VPT == 12) rad[nst++] = 3;
-    }
-    const C<R> *lin = nullptr;
-    int Ns = 1;
-    for (int s = 0; s < nst; ++s) {
-        const bool last = (s == nst - 1);
-        C<R> *lout = last ? nullptr : ((wc & 1) ? bufB : bufA);
-        if (rad[s] == 8) reg_stage<R, 8, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
-        else if (rad[s] == 4) reg_stage<R, 4, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
-        else if (rad[s] == 3) reg_stage<R, 3, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
-        else reg_stage<R, 2, DIR, VPT>(x, lin, lout, n, T, t, Ns, tw);
-        if (!last) {
-            if (wave_line) wave_fence();
-            else lds_barrier();
-            lin = lout;
-            ++wc;
-        }
-        Ns *= rad[s];
-    }
-}
-
-// Four values per thread (n = 2^log2n <= 2048, T = n/4 threads): the stage
-// schedule is fixed at compile time (radix-4 stage s has span 4^s; radix 2
-// last when log2 n is odd) and every twiddle a thread needs is read once, at
-// kernel entry, into registers.  Both transforms of a pass share them (the
-// direction only conjugates), so no stage waits on a table load: these
-// grids run one wavefront per SIMD, where each such wait is exposed.
+// Twiddles of four-value-per-thread lines (n = 2^log2n <= 2048, T = n/4):
+// every twiddle a thread needs is read once, at kernel entry, into registers.
+// Both transforms of a pass share them (the direction only conjugates), so
+// no stage waits on a table load: these grids run one wavefront per SIMD,
+// where each such wait is exposed.
 #define QND_TW4_STAGES 4   // radix-4 stages with twiddles: s = 1..4 (n <= 2048)
 template <typename R> struct Tw4 {
     typename TwT<R>::T w4[QND_TW4_STAGES][3];  // stage s: W^{r k n/(4 Ns)}, k = t mod Ns, r = 1..3
@@ -353,48 +298,108 @@ __device__ __forceinline__ void tw4_load(Tw4<R> &P, const typename TwT<R>::T *__
         P.w2[1] = tw[t + T];
     }
 }
-template <typename R, int DIR>
-__device__ __forceinline__ void fft_line4(C<R> (&x)[4], C<R> *bufA, C<R> *bufB, int &wc, int n, int log2n, int T,
-                                          int t, const Tw4<R> &P, bool wave_line)
-{
-    const int ns4 = log2n >> 1;
-    const bool odd = (log2n & 1) != 0;
-    typename TwT<R>::T w[4];
-    const C<R> *lin = nullptr;
-#pragma unroll
-    for (int s = 0; s <= QND_TW4_STAGES; ++s) {
-        if (s < ns4) {
-            const bool last = !odd && s == ns4 - 1;
-            C<R> *lout = last ? nullptr : ((wc & 1) ? bufB : bufA);
-            if (s > 0) {
-#pragma unroll
-                for (int r = 1; r < 4; ++r) w[r] = P.w4[s - 1][r - 1];
+
+// Stage schedule of a line of n = 2^L points (VPT 4 or 8) or n = 3 * 2^L
+// points (VPT 12), fixed at compile time:
+//   VPT 4:  radix-4 stages, then radix 2 if L is odd;
+//   VPT 8:  radix-8 stages, then radix 4 or 2 for the remainder;
+//   VPT 12: radix-4 stages, radix 2 if L is odd, radix 3 last (every span Ns
+//           before it is then a power of two, so k = j mod Ns is a mask).
+template <int VPT, int L> struct Sched {
+    static constexpr int MAINR = (VPT == 8) ? 8 : 4;
+    static constexpr int NMAIN = (VPT == 8) ? L / 3 : L / 2;
+    static constexpr int REM = (VPT == 8) ? L % 3 : (L & 1);
+    static constexpr int NST = NMAIN + (REM != 0 ? 1 : 0) + (VPT == 12 ? 1 : 0);
+    static constexpr int N = (VPT == 12 ? 3 : 1) << L;
+    static constexpr int rad(int s)
+    {
+        return s < NMAIN ? MAINR : (s == NMAIN && REM != 0) ? ((VPT == 8 && REM == 2) ? 4 : 2) : 3;
+    }
+    static constexpr int span(int s) { return s == 0 ? 1 : span(s - 1) * rad(s - 1); }
+};
+// Stage S of the schedule, then the rest (compile-time recursion).  The
+// first stage reads registers, the last writes them, the others exchange
+// through the two LDS buffers: stage S of a transform that starts at the
+// kernel's exchange count wc0 writes buffer (wc0 + S) mod 2.  Twiddles come
+// from the registers of Tw4 (VPT 4) or from the table tw.
+template <typename R, int DIR, int VPT, int L, bool WL, int S>
+struct FftStages {
+    static __device__ __forceinline__ void run(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int wc0, int t,
+                                               const typename TwT<R>::T *__restrict__ tw, const Tw4<R> &P)
+    {
+        using Sc = Sched<VPT, L>;
+        if constexpr (S < Sc::NST) {
+            constexpr int n = Sc::N, T = n / VPT, RAD = Sc::rad(S), Ns = Sc::span(S);
+            constexpr bool IN = S > 0, OUT = S < Sc::NST - 1;
+            const C<R> *lin = ((wc0 + S - 1) & 1) ? bufB : bufA;
+            C<R> *lout = ((wc0 + S) & 1) ? bufB : bufA;
+            if constexpr (VPT == 4 && RAD == 4) {
+                constexpr int SW = S > 0 ? S - 1 : 0;
+                auto twf = [&](int, int r) { return P.w4[SW][r - 1]; };
+                stage_core<R, RAD, DIR, VPT, IN, OUT>(x, lin, lout, n, T, t, Ns, twf);
+            } else if constexpr (VPT == 4) {
+                auto twf = [&](int b, int) { return P.w2[b]; };
+                stage_core<R, RAD, DIR, VPT, IN, OUT>(x, lin, lout, n, T, t, Ns, twf);
+            } else {
+                constexpr int stride = n / (Ns * RAD);
+                auto twf = [&](int b, int r) { return tw[r * ((t + b * T) & (Ns - 1)) * stride]; };
+                stage_core<R, RAD, DIR, VPT, IN, OUT>(x, lin, lout, n, T, t, Ns, twf);
             }
-            stage_core<R, 4, DIR, 4>(x, s ? lin : nullptr, lout, n, T, t, 1 << (2 * s), w);
-            if (!last) {
-                if (wave_line) wave_fence();
+            if constexpr (OUT) {
+                if constexpr (WL) wave_fence();
                 else lds_barrier();
-                lin = lout;
-                ++wc;
             }
+            FftStages<R, DIR, VPT, L, WL, S + 1>::run(x, bufA, bufB, wc0, t, tw, P);
         }
     }
-    if (odd) {
-        w[1] = P.w2[0];
-        w[3] = P.w2[1];
-        stage_core<R, 2, DIR, 4>(x, lin, nullptr, n, T, t, n >> 1, w);
+};
+template <typename R, int DIR, int VPT, int L>
+__device__ __forceinline__ void fft_fixed(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int &wc, int t,
+                                          const typename TwT<R>::T *__restrict__ tw, const Tw4<R> &P,
+                                          bool wave_line)
+{
+    // wavefront-local exchanges exist only for 64-thread lines
+    if constexpr (Sched<VPT, L>::N / VPT == 64) {
+        if (wave_line) FftStages<R, DIR, VPT, L, true, 0>::run(x, bufA, bufB, wc, t, tw, P);
+        else FftStages<R, DIR, VPT, L, false, 0>::run(x, bufA, bufB, wc, t, tw, P);
+    } else {
+        FftStages<R, DIR, VPT, L, false, 0>::run(x, bufA, bufB, wc, t, tw, P);
     }
+    wc += Sched<VPT, L>::NST - 1;  // exchanges of the transform
 }
 
-// The transform of a pass: the preloaded-twiddle schedule for four values
-// per thread, the table-driven one otherwise.
+// Full transform of the line whose two LDS buffers are bufA / bufB: one
+// uniform switch on the line length picks its fixed schedule.  log2n is
+// log2 n (VPT 4, 8; n <= 2048 resp. 4096) or log2(n/3) (VPT 12, n <= 3072).
+// Every thread of the block calls it (barriers inside).
 template <typename R, int DIR, int VPT>
 __device__ __forceinline__ void transform(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int &wc, int n, int log2n, int T,
                                           int t, const typename TwT<R>::T *__restrict__ tw, const Tw4<R> &P,
                                           bool wave_line)
 {
-    if constexpr (VPT == 4) fft_line4<R, DIR>(x, bufA, bufB, wc, n, log2n, T, t, P, wave_line);
-    else fft_line<R, DIR, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, wave_line);
+#define QND_FFT_CASE(Lc)                                                                                            \
+    case Lc: fft_fixed<R, DIR, VPT, Lc>(x, bufA, bufB, wc, t, tw, P, wave_line); break;
+    if constexpr (VPT == 4) {
+        switch (log2n) {
+            QND_FFT_CASE(3) QND_FFT_CASE(4) QND_FFT_CASE(5) QND_FFT_CASE(6) QND_FFT_CASE(7) QND_FFT_CASE(8)
+            QND_FFT_CASE(9) QND_FFT_CASE(10) default: fft_fixed<R, DIR, VPT, 11>(x, bufA, bufB, wc, t, tw, P,
+                                                                                  wave_line);
+        }
+    } else if constexpr (VPT == 8) {
+        switch (log2n) {
+            QND_FFT_CASE(3) QND_FFT_CASE(4) QND_FFT_CASE(5) QND_FFT_CASE(6) QND_FFT_CASE(7) QND_FFT_CASE(8)
+            QND_FFT_CASE(9) QND_FFT_CASE(10) QND_FFT_CASE(11) default: fft_fixed<R, DIR, VPT, 12>(x, bufA, bufB, wc,
+                                                                                                   t, tw, P,
+                                                                                                   wave_line);
+        }
+    } else {
+        switch (log2n) {
+            QND_FFT_CASE(2) QND_FFT_CASE(3) QND_FFT_CASE(4) QND_FFT_CASE(5) QND_FFT_CASE(6) QND_FFT_CASE(7)
+            QND_FFT_CASE(8) QND_FFT_CASE(9) default: fft_fixed<R, DIR, VPT, 10>(x, bufA, bufB, wc, t, tw, P,
+                                                                                 wave_line);
+        }
+    }
+#undef QND_FFT_CASE
 }
 
 // ---------------------------------------------------------------------
