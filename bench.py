@@ -360,6 +360,36 @@ def big3d_run(q, F, imag_steps, real_steps):
             "steps_timed": {"imag": imag_steps, "real": real_steps}}
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_passes_ne2d_512x512_c64.txt")
+
+
+def measured_traffic(kernel="void qnd::k_pass<float, 2, 1, 4>"):
+    """HBM bytes per launch of `kernel` from the committed counter profile
+    (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, of `bench.py --traffic-probe`):
+    each counter's KiB per dispatch scaled by its reading for the calibration
+    copy in the same run (an FP64 -> FP32 conversion of 32 Mi elements: 256 MiB
+    read, 128 MiB written).  None when the profile is absent."""
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    vals = {}
+    for line in open(TRAFFIC_PROFILE):
+        f = line.split()
+        if len(f) < 4 or f[-3] not in ("FETCH_SIZE", "WRITE_SIZE"):
+            continue
+        ctr, mean = f[-3], float(f[-2])
+        if "elementwise_kernel" in line and f[-4] == "8388608":
+            vals[("cal", ctr)] = mean
+        elif line.strip().startswith(kernel + " ") or (kernel + "  ") in line:
+            vals[("k", ctr)] = mean
+    try:
+        fetch = vals[("k", "FETCH_SIZE")] * (256 * 1024) / vals[("cal", "FETCH_SIZE")]
+        write = vals[("k", "WRITE_SIZE")] * (128 * 1024) / vals[("cal", "WRITE_SIZE")]
+    except KeyError:
+        return None
+    return {"bytes": (fetch + write) * 1024, "fetch_bytes": fetch * 1024, "write_bytes": write * 1024,
+            "source": os.path.relpath(TRAFFIC_PROFILE, ROOT) + " (default launch geometry)"}
+
+
 def spectrum(a, dt):
     w = np.hanning(len(a))
     return np.abs(np.fft.rfft((a - a.mean()) * w)) ** 2
@@ -509,7 +539,9 @@ def main():
     bytes_launch = 2 * N * 8
     ach = bytes_launch / (avg_ms * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "kernel": names[k], "achieved": ach, "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                       "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                       "traffic": (tr["bytes"] if (tr := measured_traffic()) else None),
+                       "traffic_detail": tr,
                        "avg_launch_us": avg_ms * 1e3, "bytes_per_launch": bytes_launch,
                        "launches_per_step": {names[i]: float(cnt[i]) / args.profile_steps for i in range(4)},
                        "class_ms_per_step": {names[i]: float(ms[i]) / args.profile_steps for i in range(4)}}

@@ -292,10 +292,14 @@ template <typename R> PassArgs<R> base_args(qnd_ctx *c, C<R> *data, const Geo &g
 template <typename R, int KIND, int DIR, int VPT>
 void launch_vpt(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, hipEvent_t e0, hipEvent_t e1)
 {
+    // leading scalars: what the addresses of the first loads need
+    const unsigned geo = pass_geo(a.log2n, a.log2T, a.log2TW, a.P.pow2);
     if (e0)
         hipExtLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, e0, e1,
-                              0, a);
-    else hipLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, a);
+                              0, a.data, a.tw, a.S, geo, a.flags, (int)g.grid, a);
+    else
+        hipLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, a.data,
+                           a.tw, a.S, geo, a.flags, (int)g.grid, a);
 }
 
 template <typename R, int KIND, int DIR>
@@ -454,12 +458,23 @@ template <typename R> int ensure_rep(qnd_ctx *c, int slot, int want)
     return 0;
 }
 
+// Workgroups (256 threads) of the grid-stride flat kernels over N points:
+// about `per` points per thread, clamped to [lo, hi], so that large grids
+// fill every SIMD with several wavefronts (a fixed 512-1024 groups left an
+// N = 512^3 reduction at 1 TB/s).  A reduction's order, hence its rounding,
+// is fixed for a given N.
+int flat_grid(int64_t N, int per, int lo, int hi)
+{
+    const int64_t g = N / (256 * (int64_t)per);
+    return (int)(g < lo ? lo : g > hi ? hi : g);
+}
+
 // Reduction helper: launch k_red and sum partials into d_small[SMALL_SUMS..]
 template <typename R>
 int reduce(qnd_ctx *c, int mode, const C<R> *a, C<R> *b, const C<R> *cc, int axis, double fs,
            const double *alpha_dev, int nq, double scale, double *host_out)
 {
-    const int grid = 512, npart = QND_NOBS;
+    const int grid = flat_grid(c->N, 8, 512, 8192), npart = QND_NOBS;
     int rc = ensure_part(c, (size_t)grid * npart);
     if (rc) return rc;
     {
@@ -482,7 +497,8 @@ int reduce(qnd_ctx *c, int mode, const C<R> *a, C<R> *b, const C<R> *cc, int axi
 template <typename R> int elem(qnd_ctx *c, int mode, C<R> *dst, const C<R> *src, int axis, double shift)
 {
     ProfScope ps(c, PC_SMALL);
-    hipLaunchKernelGGL(k_elem<R>, dim3(1024), dim3(256), 0, c->stream, mode, dst, src, axis, shift,
+    hipLaunchKernelGGL(k_elem<R>, dim3(flat_grid(c->N, 4, 1024, 16384)), dim3(256),
+                       0, c->stream, mode, dst, src, axis, shift,
                        c->N, c->P, (const C<R> *)nullptr, (int64_t)0, 0, (const double *)nullptr);
     HIPCK(c, hipGetLastError());
     return 0;
@@ -693,7 +709,8 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
     }
     if (pending) {
         ProfScope ps(c, PC_SMALL);
-        hipLaunchKernelGGL(k_elem<R>, dim3(1024), dim3(256), 0, c->stream, (int)EM_GS_UPDATE, psi,
+        hipLaunchKernelGGL(k_elem<R>, dim3(flat_grid(c->N, 4, 1024, 16384)), dim3(256),
+                           0, c->stream, (int)EM_GS_UPDATE, psi,
                            (const C<R> *)psi, 0, 0.0, c->N, c->P, (const C<R> *)state_ptr<R>(c, 0),
                            c->N, nprev, (const double *)coef);
         HIPCK(c, hipGetLastError());
@@ -731,7 +748,8 @@ template <typename R> int gram_schmidt(qnd_ctx *c, int slot)
         HIPCK(c, hipMemcpyAsync(c->d_small + SMALL_COEF, cj.data(), 3 * sizeof(double),
                                 hipMemcpyHostToDevice, c->stream));
         ProfScope ps(c, PC_SMALL);
-        hipLaunchKernelGGL(k_elem<R>, dim3(1024), dim3(256), 0, c->stream, (int)EM_GS_UPDATE, v,
+        hipLaunchKernelGGL(k_elem<R>, dim3(flat_grid(c->N, 4, 1024, 16384)), dim3(256),
+                           0, c->stream, (int)EM_GS_UPDATE, v,
                            (const C<R> *)v, 0, 0.0, c->N, c->P, (const C<R> *)state_ptr<R>(c, j),
                            c->N, 1, (const double *)(c->d_small + SMALL_COEF));
         HIPCK(c, hipGetLastError());
@@ -1243,7 +1261,8 @@ int final_observables(qnd_ctx *c, int slot, const double *R_mask, const double *
     HIPCK(c, hipMemcpyAsync(r1, &R_mask[1], sizeof(double), hipMemcpyHostToDevice, c->stream));
     {
         ProfScope ps(c, PC_SMALL);
-        hipLaunchKernelGGL(k_elem<R>, dim3(1024), dim3(256), 0, c->stream, (int)EM_RMASK, A, (const C<R> *)psi, 0,
+        hipLaunchKernelGGL(k_elem<R>, dim3(flat_grid(c->N, 4, 1024, 16384)), dim3(256),
+                           0, c->stream, (int)EM_RMASK, A, (const C<R> *)psi, 0,
                            R_mask[0], N, c->P, (const C<R> *)nullptr, (int64_t)0, 0, (const double *)r1);
         HIPCK(c, hipGetLastError());
     }

@@ -885,56 +885,73 @@ template <int BYTES> __device__ __forceinline__ void kernarg_touch()
                  "s"(v[8]), "s"(v[9]), "s"(v[10]), "s"(v[11]));
 }
 
-template <typename R, int KIND, int DIR1, int VPT>
-__global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
+// The pass kernel's leading scalar arguments are what the addresses of its
+// first loads need: the state and twiddle pointers, the element stride S of
+// a line (1: contiguous lines), the packed shape `geo` (below), the flags
+// and the number of workgroups (gridDim.x would be read from the argument
+// block).
+// Built with kernel-argument preloading, they arrive in SGPRs with the
+// wavefront, so the state and twiddle loads go out before the argument block
+// (PassArgs) has been read.
+// geo = log2n | log2T << 4 | log2TW << 8 | pow2 << 12
+__host__ __device__ inline unsigned pass_geo(int log2n, int log2T, int log2TW, int pow2)
 {
-    kernarg_touch<sizeof(PassArgs<R>)>();
+    return (unsigned)log2n | ((unsigned)log2T << 4) | ((unsigned)log2TW << 8) | ((unsigned)(pow2 != 0) << 12);
+}
+
+template <typename R, int KIND, int DIR1, int VPT>
+__global__ void __launch_bounds__(QND_PASS_THREADS)
+    k_pass(C<R> *data, const typename TwT<R>::T *tw, int64_t S, unsigned geo, unsigned fl, int nblk,
+           const PassArgs<R> a)
+{
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[16 * QND_NOBS];
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
     constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D);
 
     const int tid = threadIdx.x;
-    const int n = a.n, T = a.T, TW = a.TW, ldsS = a.ldsStride;
+    const int log2n = geo & 15, log2T = (geo >> 4) & 15, log2TW = (geo >> 8) & 15;
+    const bool pow2 = (geo >> 12) & 1;
+    const int n = (VPT == 12 ? 3 : 1) << log2n, T = 1 << log2T, TW = 1 << log2TW;
     const Phys &P = a.P;
-    const unsigned fl = a.flags;
-
-    if (KIND == PK_KIN && (fl & PF_FINALIZE) && (int)blockIdx.x < a.B.nb)
-        obs_finalize_instance(a.fin_part, a.fin_nwg, P.dim, P.dNr, a.fin_spec, a.B, blockIdx.x, a.fin_rec, red);
 
     // ---- tile geometry: TW lines (TW | S or S == 1).  Contiguous lines put
     // t on the fastest lanes, strided lines the line index l, so that every
     // wavefront access covers consecutive addresses. ----
-    const bool contig = (a.S == 1);
+    const bool contig = (S == 1);
     int log2S = 0;
-    while (((int64_t)1 << log2S) < a.S) ++log2S;
+    while (((int64_t)1 << log2S) < S) ++log2S;
     int l, t;
-    if (contig) { l = tid >> a.log2T; t = tid & (T - 1); }
-    else { l = tid & (TW - 1); t = tid >> a.log2TW; }
+    if (contig) { l = tid >> log2T; t = tid & (T - 1); }
+    else { l = tid & (TW - 1); t = tid >> log2TW; }
     // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs,
     // so workgroup w runs on XCD w % 8.  Tile tb = (w % 8) * (grid / 8) + w / 8
     // gives each XCD a contiguous run of tiles: neighbouring strided tiles,
     // which share cache lines, then share one L2 instead of fetching the same
     // lines into several.
-    const int nblk = gridDim.x;
     const int tb = (nblk % 8 == 0) ? (int)(blockIdx.x % 8) * (nblk / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
     const int64_t L = (int64_t)tb * TW + l;  // this thread's line
     int64_t lbase, estride;                          // element i at lbase + i * estride
     if (contig) { lbase = L * n; estride = 1; }
-    else if (P.pow2) { lbase = (((L >> log2S) << a.log2n) << log2S) + (L & (a.S - 1)); estride = a.S; }
-    else { const int64_t o = L / a.S; lbase = o * n * a.S + (L - o * a.S); estride = a.S; }
-    C<R> *bufA = lds + l * ldsS;
-    C<R> *bufB = lds + (TW + l) * ldsS;
-    int wc = 0;
+    else if (pow2) { lbase = (((L >> log2S) << log2n) << log2S) + (L & (S - 1)); estride = S; }
+    else { const int64_t o = L / S; lbase = o * n * S + (L - o * S); estride = S; }
     Tw4<R> tw4;  // four values per thread: all twiddles requested before the state
-    if constexpr (VPT == 4) tw4_load<R>(tw4, a.tw, a.log2n, T, t);
+    if constexpr (VPT == 4) tw4_load<R>(tw4, tw, log2n, T, t);
 
     // ---- load x[m] = element t + m T (with the Gram-Schmidt update prologue).
     // Twiddles are read from the table in memory (L1/L2 hits); staging it in
     // LDS measured slower. ----
     C<R> x[VPT];
 #pragma unroll
-    for (int m = 0; m < VPT; ++m) x[m] = a.data[lbase + (int64_t)(t + m * T) * estride];
+    for (int m = 0; m < VPT; ++m) x[m] = data[lbase + (int64_t)(t + m * T) * estride];
+    // the argument block, now that the first loads are out
+    kernarg_touch<sizeof(PassArgs<R>) + 40>();
+    const int ldsS = a.ldsStride;
+    C<R> *bufA = lds + l * ldsS;
+    C<R> *bufB = lds + (TW + l) * ldsS;
+    int wc = 0;
+    if (KIND == PK_KIN && (fl & PF_FINALIZE) && (int)blockIdx.x < a.B.nb)
+        obs_finalize_instance(a.fin_part, a.fin_nwg, P.dim, P.dNr, a.fin_spec, a.B, blockIdx.x, a.fin_rec, red);
     if (POTK && (fl & PF_GS_UPDATE)) {
         const double gsinv = a.gs_coef[2 * a.nprev];
         double vr[VPT], vi[VPT];
@@ -962,7 +979,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
     // need no workgroup barrier
     const bool wave_line = contig && T == 64 && a.wave_lines;
     if (KIND == PK_PLAIN) {
-        transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, tw4, wave_line);
+        transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
         if (a.scale1 != 1.0) {
             const R s1 = (R)a.scale1;
 #pragma unroll
@@ -970,7 +987,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
         }
     } else {
         int idx[QND_MAXD];
-        line_index(P, ax, a.S, log2S, L, idx);
+        line_index(P, ax, S, log2S, L, idx);
         // per-axis CAP and mask values of this thread's elements, requested
         // before the transform so that the loads overlap it (both tables
         // always exist)
@@ -1017,7 +1034,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
         }
 
         // ---- first transform ----
-        transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, tw4, wave_line);
+        transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
 
         if (KIND == PK_KIN) {
             // all axes in k-space: kinetic factor (reference order of the sums)
@@ -1130,15 +1147,27 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                             v[i] = scl<R>(v[i], real_exp<R>(-a.cv1 * V[i]));
                         }
                     } else if (a.vmode == VM_LENGTH) {
+                        // without a CAP the damping argument is the constant 0 (no exponential)
+                        if (P.cap_on) {
 #pragma unroll
-                        for (int i = 0; i < G; ++i) {
-                            v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv1 * Vc[i],
-                                                                    cs * a.cv1 * (V[i] + (xc[i] * fv[ax] + Wv))));
+                            for (int i = 0; i < G; ++i)
+                                v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv1 * Vc[i],
+                                                                        cs * a.cv1 * (V[i] + (xc[i] * fv[ax] + Wv))));
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < G; ++i)
+                                v[i] = fac_mul<R>(v[i],
+                                                  phase_factor<R>(0.0, cs * a.cv1 * (V[i] + (xc[i] * fv[ax] + Wv))));
                         }
                     } else {
+                        if (P.cap_on) {
 #pragma unroll
-                        for (int i = 0; i < G; ++i) {
-                            v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv1 * Vc[i], cs * a.cv1 * V[i]));
+                            for (int i = 0; i < G; ++i)
+                                v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv1 * Vc[i], cs * a.cv1 * V[i]));
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < G; ++i)
+                                v[i] = fac_mul<R>(v[i], phase_factor<R>(0.0, cs * a.cv1 * V[i]));
                         }
                     }
                 }
@@ -1170,11 +1199,13 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
                         for (int i = 0; i < G; ++i) {
                             v[i] = scl<R>(v[i], real_exp<R>(-a.cv2 * V[i]));
                         }
+                    } else if (P.cap_on) {
+#pragma unroll
+                        for (int i = 0; i < G; ++i)
+                            v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv2 * Vc[i], cs * a.cv2 * V[i]));
                     } else {
 #pragma unroll
-                        for (int i = 0; i < G; ++i) {
-                            v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv2 * Vc[i], cs * a.cv2 * V[i]));
-                        }
+                        for (int i = 0; i < G; ++i) v[i] = fac_mul<R>(v[i], phase_factor<R>(0.0, cs * a.cv2 * V[i]));
                     }
                 }
                 if (fl & PF_SAVE) {
@@ -1196,7 +1227,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
 
         // ---- second transform ----
         if (KIND == PK_KIN || (fl & PF_SECOND)) {
-            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, a.log2n, T, t, a.tw, tw4, wave_line);
+            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
             const R s2 = (R)a.scale2;
             if (KIND == PK_POT1D && (fl & PF_KPOST)) {
 #pragma unroll
@@ -1218,7 +1249,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass(const PassArgs<R> a)
 
     // ---- store ----
 #pragma unroll
-    for (int m = 0; m < VPT; ++m) a.data[lbase + (int64_t)(t + m * T) * estride] = x[m];
+    for (int m = 0; m < VPT; ++m) data[lbase + (int64_t)(t + m * T) * estride] = x[m];
 
     // ---- Gram-Schmidt dots on the stored representation ----
     if (POTK && (fl & PF_GS_DOTS)) {
