@@ -294,12 +294,16 @@ void launch_vpt(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, hipEvent_t e0, h
 {
     // leading scalars: what the addresses of the first loads need
     const unsigned geo = pass_geo(a.log2n, a.log2T, a.log2TW, a.P.pow2);
+    // a finalizing kinetic pass appends one workgroup per instance after the
+    // line tiles (whose order and XCD placement stay as they are)
+    const int extra = (KIND == PK_KIN && (a.flags & PF_FINALIZE)) ? a.B.nb : 0;
+    const dim3 grid(g.grid + extra);
     if (e0)
-        hipExtLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, e0, e1,
-                              0, a.data, a.tw, a.S, geo, a.flags, (int)g.grid, a);
+        hipExtLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), grid, dim3(g.threads), g.lds, c->stream, e0, e1, 0,
+                              a.data, a.tw, a.S, geo, a.flags, (int)g.grid, a);
     else
-        hipLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), dim3(g.grid), dim3(g.threads), g.lds, c->stream, a.data,
-                           a.tw, a.S, geo, a.flags, (int)g.grid, a);
+        hipLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), grid, dim3(g.threads), g.lds, c->stream, a.data, a.tw,
+                           a.S, geo, a.flags, (int)g.grid, a);
 }
 
 template <typename R, int KIND, int DIR>
@@ -348,7 +352,7 @@ template <typename R> int full_fft(qnd_ctx *c, C<R> *data, int dir)
 // kinetic block for dim >= 2: FFT axes 1..d-2, kinetic pass on axis d-1,
 // inverse on axes d-2..1.
 // A pending observable record (partials of the previous axis-0 pass) is
-// finalized by block 0 of the kinetic pass when fin is given.
+// finalized by workgroups appended to the kinetic pass when fin is given.
 struct Finalize {
     const double *part;
     int nwg;         // workgroups per instance of the pass that wrote the partials

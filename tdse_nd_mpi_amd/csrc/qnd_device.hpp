@@ -673,7 +673,7 @@ enum PassFlags : unsigned {
     PF_SECOND = 1u << 6,     // second (reverse) transform
     PF_KPOST = 1u << 7,      // (dim = 1) kinetic factor after the second transform
     PF_GS_DOTS = 1u << 8,    // Gram-Schmidt partial dots after the second transform
-    PF_FINALIZE = 1u << 9,   // block 0 first turns the previous pass's observable partials into a record
+    PF_FINALIZE = 1u << 9,   // extra workgroups turn the previous pass's observable partials into records
     // adjoint (costate) passes
     PF_SAVE = 1u << 10,      // store the position-space state after the r-space operators into `save`
     PF_CONJ = 1u << 11,      // conjugate phase factors (damping unchanged): the adjoint of the pass
@@ -909,6 +909,15 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
     constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D);
 
+    // workgroups after the nblk line tiles finalize the previous pass's
+    // observable record (one per instance) and do no line work, so the
+    // record does not delay a tile
+    if (KIND == PK_KIN && (int)blockIdx.x >= nblk) {
+        const int b = (int)blockIdx.x - nblk;
+        if ((fl & PF_FINALIZE) && b < a.B.nb)
+            obs_finalize_instance(a.fin_part, a.fin_nwg, a.P.dim, a.P.dNr, a.fin_spec, a.B, b, a.fin_rec, red);
+        return;
+    }
     const int tid = threadIdx.x;
     const int log2n = geo & 15, log2T = (geo >> 4) & 15, log2TW = (geo >> 8) & 15;
     const bool pow2 = (geo >> 12) & 1;
@@ -950,8 +959,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     C<R> *bufA = lds + l * ldsS;
     C<R> *bufB = lds + (TW + l) * ldsS;
     int wc = 0;
-    if (KIND == PK_KIN && (fl & PF_FINALIZE) && (int)blockIdx.x < a.B.nb)
-        obs_finalize_instance(a.fin_part, a.fin_nwg, P.dim, P.dNr, a.fin_spec, a.B, blockIdx.x, a.fin_rec, red);
+
     if (POTK && (fl & PF_GS_UPDATE)) {
         const double gsinv = a.gs_coef[2 * a.nprev];
         double vr[VPT], vi[VPT];
