@@ -89,6 +89,14 @@ struct qnd_ctx {
     int adj_slot = -1, adj_prec = -1;
     int64_t setup_gen = 0, adj_gen = -1;  // potential/integrator/mask/field changes; the forward's value
     int64_t adj_nsteps = 0, adj_K = 0;
+    // tabulated kinetic factors (length gauge, four values per thread): one
+    // table of N entries per distinct kinetic coefficient of a step
+    void *d_ktab = nullptr;
+    double *d_kcoef = nullptr;
+    size_t ktab_cap = 0;
+    std::vector<double> ktab_coef;
+    int64_t ktab_gen = -1;
+    int ktab_prec = -1;
     int bnb = 0;
     int64_t bntime = 0;
     hipEvent_t ev[16] = {};
@@ -124,6 +132,28 @@ int fail(qnd_ctx *c, int code, const char *fmt, ...)
             return fail((c), QND_EHIP, "HIP error %s at %s:%d (%s)", hipGetErrorString(e_),  \
                         __FILE__, __LINE__, #call);                                           \
     } while (0)
+
+// HIPCK for functions that return a pointer: sets *rcp and returns nullptr
+#define HIPCK_P(c, call, rcp)                                                                 \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            *(rcp) = fail((c), QND_EHIP, "HIP error %s at %s:%d (%s)", hipGetErrorString(e_), \
+                          __FILE__, __LINE__, #call);                                         \
+            return nullptr;                                                                   \
+        }                                                                                     \
+    } while (0)
+
+// Workgroups (256 threads) of the grid-stride flat kernels over N points:
+// about `per` points per thread, clamped to [lo, hi], so that large grids
+// fill every SIMD with several wavefronts (a fixed 512-1024 groups left an
+// N = 512^3 reduction at 1 TB/s).  A reduction's order, hence its rounding,
+// is fixed for a given N.
+int flat_grid(int64_t N, int per, int lo, int hi)
+{
+    const int64_t g = N / (256 * (int64_t)per);
+    return (int)(g < lo ? lo : g > hi ? hi : g);
+}
 
 int ilog2(int64_t n)
 {
@@ -349,6 +379,54 @@ template <typename R> int full_fft(qnd_ctx *c, C<R> *data, int dir)
     return 0;
 }
 
+// Kinetic factor tables.  A length-gauge step (and its adjoint) uses s + 1
+// kinetic coefficients: a_0, a_{2i+1} + a_{2i+2} (i < s-1) and a_{2s-1}.  For
+// grids where a pass has four values per thread (one wavefront per SIMD,
+// latency-bound) the factors e^{-i c K} of every element are tabulated once
+// per set-up, so the kinetic pass reads its factor with the state instead
+// of forming it.  Tables above QND_KTAB_MAX_BYTES are not built (large
+// grids are not latency-bound, and the extra read would cost bandwidth).
+#define QND_KTAB_MAX_BYTES ((size_t)256 << 20)
+template <typename R>
+const typename TwT<R>::T *ktab_for(qnd_ctx *c, double ck, const Geo &g, int *rc)
+{
+    using T = typename TwT<R>::T;
+    *rc = 0;
+    if (g.vpt != 4 || c->dim < 2 || !c->have_int) return nullptr;
+    if (c->ktab_gen != c->setup_gen || c->ktab_prec != c->prec) {
+        c->ktab_gen = c->setup_gen;
+        c->ktab_prec = c->prec;
+        c->ktab_coef.clear();
+        const int s = c->s;
+        const double *as = c->as_scaled;
+        std::vector<double> coef{as[0]};
+        for (int i = 0; i < s - 1; ++i) coef.push_back(as[2 * i + 1] + as[2 * i + 2]);
+        coef.push_back(as[2 * s - 1] + 0.0);
+        const size_t bytes = coef.size() * (size_t)c->N * sizeof(T);
+        if (bytes > QND_KTAB_MAX_BYTES) return nullptr;
+        if (bytes > c->ktab_cap) {
+            if (c->d_ktab) HIPCK_P(c, hipFree(c->d_ktab), rc);
+            c->d_ktab = nullptr;
+            c->ktab_cap = 0;
+            HIPCK_P(c, hipMalloc(&c->d_ktab, bytes), rc);
+            c->ktab_cap = bytes;
+        }
+        if (!c->d_kcoef) HIPCK_P(c, hipMalloc(&c->d_kcoef, 64 * sizeof(double)), rc);
+        HIPCK_P(c, hipMemcpyAsync(c->d_kcoef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice,
+                                  c->stream), rc);
+        {
+            ProfScope ps(c, PC_SMALL);
+            hipLaunchKernelGGL(k_ktab<T>, dim3(flat_grid(c->N, 4, 64, 4096)), dim3(256), 0, c->stream,
+                               (T *)c->d_ktab, (const double *)c->d_kcoef, (int)coef.size(), c->N, c->P);
+            HIPCK_P(c, hipGetLastError(), rc);
+        }
+        c->ktab_coef = coef;
+    }
+    for (size_t q = 0; q < c->ktab_coef.size(); ++q)
+        if (c->ktab_coef[q] == ck) return (const T *)c->d_ktab + q * c->N;
+    return nullptr;
+}
+
 // kinetic block for dim >= 2: FFT axes 1..d-2, kinetic pass on axis d-1,
 // inverse on axes d-2..1.
 // A pending observable record (partials of the previous axis-0 pass) is
@@ -408,6 +486,11 @@ int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A,
     a.kmode = kmode;
     a.ck_mid = ck;
     a.scale2 = 1.0 / (double)c->nr[d - 1];
+    if (kmode == KM_PHASE) {
+        a.ktab = ktab_for<R>(c, ck, g, &rc);
+        if (rc) return rc;
+        if (a.ktab) a.flags |= PF_KTAB;
+    }
     if (A)
         for (int j = 0; j < d; ++j) a.Ak[j] = A[j];
     a.B.nb = nb;
@@ -460,17 +543,6 @@ template <typename R> int ensure_rep(qnd_ctx *c, int slot, int want)
         st.rep = REP_M;
     }
     return 0;
-}
-
-// Workgroups (256 threads) of the grid-stride flat kernels over N points:
-// about `per` points per thread, clamped to [lo, hi], so that large grids
-// fill every SIMD with several wavefronts (a fixed 512-1024 groups left an
-// N = 512^3 reduction at 1 TB/s).  A reduction's order, hence its rounding,
-// is fixed for a given N.
-int flat_grid(int64_t N, int per, int lo, int hi)
-{
-    const int64_t g = N / (256 * (int64_t)per);
-    return (int)(g < lo ? lo : g > hi ? hi : g);
 }
 
 // Reduction helper: launch k_red and sum partials into d_small[SMALL_SUMS..]
@@ -1113,7 +1185,11 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
         a.ck_mid = ck;
         a.scale2 = 1.0 / (double)c->nr[d - 1];
         a.flags = PF_CONJ;
-        if (int r = launch<R, PK_KIN, -1>(c, g, a, PC_KIN)) return r;
+        int r = 0;
+        a.ktab = ktab_for<R>(c, ck, g, &r);
+        if (r) return r;
+        if (a.ktab) a.flags |= PF_KTAB;
+        if ((r = launch<R, PK_KIN, -1>(c, g, a, PC_KIN))) return r;
         for (int ax = d - 2; ax >= 1; --ax)
             if (int r = plain_pass<R>(c, data, ax, +1, 1.0 / (double)c->nr[ax])) return r;
         return 0;
@@ -1568,6 +1644,8 @@ void qnd_destroy(qnd_ctx *c)
     if (c->d_bEt) hipFree(c->d_bEt);
     if (c->d_bAt) hipFree(c->d_bAt);
     if (c->d_bEo) hipFree(c->d_bEo);
+    if (c->d_ktab) hipFree(c->d_ktab);
+    if (c->d_kcoef) hipFree(c->d_kcoef);
     for (void *p : {c->d_adj_ckpt, c->d_adj_final, c->d_adj_save, (void *)c->d_adj_part})
         if (p) hipFree(p);
     for (int k = 0; k < 16; ++k)

@@ -589,6 +589,31 @@ template <> __device__ __forceinline__ Fac<float> phase_factor<float>(double arg
     f.lo = mk<float>(fmaf(g, c, -hx) + he * hx, -fmaf(g, s, hy) + he * hy);
     return f;
 }
+// A tabulated unit factor (cos, -sin) as a Fac: complex64 tables hold the
+// FP64 value as an FP32 hi + lo pair (x, y, z, w) = (hi.re, hi.im, lo.re,
+// lo.im); conj gives the adjoint's conjugated factor.
+template <typename R> __device__ __forceinline__ Fac<R> tab_fac(const typename TwT<R>::T &v, bool conj);
+template <> __device__ __forceinline__ Fac<double> tab_fac<double>(const double2 &v, bool conj)
+{
+    Fac<double> f;
+    f.v = mk<double>(v.x, conj ? -v.y : v.y);
+    return f;
+}
+template <> __device__ __forceinline__ Fac<float> tab_fac<float>(const float4 &v, bool conj)
+{
+    Fac<float> f;
+    f.hi = mk<float>(v.x, conj ? -v.y : v.y);
+    f.lo = mk<float>(v.z, conj ? -v.w : v.w);
+    return f;
+}
+// Store the unit factor (c, s) into a table entry (FP32 hi + lo, or FP64).
+__device__ __forceinline__ void tab_store(float4 *t, double c, double s)
+{
+    const float ch = (float)c, sh = (float)s;
+    *t = make_float4(ch, sh, (float)(c - (double)ch), (float)(s - (double)sh));
+}
+__device__ __forceinline__ void tab_store(double2 *t, double c, double s) { *t = make_double2(c, s); }
+
 template <typename R> __device__ __forceinline__ R real_exp(double arg);
 template <> __device__ __forceinline__ double real_exp<double>(double arg)
 {
@@ -679,6 +704,7 @@ enum PassFlags : unsigned {
     PF_CONJ = 1u << 11,      // conjugate phase factors (damping unchanged): the adjoint of the pass
     PF_GRAD = 1u << 12,      // partials of x_j Im(conj(u) phi) and dV/da Im(conj(u) phi), phi from `save`
     PF_SRC = 1u << 13,       // u += sum_j src_j (w_j - E_j) y + src_norm y (y from `save`), w = -grad V
+    PF_KTAB = 1u << 14,      // kinetic factor read from `ktab` (tabulated e^{-i c K}, four values per thread)
 };
 enum KMode { KM_PHASE = 0, KM_VELOCITY = 1, KM_IMAG = 2 };
 enum VMode { VM_LENGTH = 0, VM_VELOCITY = 1, VM_IMAG = 2 };
@@ -720,6 +746,9 @@ template <typename R> struct PassArgs {
     int vmode;
     double cv1, cv2;
     double Ev[QND_MAXD];
+    // kinetic factors e^{-i c K} of this pass's coefficient at every element of
+    // one instance (row-major), when tabulated (PF_KTAB)
+    const typename TwT<R>::T *ktab;
     // adjoint passes: forward states saved in position space, source weights
     C<R> *save;
     double src[QND_MAXD];
@@ -1031,6 +1060,17 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             }
         }
 
+        // tabulated kinetic factors of this thread's elements (contiguous
+        // lines of the last axis), requested before the transform
+        typename TwT<R>::T ktv[VPT];
+        if constexpr (KIND == PK_KIN && VPT == 4) {
+            if (fl & PF_KTAB) {
+                const int64_t e0 = (L % a.B.lpi) * n + t;
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) ktv[m] = a.ktab[e0 + m * T];
+            }
+        }
+
         if (KIND == PK_POT1D && (fl & PF_KPRE)) {
 #pragma unroll
             for (int m = 0; m < VPT; ++m) {
@@ -1057,7 +1097,10 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             }
             const double Aax = fv[ax];
             // one loop per mode (mode tests outside the loop over the points)
-            if (a.kmode == KM_IMAG) {
+            if (KIND == PK_KIN && VPT == 4 && (fl & PF_KTAB)) {
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(ktv[m], cs < 0.0));
+            } else if (a.kmode == KM_IMAG) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) {
                     const double p = coord_p(P, ax, t + m * T);
@@ -1399,6 +1442,35 @@ __device__ __forceinline__ void flat_index(const Phys &P, int64_t f, int *idx)
     }
 }
 // idx[j] for a runtime j without dynamic register indexing
+// Kinetic factor tables: entry f of table q is e^{-i c_q K(f)}, K = sum_j
+// p_j^2 / 2 over the row-major multi-index of f, with the terms in the
+// kinetic pass's order (the last axis added last) and the factor formed in
+// FP64.
+template <typename T>
+__global__ void __launch_bounds__(256) k_ktab(T *tab, const double *__restrict__ coef, int ncoef, int64_t N, Phys P)
+{
+    const int ax = P.dim - 1;
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < N; f += (int64_t)gridDim.x * blockDim.x) {
+        int idx[QND_MAXD];
+        flat_index(P, f, idx);
+        double Ko = 0.0;
+#pragma unroll
+        for (int j = 0; j < QND_MAXD; ++j) {
+            if (j < P.dim && j != ax) {
+                const double pj = coord_p(P, j, idx[j]);
+                Ko += .5 * pj * pj;
+            }
+        }
+        const double p = coord_p(P, ax, idx[ax]);
+        const double K = Ko + .5 * p * p;
+        for (int q = 0; q < ncoef; ++q) {
+            double sn, cn;
+            sincos(coef[q] * K, &sn, &cn);
+            tab_store(tab + (int64_t)q * N + f, cn, -sn);
+        }
+    }
+}
+
 __device__ __forceinline__ int pick(const int *idx, int j)
 {
     int v = idx[0];
