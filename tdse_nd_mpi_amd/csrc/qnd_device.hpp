@@ -190,28 +190,31 @@ template <> __device__ __forceinline__ C<float> twmul<float, +1>(C<float> x, flo
 // unitary to FP32 rounding instead of drifting by a per-angle bias.
 template <typename R> struct Fac;
 template <> struct Fac<double> { double2 v; };
-template <> struct Fac<float> { float2 hi, lo; };
+// complex64: the factor is (-i)^q (hi + lo), hi + lo an FP32 pair with
+// |hi.x| >= |hi.y| (the reduced angle, or a rotated table entry)
+template <> struct Fac<float> { float2 hi, lo; int q; };
 template <typename R> __device__ __forceinline__ C<R> fac_mul(C<R> x, const Fac<R> &f);
 template <> __device__ __forceinline__ C<double> fac_mul<double>(C<double> x, const Fac<double> &f)
 {
     return mul<double>(x, f.v);
 }
-// The larger hi product of each component is added last.  The inner fma then
-// rounds at the smaller product's scale, so the lo terms reach the final
+// x is first rotated by (-i)^q, exactly (a swap and signs); then each
+// component adds its hi.x product last.  Since |hi.x| >= |hi.y|, the inner
+// fma rounds at the smaller product's scale and the lo terms reach the final
 // rounding.  With a factor near 1 (cos θ rounds to 1.0f below θ ~ 2.4e-4 and
-// the -θ²/2 sits in lo) the other order rounded x.y·hi.x + lo first and
-// dropped the correction every time, a shear that grew the norm by about
-// θ²/2 per application (1e-9 to 5e-9 for θ in [1e-5, 1e-3], float emulation).
-// Small angles are common: kinetic factors near k = 0, weak potentials.
+// the -θ²/2 sits in lo) the opposite order rounded x.y·hi.x + lo first and
+// dropped the correction every time: a shear that grew the norm by about
+// θ²/2 per application (1e-9 to 5e-9 for θ in [1e-5, 1e-3], float
+// emulation).  Small angles are common: kinetic factors near k = 0, weak
+// potentials.
 template <> __device__ __forceinline__ C<float> fac_mul<float>(C<float> x, const Fac<float> &f)
 {
-    const bool X = fabsf(f.hi.x) >= fabsf(f.hi.y);
-    const float lr = x.x * f.lo.x - x.y * f.lo.y, li = x.x * f.lo.y + x.y * f.lo.x;
-    // re = x.x hi.x - x.y hi.y + lr,  im = x.y hi.x + x.x hi.y + li; operands
-    // selected (no branch), two fmas per component
-    const float ra = X ? x.x : -x.y, rb = X ? f.hi.x : f.hi.y, rc = X ? -x.y : x.x, rd = X ? f.hi.y : f.hi.x;
-    const float ia = X ? x.y : x.x, ib = X ? f.hi.x : f.hi.y, ic = X ? x.x : x.y, id = X ? f.hi.y : f.hi.x;
-    return mk<float>(fmaf(ra, rb, fmaf(rc, rd, lr)), fmaf(ia, ib, fmaf(ic, id, li)));
+    // (-i)^q x: q = 0 (a, b), 1 (b, -a), 2 (-a, -b), 3 (-b, a)
+    const bool odd = (f.q & 1) != 0;
+    const float u = odd ? x.y : x.x, v = odd ? x.x : x.y;
+    const float xr = (f.q == 2 || f.q == 3) ? -u : u, xi = (f.q == 1 || f.q == 2) ? -v : v;
+    const float lr = xr * f.lo.x - xi * f.lo.y, li = xr * f.lo.y + xi * f.lo.x;
+    return mk<float>(fmaf(xr, f.hi.x, fmaf(-xi, f.hi.y, lr)), fmaf(xi, f.hi.x, fmaf(xr, f.hi.y, li)));
 }
 
 // ---------------------------------------------------------------------
@@ -586,22 +589,19 @@ template <> __device__ __forceinline__ Fac<float> phase_factor<float>(double arg
     const float ps = sr * sr, es = fmaf(sr, sr, -ps);
     const float t = pc - 1.0f, terr = pc - (t + 1.0f);
     const float e = ((t + ps) + terr) + (ec + es);
-    const int q = (int)j & 3;
-    float c, s;
-    if (q == 0) { c = cr; s = sr; }
-    else if (q == 1) { c = -sr; s = cr; }
-    else if (q == 2) { c = -cr; s = -sr; }
-    else { c = sr; s = -cr; }
+    // e^{-i ph} = (-i)^q e^{-i rd}: the quadrant is applied to the operand in
+    // fac_mul (exactly), the factor keeps the reduced angle (|cos| >= |sin|)
     Fac<float> f;
+    f.q = (int)j & 3;
     const float he = -0.5f * e;
     // the damping factor is computed for every point and selected (no
     // branch); with g = 1 the products below are exact and reduce to the
     // undamped pair
     const float gx = exp_f32(arg > QND_MAX_EXP_ARG_F32 ? QND_MAX_EXP_ARG_F32 : arg);
     const float g = (arg == 0.0) ? 1.0f : gx;
-    const float hx = g * c, hy = -(g * s);
+    const float hx = g * cr, hy = -(g * sr);
     f.hi = mk<float>(hx, hy);
-    f.lo = mk<float>(fmaf(g, c, -hx) + he * hx, -fmaf(g, s, hy) + he * hy);
+    f.lo = mk<float>(fmaf(g, cr, -hx) + he * hx, -fmaf(g, sr, hy) + he * hy);
     return f;
 }
 // A tabulated unit factor (cos, -sin) as a Fac: complex64 tables hold the
@@ -616,9 +616,14 @@ template <> __device__ __forceinline__ Fac<double> tab_fac<double>(const double2
 }
 template <> __device__ __forceinline__ Fac<float> tab_fac<float>(const float4 &v, bool conj)
 {
+    // (x, y) the factor, conjugated if asked; with |y| > |x| it is stored as
+    // (-i)^1 times i (x + i y) = (-y, x), whose first component is the larger
+    const float x = v.x, y = conj ? -v.y : v.y, xl = v.z, yl = conj ? -v.w : v.w;
+    const bool rot = fabsf(y) > fabsf(x);
     Fac<float> f;
-    f.hi = mk<float>(v.x, conj ? -v.y : v.y);
-    f.lo = mk<float>(v.z, conj ? -v.w : v.w);
+    f.q = rot ? 1 : 0;
+    f.hi = mk<float>(rot ? -y : x, rot ? x : y);
+    f.lo = mk<float>(rot ? -yl : xl, rot ? xl : yl);
     return f;
 }
 // Store the unit factor (c, s) into a table entry (FP32 hi + lo, or FP64).
