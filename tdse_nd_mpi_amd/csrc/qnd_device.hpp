@@ -1045,16 +1045,22 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     } else {
         int idx[QND_MAXD];
         line_index(P, ax, S, log2S, L, idx);
-        // per-axis CAP and mask values of this thread's elements, requested
-        // before the transform so that the loads overlap it (both tables
-        // always exist)
+        // per-axis CAP and mask values of this thread's elements (both tables
+        // always exist).  With four values per thread (one wavefront per
+        // SIMD, latency-bound) they are requested before the transform so
+        // that the loads overlap it; with more values per thread the passes
+        // hide latency with occupancy, and each value is read where it is
+        // used, which keeps 2 VPT doubles out of the registers of the
+        // transforms and of the other points.
         double capv[VPT], mskv[VPT];
         double Vco = 0.0, mko = 1.0;  // CAP and mask of the other axes (same order of terms)
         if (POTK) {
+            if constexpr (VPT == 4) {
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) {
-                capv[m] = P.capt[P.off[ax] + t + m * T];
-                mskv[m] = P.maskt[P.off[ax] + t + m * T];
+                for (int m = 0; m < VPT; ++m) {
+                    capv[m] = P.capt[P.off[ax] + t + m * T];
+                    mskv[m] = P.maskt[P.off[ax] + t + m * T];
+                }
             }
 #pragma unroll
             for (int j = 0; j < QND_MAXD; ++j) {
@@ -1064,6 +1070,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                 }
             }
         }
+        auto cap_at = [&](int m) { return VPT == 4 ? capv[m] : P.capt[P.off[ax] + t + m * T]; };
+        auto msk_at = [&](int m) { return VPT == 4 ? mskv[m] : P.maskt[P.off[ax] + t + m * T]; };
         // this instance's stage field (E in the length gauge, A in the
         // velocity gauge): kernel arguments, or a row of the batch tables
         // requested here so that the load overlaps the transform
@@ -1165,7 +1173,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             // this takes).  Every point sees the operators in the same order
             // (potential, adjoint terms, V1, mask, observables, V2), and the
             // partial sums take the points in order m = 0 .. VPT-1.
-            constexpr int G = 4;
+            constexpr int G = VPT == 4 ? 4 : 2;  // points per group (registers vs interleaving)
 #pragma unroll
             for (int m0 = 0; m0 < VPT; m0 += G) {
                 C<R> v[G];
@@ -1178,7 +1186,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                     rr[i] = xc[i] * xc[i] + rro;
                     V[i] = 0.0;
                     dVdu[i] = 0.0;
-                    Vc[i] = P.cap_on ? (capv[m] + Vco) : 0.0;
+                    Vc[i] = P.cap_on ? (cap_at(m) + Vco) : 0.0;
                 }
                 if (need_v) potential_pts<R, G>(P, rr, V, dVdu);
                 if (fl & (PF_GRAD | PF_SRC)) {
@@ -1246,7 +1254,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
                         const int m = m0 + i;
-                        v[i] = scl<R>(v[i], (R)(mskv[m] * mko));
+                        v[i] = scl<R>(v[i], (R)(msk_at(m) * mko));
                     }
                 }
                 if (fl & PF_OBS) {
