@@ -18,6 +18,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -54,6 +56,8 @@ struct qnd_ctx {
     int n_slots = 0;
     int device = 0;
     size_t lds_max = 65536;
+    int n_cu = 256;              // compute units of the device
+    std::map<uint64_t, int> occ; // resident workgroups per CU of a pass shape (occupancy calculator)
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // second stream (default priority): adjoint recompute overlap
     Phys P{};
@@ -228,6 +232,29 @@ int pow2floor(int64_t x)
     return p;
 }
 
+// Workgroups of a strided-line pass kernel resident per CU with `threads`
+// threads and `lds` bytes of LDS, from HIP's occupancy calculator (VGPRs,
+// LDS and wavefront slots), cached per shape.  Axis 0 of a grid of two or
+// more dimensions is the potential pass; other strided axes are plain passes.
+template <typename R> int blocks_per_cu(qnd_ctx *c, bool pot, int vpt, int threads, size_t lds)
+{
+    const uint64_t key = ((uint64_t)lds << 24) | ((uint64_t)threads << 8) | ((uint64_t)vpt << 1) | (pot ? 1u : 0u);
+    auto it = c->occ.find(key);
+    if (it != c->occ.end()) return it->second;
+    const void *f;
+    if (pot) f = vpt == 4 ? (const void *)k_pass<R, PK_POT, 1, 4>
+               : vpt == 12 ? (const void *)k_pass<R, PK_POT, 1, 12> : (const void *)k_pass<R, PK_POT, 1, 8>;
+    else f = vpt == 4 ? (const void *)k_pass<R, PK_PLAIN, 1, 4>
+             : vpt == 12 ? (const void *)k_pass<R, PK_PLAIN, 1, 12> : (const void *)k_pass<R, PK_PLAIN, 1, 8>;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, threads, lds) != hipSuccess) {
+        (void)hipGetLastError();
+        nb = 0;
+    }
+    c->occ[key] = nb;
+    return nb;
+}
+
 // Launch geometry of a pass along `axis` over nb consecutive slots (a batch
 // of instances: an extra, untransformed leading axis).
 Geo geometry(qnd_ctx *c, int axis, int nb = 1)
@@ -280,6 +307,34 @@ Geo geometry(qnd_ctx *c, int axis, int nb = 1)
     while (tw > 1 && lpi % tw != 0) tw /= 2;  // a workgroup never straddles two instances
     // one padding element per line spreads the strided tile rows over the banks
     while (tw > 1 && 2 * (size_t)tw * (g.n + 1) * c->esz > c->lds_max) tw /= 2;
+    if (g.S > 1 && ov == 0) {
+        // Strided lines: of tw, tw/2 and tw/4 take the one that keeps the most
+        // wavefronts resident on the chip (workgroups per CU from the kernel's
+        // VGPRs and LDS, capped by the tiles there are), then the one that
+        // uses more CUs, then the wider tile (longer row segments).  A kernel
+        // with, e.g., 3 wavefronts per SIMD holds one 8-wavefront workgroup
+        // per CU but three 4-wavefront ones.
+        const bool pot = (axis == 0 && c->dim > 1);
+        int64_t best_w = -1, best_cu = -1;
+        int best = tw;
+        for (int cand = tw; cand >= 1 && cand >= tw / 4; cand /= 2) {
+            const int thr = g.T * cand;
+            const size_t lds = 2 * (size_t)cand * (g.n + 1) * c->esz;
+            const int bpc = c->esz == 8 ? blocks_per_cu<float>(c, pot, g.vpt, thr, lds)
+                                        : blocks_per_cu<double>(c, pot, g.vpt, thr, lds);
+            if (bpc <= 0) continue;
+            const int64_t tiles = g.nlines / cand;
+            const int64_t resident = std::min<int64_t>(tiles, (int64_t)bpc * c->n_cu);
+            const int64_t waves = resident * ((thr + 63) / 64);
+            const int64_t cus = std::min<int64_t>(tiles, c->n_cu);
+            if (waves > best_w || (waves == best_w && cus > best_cu)) {
+                best_w = waves;
+                best_cu = cus;
+                best = cand;
+            }
+        }
+        tw = best;
+    }
     g.TW = tw;
     g.log2TW = ilog2(tw);
     g.log2T = ilog2(g.T);
@@ -1539,6 +1594,7 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
         size_t m = prop.sharedMemPerBlock;
         if (m > 163840) m = 163840;
         c->lds_max = m > 4096 ? m - 4096 : m;
+        if (prop.multiProcessorCount > 0) c->n_cu = prop.multiProcessorCount;
     }
     for (int j = 0; j < dim; ++j) {
         const int64_t tpl = (nr[j] % 3 == 0) ? nr[j] / 12 : nr[j] / 8;  // threads per line

@@ -18,6 +18,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace qnd {
 
@@ -446,9 +447,10 @@ __device__ __forceinline__ double coord_p(const Phys &P, int j, int i)
 {
     return (i < (P.n[j] >> 1)) ? i * P.dp[j] : (i - P.n[j]) * P.dp[j];
 }
-// V(r) and dV/du (u = r^2) for the named potentials (src/meshgrid.c:140-153).
-// complex128 evaluates them in FP64 like the reference; complex64 in FP32
-// (a relative change of V of a few 1e-7, below the FP32 state rounding).
+// V(r) and dV/du (u = r^2) for the named potentials (src/meshgrid.c:140-153),
+// in the precision of the state: complex128 evaluates them in FP64 like the
+// reference; complex64 in FP32 (a relative change of V of a few 1e-7, below
+// the FP32 state rounding).
 // One point of a potential family known at compile time: SOFT (soft-core,
 // PK = 0) with or without the Gaussian term (HASB), else the Gaussian well.
 template <typename R, bool SOFT, bool HASB> struct PotPoint;
@@ -470,22 +472,22 @@ template <bool SOFT, bool HASB> struct PotPoint<double, SOFT, HASB> {
     }
 };
 template <bool SOFT, bool HASB> struct PotPoint<float, SOFT, HASB> {
-    static __device__ __forceinline__ double eval(const Phys &P, double rr, double &dVdu)
+    static __device__ __forceinline__ float eval(const Phys &P, double rr, float &dVdu)
     {
         const float u = (float)rr;
         if constexpr (SOFT) {
             const float id = rsqrtf(u + (float)P.pot_c2);
             const float g = HASB ? (float)P.pot_b * __expf(-u) : 0.0f;
-            dVdu = (double)(g * id + .5f * (1.f + g) * (id * id * id));
-            return (double)(-(1.f + g) * id);
+            dVdu = g * id + .5f * (1.f + g) * (id * id * id);
+            return -(1.f + g) * id;
         } else {
             const float g = 2.7459f * __expf(u * (-.5f / 1.96f));
-            dVdu = (double)(g * (.5f / 1.96f));
-            return (double)(-g);
+            dVdu = g * (.5f / 1.96f);
+            return -g;
         }
     }
 };
-template <typename R> __device__ __forceinline__ double potential(const Phys &P, double rr, double &dVdu)
+template <typename R> __device__ __forceinline__ R potential(const Phys &P, double rr, R &dVdu)
 {
     if (P.pot_kind != 0) return PotPoint<R, false, false>::eval(P, rr, dVdu);
     if (P.pot_b != 0.0) return PotPoint<R, true, true>::eval(P, rr, dVdu);
@@ -494,8 +496,7 @@ template <typename R> __device__ __forceinline__ double potential(const Phys &P,
 // V and dV/du at the VPT points of a thread, rr[m] = r^2: the family is
 // chosen once, outside the loop, so the points form independent chains.
 template <typename R, int VPT>
-__device__ __forceinline__ void potential_pts(const Phys &P, const double (&rr)[VPT], double (&V)[VPT],
-                                              double (&dVdu)[VPT])
+__device__ __forceinline__ void potential_pts(const Phys &P, const double (&rr)[VPT], R (&V)[VPT], R (&dVdu)[VPT])
 {
     if (P.pot_kind != 0) {
 #pragma unroll
@@ -577,14 +578,15 @@ __device__ __forceinline__ float exp_f32(double arg)
 // |hi + lo| = 1 to ~1e-14.  A residual error of the angle (~1e-7 rad) acts
 // like a relative change of the Hamiltonian of that size; an error of the
 // modulus would instead change the norm at every step.
-template <> __device__ __forceinline__ Fac<float> phase_factor<float>(double arg, double ph)
+//
+// fac_octant_f32 forms that factor from the reduced angle rd, its quadrant q
+// and the damping g; the two entry points below differ in how the angle is
+// formed and reduced.
+__device__ __forceinline__ Fac<float> fac_octant_f32(float rd, int q, float g)
 {
 #pragma clang fp contract(off)  // the error-free products and sums below must not be fused
-    const double j = rint(ph * 0.63661977236758134307553505349006);          // 2/pi
-    double rd = fma(-j, 1.5707963267948965579989817342721, ph);               // pi/2 hi
-    rd = fma(-j, 6.1232339957367660359e-17, rd);                             // pi/2 lo
     float sr, cr;
-    sincos_poly_f((float)rd, sr, cr);
+    sincos_poly_f(rd, sr, cr);
     const float pc = cr * cr, ec = fmaf(cr, cr, -pc);
     const float ps = sr * sr, es = fmaf(sr, sr, -ps);
     const float t = pc - 1.0f, terr = pc - (t + 1.0f);
@@ -592,18 +594,41 @@ template <> __device__ __forceinline__ Fac<float> phase_factor<float>(double arg
     // e^{-i ph} = (-i)^q e^{-i rd}: the quadrant is applied to the operand in
     // fac_mul (exactly), the factor keeps the reduced angle (|cos| >= |sin|)
     Fac<float> f;
-    f.q = (int)j & 3;
+    f.q = q & 3;
     const float he = -0.5f * e;
-    // the damping factor is computed for every point and selected (no
-    // branch); with g = 1 the products below are exact and reduce to the
-    // undamped pair
-    const float gx = exp_f32(arg > QND_MAX_EXP_ARG_F32 ? QND_MAX_EXP_ARG_F32 : arg);
-    const float g = (arg == 0.0) ? 1.0f : gx;
+    // with g = 1 the products below are exact and reduce to the undamped pair
     const float hx = g * cr, hy = -(g * sr);
     f.hi = mk<float>(hx, hy);
     f.lo = mk<float>(fmaf(g, cr, -hx) + he * hx, -fmaf(g, sr, hy) + he * hy);
     return f;
 }
+// FP64 angle (kinetic and field phases, whose angles reach many turns)
+template <> __device__ __forceinline__ Fac<float> phase_factor<float>(double arg, double ph)
+{
+    const double j = rint(ph * 0.63661977236758134307553505349006);          // 2/pi
+    double rd = fma(-j, 1.5707963267948965579989817342721, ph);               // pi/2 hi
+    rd = fma(-j, 6.1232339957367660359e-17, rd);                             // pi/2 lo
+    // the damping factor is computed for every point and selected (no branch)
+    const float gx = exp_f32(arg > QND_MAX_EXP_ARG_F32 ? QND_MAX_EXP_ARG_F32 : arg);
+    return fac_octant_f32((float)rd, (int)j, (arg == 0.0) ? 1.0f : gx);
+}
+// FP32 angle, for the complex64 potential phases: the angle is formed from
+// the FP32 potential in FP32 (a relative error of a few 1e-8 of an angle of
+// order one, the size of the potential's own rounding) and reduced by pi/2
+// in three FP32 parts (Cody-Waite; exact products inside the FMAs).
+__device__ __forceinline__ Fac<float> phase_factor_f32(float arg, float ph)
+{
+    const float j = rintf(ph * 0.636619772f);                // 2/pi
+    float rd = fmaf(-j, 1.57079637050628662109375f, ph);      // pi/2 = hi + mid + lo
+    rd = fmaf(-j, -4.371138828673793e-8f, rd);
+    rd = fmaf(-j, -1.7151245100058819e-15f, rd);
+    const float gx = exp2f((arg > (float)QND_MAX_EXP_ARG_F32 ? (float)QND_MAX_EXP_ARG_F32 : arg) * 1.44269504f);
+    return fac_octant_f32(rd, (int)j, (arg == 0.0f) ? 1.0f : gx);
+}
+// Potential factor e^{damp} (cos ph - i sin ph) with the angle formed in the
+// precision of the state.
+__device__ __forceinline__ Fac<double> pot_factor(double damp, double ph) { return phase_factor<double>(damp, ph); }
+__device__ __forceinline__ Fac<float> pot_factor(float damp, float ph) { return phase_factor_f32(damp, ph); }
 // A tabulated unit factor (cos, -sin) as a Fac: complex64 tables hold the
 // FP64 value as an FP32 hi + lo pair (x, y, z, w) = (hi.re, hi.im, lo.re,
 // lo.im); conj gives the adjoint's conjugated factor.
@@ -1163,9 +1188,26 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                     Wv += xj * fv[j];
                 }
             }
-            double obs[QND_NOBS];
+            double obs[QND_NOBS];  // FP64 partials of this thread (adjoint terms, observables)
+            // Per-point arithmetic runs in the precision of the state: the
+            // potential, its phase angle and this thread's observable sums are
+            // FP64 for complex128 and FP32 for complex64 (a thread sums its VPT
+            // points in obr, which then enter the FP64 workgroup reduction).
+            constexpr bool F64 = std::is_same<R, double>::value;
+            R obr[QND_NOBS];
 #pragma unroll
-            for (int q = 0; q < QND_NOBS; ++q) obs[q] = 0.0;
+            for (int q = 0; q < QND_NOBS; ++q) {
+                obs[q] = 0.0;
+                obr[q] = (R)0;
+            }
+            auto add_obs = [&](int q, R val) {
+                if constexpr (F64) obs[q] += val;
+                else obr[q] += val;
+            };
+            const R cv1 = (R)a.cv1, cv2 = (R)a.cv2, csr = (R)cs, fva = (R)fv[ax], Wvr = (R)Wv;
+            R xor_[QND_MAXD];
+#pragma unroll
+            for (int j = 0; j < QND_MAXD; ++j) xor_[j] = (R)xo[j];
             const bool need_v = (fl & (PF_V1 | PF_V2 | PF_OBS | PF_SRC)) != 0;
             // Groups of four points: each operator is one unrolled loop over the
             // group with its mode tests outside, so the points are independent
@@ -1177,16 +1219,18 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
 #pragma unroll
             for (int m0 = 0; m0 < VPT; m0 += G) {
                 C<R> v[G];
-                double xc[G], rr[G], V[G], dVdu[G], Vc[G];
+                double xc[G], rr[G];
+                R xr[G], V[G], dVdu[G], Vc[G];
 #pragma unroll
                 for (int i = 0; i < G; ++i) {
                     const int m = m0 + i;
                     v[i] = scl<R>(x[m], (R)a.scale1);
                     xc[i] = coord_x(P, ax, t + m * T);
+                    xr[i] = (R)xc[i];
                     rr[i] = xc[i] * xc[i] + rro;
-                    V[i] = 0.0;
-                    dVdu[i] = 0.0;
-                    Vc[i] = P.cap_on ? (cap_at(m) + Vco) : 0.0;
+                    V[i] = (R)0;
+                    dVdu[i] = (R)0;
+                    Vc[i] = P.cap_on ? (R)(cap_at(m) + Vco) : (R)0;
                 }
                 if (need_v) potential_pts<R, G>(P, rr, V, dVdu);
                 if (fl & (PF_GRAD | PF_SRC)) {
@@ -1230,23 +1274,20 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                         if (P.cap_on) {
 #pragma unroll
                             for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv1 * Vc[i],
-                                                                        cs * a.cv1 * (V[i] + (xc[i] * fv[ax] + Wv))));
+                                v[i] = fac_mul<R>(v[i], pot_factor(-cv1 * Vc[i], csr * cv1 * (V[i] + (xr[i] * fva + Wvr))));
                         } else {
 #pragma unroll
                             for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i],
-                                                  phase_factor<R>(0.0, cs * a.cv1 * (V[i] + (xc[i] * fv[ax] + Wv))));
+                                v[i] = fac_mul<R>(v[i], pot_factor((R)0, csr * cv1 * (V[i] + (xr[i] * fva + Wvr))));
                         }
                     } else {
                         if (P.cap_on) {
 #pragma unroll
                             for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv1 * Vc[i], cs * a.cv1 * V[i]));
+                                v[i] = fac_mul<R>(v[i], pot_factor(-cv1 * Vc[i], csr * cv1 * V[i]));
                         } else {
 #pragma unroll
-                            for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i], phase_factor<R>(0.0, cs * a.cv1 * V[i]));
+                            for (int i = 0; i < G; ++i) v[i] = fac_mul<R>(v[i], pot_factor((R)0, csr * cv1 * V[i]));
                         }
                     }
                 }
@@ -1260,14 +1301,14 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                 if (fl & PF_OBS) {
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
-                        const double d = (double)v[i].x * (double)v[i].x + (double)v[i].y * (double)v[i].y;
-                        obs[0] += d;
+                        const R d = v[i].x * v[i].x + v[i].y * v[i].y;
+                        add_obs(0, d);
 #pragma unroll
                         for (int j = 0; j < QND_MAXD; ++j) {
                             if (j < P.dim) {
-                                const double xj = (j == ax) ? xc[i] : xo[j];
-                                obs[1 + j] += d * xj;
-                                obs[1 + QND_MAXD + j] += d * dVdu[i] * 2.0 * xj;
+                                const R xj = (j == ax) ? xr[i] : xor_[j];
+                                add_obs(1 + j, d * xj);
+                                add_obs(1 + QND_MAXD + j, d * dVdu[i] * (R)2 * xj);
                             }
                         }
                     }
@@ -1280,11 +1321,10 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                         }
                     } else if (P.cap_on) {
 #pragma unroll
-                        for (int i = 0; i < G; ++i)
-                            v[i] = fac_mul<R>(v[i], phase_factor<R>(-a.cv2 * Vc[i], cs * a.cv2 * V[i]));
+                        for (int i = 0; i < G; ++i) v[i] = fac_mul<R>(v[i], pot_factor(-cv2 * Vc[i], csr * cv2 * V[i]));
                     } else {
 #pragma unroll
-                        for (int i = 0; i < G; ++i) v[i] = fac_mul<R>(v[i], phase_factor<R>(0.0, cs * a.cv2 * V[i]));
+                        for (int i = 0; i < G; ++i) v[i] = fac_mul<R>(v[i], pot_factor((R)0, csr * cv2 * V[i]));
                     }
                 }
                 if (fl & PF_SAVE) {
@@ -1299,6 +1339,10 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                     const int m = m0 + i;
                     x[m] = v[i];
                 }
+            }
+            if (!F64 && (fl & PF_OBS)) {
+#pragma unroll
+                for (int q = 0; q < QND_NOBS; ++q) obs[q] += (double)obr[q];
             }
             if (fl & (PF_OBS | PF_GRAD | PF_SRC))
                 block_sum_vec<QND_NOBS>(obs, red, a.partials + (int64_t)tb * a.npart);
