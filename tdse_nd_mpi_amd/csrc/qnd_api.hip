@@ -236,16 +236,20 @@ int pow2floor(int64_t x)
 // threads and `lds` bytes of LDS, from HIP's occupancy calculator (VGPRs,
 // LDS and wavefront slots), cached per shape.  Axis 0 of a grid of two or
 // more dimensions is the potential pass; other strided axes are plain passes.
-template <typename R> int blocks_per_cu(qnd_ctx *c, bool pot, int vpt, int threads, size_t lds)
+template <typename R, int KIND, int DIR> const void *pass_fn(int vpt)
 {
-    const uint64_t key = ((uint64_t)lds << 24) | ((uint64_t)threads << 8) | ((uint64_t)vpt << 1) | (pot ? 1u : 0u);
+    return vpt == 4 ? (const void *)k_pass<R, KIND, DIR, 4>
+           : vpt == 12 ? (const void *)k_pass<R, KIND, DIR, 12> : (const void *)k_pass<R, KIND, DIR, 8>;
+}
+
+// Workgroups per CU of a pass kernel of kind `kind` (PK_PLAIN, PK_KIN or PK_POT).
+template <typename R> int blocks_per_cu(qnd_ctx *c, int kind, int vpt, int threads, size_t lds)
+{
+    const uint64_t key = ((uint64_t)lds << 24) | ((uint64_t)threads << 8) | ((uint64_t)vpt << 2) | (uint64_t)kind;
     auto it = c->occ.find(key);
     if (it != c->occ.end()) return it->second;
-    const void *f;
-    if (pot) f = vpt == 4 ? (const void *)k_pass<R, PK_POT, 1, 4>
-               : vpt == 12 ? (const void *)k_pass<R, PK_POT, 1, 12> : (const void *)k_pass<R, PK_POT, 1, 8>;
-    else f = vpt == 4 ? (const void *)k_pass<R, PK_PLAIN, 1, 4>
-             : vpt == 12 ? (const void *)k_pass<R, PK_PLAIN, 1, 12> : (const void *)k_pass<R, PK_PLAIN, 1, 8>;
+    const void *f = kind == PK_POT ? pass_fn<R, PK_POT, 1>(vpt)
+                    : kind == PK_KIN ? pass_fn<R, PK_KIN, -1>(vpt) : pass_fn<R, PK_PLAIN, 1>(vpt);
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, threads, lds) != hipSuccess) {
         (void)hipGetLastError();
@@ -256,8 +260,10 @@ template <typename R> int blocks_per_cu(qnd_ctx *c, bool pot, int vpt, int threa
 }
 
 // Launch geometry of a pass along `axis` over nb consecutive slots (a batch
-// of instances: an extra, untransformed leading axis).
-Geo geometry(qnd_ctx *c, int axis, int nb = 1)
+// of instances: an extra, untransformed leading axis).  `kind` is the pass
+// kernel whose occupancy sizes strided tiles (-1: the potential pass on axis
+// 0 of a grid of dim >= 2, else the plain pass).
+Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1)
 {
     Geo g{};
     g.axis = axis;
@@ -314,14 +320,14 @@ Geo geometry(qnd_ctx *c, int axis, int nb = 1)
         // uses more CUs, then the wider tile (longer row segments).  A kernel
         // with, e.g., 3 wavefronts per SIMD holds one 8-wavefront workgroup
         // per CU but three 4-wavefront ones.
-        const bool pot = (axis == 0 && c->dim > 1);
+        if (kind < 0) kind = (axis == 0 && c->dim > 1) ? PK_POT : PK_PLAIN;
         int64_t best_w = -1, best_cu = -1;
         int best = tw;
         for (int cand = tw; cand >= 1 && cand >= tw / 4; cand /= 2) {
             const int thr = g.T * cand;
             const size_t lds = 2 * (size_t)cand * (g.n + 1) * c->esz;
-            const int bpc = c->esz == 8 ? blocks_per_cu<float>(c, pot, g.vpt, thr, lds)
-                                        : blocks_per_cu<double>(c, pot, g.vpt, thr, lds);
+            const int bpc = c->esz == 8 ? blocks_per_cu<float>(c, kind, g.vpt, thr, lds)
+                                        : blocks_per_cu<double>(c, kind, g.vpt, thr, lds);
             if (bpc <= 0) continue;
             const int64_t tiles = g.nlines / cand;
             const int64_t resident = std::min<int64_t>(tiles, (int64_t)bpc * c->n_cu);
@@ -360,6 +366,7 @@ template <typename R> PassArgs<R> base_args(qnd_ctx *c, C<R> *data, const Geo &g
     a.log2TW = g.log2TW;
     a.ldsStride = g.ldsStride;
     a.wave_lines = c->wave_lines;
+    a.kin_axes = (1u << c->dim) - 1u;
     a.S = g.S;
     a.nlines = g.nlines;
     a.scale1 = 1.0;
@@ -447,7 +454,9 @@ const typename TwT<R>::T *ktab_for(qnd_ctx *c, double ck, const Geo &g, int *rc)
 {
     using T = typename TwT<R>::T;
     *rc = 0;
-    if (g.vpt != 4 || c->dim < 2 || !c->have_int) return nullptr;
+    // the table holds the all-axes factor: 2-D only, where the kinetic block
+    // is one pass
+    if (g.vpt != 4 || c->dim != 2 || !c->have_int) return nullptr;
     if (c->ktab_gen != c->setup_gen || c->ktab_prec != c->prec) {
         c->ktab_gen = c->setup_gen;
         c->ktab_prec = c->prec;
@@ -482,10 +491,8 @@ const typename TwT<R>::T *ktab_for(qnd_ctx *c, double ck, const Geo &g, int *rc)
     return nullptr;
 }
 
-// kinetic block for dim >= 2: FFT axes 1..d-2, kinetic pass on axis d-1,
-// inverse on axes d-2..1.
 // A pending observable record (partials of the previous axis-0 pass) is
-// finalized by workgroups appended to the kinetic pass when fin is given.
+// finalized by workgroups appended to the first kinetic pass when fin is given.
 struct Finalize {
     const double *part;
     int nwg;         // workgroups per instance of the pass that wrote the partials
@@ -522,50 +529,51 @@ struct FieldRows {
     int64_t stride = 0;
 };
 
+// Kinetic block for dim >= 2, on the axis-0 Fourier representation M.  The
+// kinetic factor exp(-i c sum_j K_j) (K_j = p_j^2 / 2, plus (p_j + A_j/2) A_j
+// in the velocity gauge; exp(-c sum_j K_j) in imaginary time) is a product of
+// commuting per-axis factors, so the block is one pass per axis 1 .. d-1:
+// forward transform along the axis, the factor of its axes, inverse.  The
+// pass along axis 1 also applies axis 0's factor (axis 0 is in k-space in M).
+// In 2-D this is the single all-k pass; in 3-D two passes replace the forward
+// transform, the all-k pass and the inverse transform (three passes).  Each
+// factor's exponent is clamped separately (a clamp that would act signals a
+// blown-up state in either form).
 template <typename R>
 int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A, const Finalize *fin = nullptr,
-                  int nb = 1, FieldRows Arows = FieldRows())
+                  int nb = 1, FieldRows Arows = FieldRows(), unsigned flags = 0)
 {
     const int d = c->dim;
     int rc;
-    if (fin && d > 2) {  // a plain pass comes first: finalize separately
-        if ((rc = launch_finalize(c, fin))) return rc;
-        fin = nullptr;
-    }
-    for (int ax = 1; ax <= d - 2; ++ax) {
-        rc = plain_pass<R>(c, data, ax, -1, 1.0, nb);
-        if (rc) return rc;
-    }
-    Geo g = geometry(c, d - 1, nb);
-    PassArgs<R> a = base_args<R>(c, data, g);
-    a.kmode = kmode;
-    a.ck_mid = ck;
-    a.scale2 = 1.0 / (double)c->nr[d - 1];
-    if (kmode == KM_PHASE) {
-        a.ktab = ktab_for<R>(c, ck, g, &rc);
-        if (rc) return rc;
-        if (a.ktab) a.flags |= PF_KTAB;
-    }
-    if (A)
-        for (int j = 0; j < d; ++j) a.Ak[j] = A[j];
-    a.B.nb = nb;
-    a.B.fld = Arows.base;
-    a.B.fld_stride = Arows.stride;
-    if (fin) {
-        a.flags |= PF_FINALIZE;
-        a.fin_part = fin->part;
-        a.fin_nwg = fin->nwg;
-        a.fin_rec = fin->rec;
-        a.fin_spec = fin->spec;
-        a.B.finE = fin->B.finE;
-        a.B.finE_stride = fin->B.finE_stride;
-        a.B.rec_stride = fin->B.rec_stride;
-    }
-    rc = launch<R, PK_KIN, -1>(c, g, a, PC_KIN);
-    if (rc) return rc;
-    for (int ax = d - 2; ax >= 1; --ax) {
-        rc = plain_pass<R>(c, data, ax, +1, 1.0 / (double)c->nr[ax], nb);
-        if (rc) return rc;
+    for (int ax = 1; ax < d; ++ax) {
+        Geo g = geometry(c, ax, nb, PK_KIN);
+        PassArgs<R> a = base_args<R>(c, data, g);
+        a.flags = flags;
+        a.kmode = kmode;
+        a.kin_axes = ax == 1 ? 3u : (1u << ax);
+        a.ck_mid = ck;
+        a.scale2 = 1.0 / (double)c->nr[ax];
+        if (kmode == KM_PHASE) {
+            a.ktab = ktab_for<R>(c, ck, g, &rc);
+            if (rc) return rc;
+            if (a.ktab) a.flags |= PF_KTAB;
+        }
+        if (A)
+            for (int j = 0; j < d; ++j) a.Ak[j] = A[j];
+        a.B.nb = nb;
+        a.B.fld = Arows.base;
+        a.B.fld_stride = Arows.stride;
+        if (fin && ax == 1) {
+            a.flags |= PF_FINALIZE;
+            a.fin_part = fin->part;
+            a.fin_nwg = fin->nwg;
+            a.fin_rec = fin->rec;
+            a.fin_spec = fin->spec;
+            a.B.finE = fin->B.finE;
+            a.B.finE_stride = fin->B.finE_stride;
+            a.B.rec_stride = fin->B.rec_stride;
+        }
+        if ((rc = launch<R, PK_KIN, -1>(c, g, a, PC_KIN))) return rc;
     }
     return 0;
 }
@@ -1232,22 +1240,7 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
         return a;
     };
     auto kin_adj = [&](C<R> *data, double ck) -> int {
-        Geo g = geometry(c, d - 1);
-        for (int ax = 1; ax <= d - 2; ++ax)
-            if (int r = plain_pass<R>(c, data, ax, -1, 1.0)) return r;
-        PassArgs<R> a = base_args<R>(c, data, g);
-        a.kmode = KM_PHASE;
-        a.ck_mid = ck;
-        a.scale2 = 1.0 / (double)c->nr[d - 1];
-        a.flags = PF_CONJ;
-        int r = 0;
-        a.ktab = ktab_for<R>(c, ck, g, &r);
-        if (r) return r;
-        if (a.ktab) a.flags |= PF_KTAB;
-        if ((r = launch<R, PK_KIN, -1>(c, g, a, PC_KIN))) return r;
-        for (int ax = d - 2; ax >= 1; --ax)
-            if (int r = plain_pass<R>(c, data, ax, +1, 1.0 / (double)c->nr[ax])) return r;
-        return 0;
+        return kinetic_block<R>(c, data, KM_PHASE, ck, nullptr, nullptr, 1, FieldRows(), PF_CONJ);
     };
     // recompute segment sg from its checkpoint on the second stream, saving y_n
     // and phi_{n,i}; it may start once the adjoint that last read its buffer is done

@@ -786,6 +786,7 @@ template <typename R> struct PassArgs {
     unsigned flags;
     double scale1, scale2;   // after the first / second transform
     int kmode;
+    unsigned kin_axes;       // kinetic pass: the axes whose kinetic terms it applies (always its own)
     double ck_pre, ck_mid, ck_post;
     double Ak[QND_MAXD];
     int vmode;
@@ -1138,11 +1139,14 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
 
         if (KIND == PK_KIN) {
-            // all axes in k-space: kinetic factor (reference order of the sums)
+            // all axes in k-space: the kinetic factor of this pass's axes
+            // (reference order of the sums).  exp(-i c sum_j K_j) is a product
+            // of commuting per-axis factors, so in 3-D and above each pass of
+            // the kinetic block applies its own axes' part.
             double Ko = 0.0, Wk = 0.0;
 #pragma unroll
             for (int j = 0; j < QND_MAXD; ++j) {
-                if (j < P.dim && j != ax) {
+                if (j < P.dim && j != ax && ((a.kin_axes >> j) & 1u)) {
                     const double pj = coord_p(P, j, idx[j]);
                     Ko += .5 * pj * pj;
                     Wk += (pj + .5 * fv[j]) * fv[j];

@@ -167,6 +167,37 @@ def test_real_steps_match_oracle(gpu, nr, dr, pot, integ, gauge, mask, cap):
     g.close()
 
 
+@pytest.mark.parametrize("prec,gauge,tun", [("c128", "length", (8, 0, 0)), ("c128", "velocity", (8, 0, 4)),
+                                            ("c64", "length", (8, 0, 0)), ("c128", "length", (4, 0, 16))])
+def test_real_steps_3d_geometries(gpu, prec, gauge, tun):
+    """3-D steps with eight values per thread and wide strided tiles (the
+    launch shapes of large grids): the kinetic block is one pass per axis,
+    the pass along axis 1 strided, with the observable record finalized by
+    workgroups appended to it."""
+    nr, dr, pot = (16, 32, 16), 0.5, "H_3D"
+    print(f"case real-3d prec={prec} gauge={gauge} tune={tun}", flush=True)
+    dt, nt = 0.05, 10
+    o = O.Oracle(nr, dr, pot, "BM4", dt, cap=(0.05, 1.0))
+    o.set_mask(0.1)
+    g = q.Grid(nr, dr, prec, n_slots=1)
+    g.set_potential(pot, cap=(0.05, 1.0))
+    g.set_integrator("BM4", dt)
+    g.set_mask(0.1)
+    g.set_tuning(*tun)
+    E, A, Eobs, Aobs = field_tables(o, nt, 3)
+    g.set_field(E, A, Eobs, Aobs)
+    psi0 = smooth_state(o, 4)[0]
+    names = ["dipole_r", "norm", "dipole_a"]
+    ref_psi, ref_obs = o.real_run(psi0, gauge, E, A, Eobs, Aobs, names)
+    g.upload(0, psi0)
+    got_obs = g.real_steps(0, gauge, 0, nt, names)
+    got = g.download(0)
+    tol = TOL128 if prec == "c128" else TOL64
+    assert rel(got, ref_psi) < tol, rel(got, ref_psi)
+    assert np.allclose(got_obs, ref_obs, rtol=tol * 10, atol=tol), np.abs(got_obs - ref_obs).max()
+    g.close()
+
+
 @pytest.mark.parametrize("gauge", ["length", "velocity"])
 def test_real_unfused_observables(gpu, gauge):
     nr, dr, pot = (32, 32), 0.4, "H_2D"
@@ -364,16 +395,16 @@ def test_tuning_geometries_agree(gpu, prec, nr):
     g.close()
 
 
-def _adjoint_problem(prec, nr=(64, 64), nsteps=40, every=8):
+def _adjoint_problem(prec, nr=(64, 64), nsteps=40, every=8, dr=0.4, pot="H_2D"):
     from tdse_nd_mpi_amd import adjoint as adj
     rng = np.random.default_rng(7)
-    o = O.Oracle(nr, 0.4, "H_2D", "BM4", 0.05)
+    o = O.Oracle(nr, dr, pot, "BM4", 0.05)
     psi0 = smooth_state(o, 4)[0]
-    psi0 = psi0 / np.sqrt(np.sum(np.abs(psi0) ** 2) * 0.16)
+    psi0 = psi0 / np.sqrt(np.sum(np.abs(psi0) ** 2) * dr ** len(nr))
     t = np.arange(nsteps) * 0.05
     E = 0.05 * np.sin(0.5 * t) * np.exp(-((t - 1.0) / 1.2) ** 2) + 0.01 * rng.standard_normal(nsteps)
-    spec = adj.Spec(nr, 0.4, "H_2D", "BM4", 0.05, psi0, precision=prec, mask=0.1, every=every)
-    wa = rng.standard_normal((nsteps, 2))
+    spec = adj.Spec(nr, dr, pot, "BM4", 0.05, psi0, precision=prec, mask=0.1, every=every)
+    wa = rng.standard_normal((nsteps, len(nr)))
     return adj, spec, E, wa
 
 
@@ -400,6 +431,27 @@ def test_adjoint_matches_finite_differences(gpu):
         assert abs(fd - dE[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, dE[k])
     hs = 1e-4
     fd = (_linear_loss(adj, spec, E, soft + hs, wa) - _linear_loss(adj, spec, E, soft - hs, wa)) / (2 * hs)
+    assert abs(fd - ds) <= 1e-6 * max(1.0, abs(fd)), (fd, ds)
+    spec.close()
+
+
+def test_adjoint_3d_matches_finite_differences(gpu):
+    """The 3-D adjoint (kinetic block of one conjugated pass per axis) against
+    central differences of the forward run, complex128."""
+    print("case adjoint fd 3-D", flush=True)
+    adj, spec, E, wa = _adjoint_problem("c128", nr=(16, 16, 16), nsteps=16, every=4, dr=0.5, pot="H_3D")
+    soft = 1.0
+    adj.forward_numpy(spec, E, soft)
+    dEt, dEo, ds = spec.grid.adjoint(wa, 0.7)
+    dE = dEt[:, :, 0].sum(axis=1) + dEo[:, 0]
+    h = 1e-4
+    for k in [0, 5, 15]:
+        Ep, Em = E.copy(), E.copy()
+        Ep[k] += h
+        Em[k] -= h
+        fd = (_linear_loss(adj, spec, Ep, soft, wa) - _linear_loss(adj, spec, Em, soft, wa)) / (2 * h)
+        assert abs(fd - dE[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, dE[k])
+    fd = (_linear_loss(adj, spec, E, soft + h, wa) - _linear_loss(adj, spec, E, soft - h, wa)) / (2 * h)
     assert abs(fd - ds) <= 1e-6 * max(1.0, abs(fd)), (fd, ds)
     spec.close()
 
