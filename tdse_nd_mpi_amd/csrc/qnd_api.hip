@@ -101,6 +101,14 @@ struct qnd_ctx {
     std::vector<double> ktab_coef;
     int64_t ktab_gen = -1;
     int ktab_prec = -1;
+    // per-axis kinetic factors of the step's first coefficient (merged step
+    // boundary), E(t_n) on the device, and the ring of saved record states
+    void *d_atab = nullptr;
+    int64_t atab_gen = -1;
+    int atab_prec = -1;
+    double *d_Eo = nullptr;
+    void *d_side = nullptr;
+    size_t side_cap = 0;  // bytes
     int bnb = 0;
     int64_t bntime = 0;
     hipEvent_t ev[16] = {};
@@ -449,6 +457,8 @@ template <typename R> int full_fft(qnd_ctx *c, C<R> *data, int dir)
 // of forming it.  Tables above QND_KTAB_MAX_BYTES are not built (large
 // grids are not latency-bound, and the extra read would cost bandwidth).
 #define QND_KTAB_MAX_BYTES ((size_t)256 << 20)
+// ring of saved record states of the merged step boundary
+#define QND_SIDE_MAX_BYTES ((size_t)256 << 20)
 template <typename R>
 const typename TwT<R>::T *ktab_for(qnd_ctx *c, double ck, const Geo &g, int *rc)
 {
@@ -489,6 +499,43 @@ const typename TwT<R>::T *ktab_for(qnd_ctx *c, double ck, const Geo &g, int *rc)
     for (size_t q = 0; q < c->ktab_coef.size(); ++q)
         if (c->ktab_coef[q] == ck) return (const T *)c->d_ktab + q * c->N;
     return nullptr;
+}
+
+// Per-axis kinetic factors e^{-i a_0 p_j(i)^2 / 2} (a_0 the step's first
+// kinetic coefficient, scaled by dt), Sigma n_j entries at the per-axis table
+// offsets, formed in FP64 on the host (complex64: FP32 hi + lo).  Rebuilt
+// when the set-up or the precision changes.
+template <typename R> const typename TwT<R>::T *atab_for(qnd_ctx *c, int *rc)
+{
+    using T = typename TwT<R>::T;
+    *rc = 0;
+    if (c->atab_gen == c->setup_gen && c->atab_prec == c->prec && c->d_atab) return (const T *)c->d_atab;
+    const int tot = c->P.off[c->dim - 1] + c->P.n[c->dim - 1];
+    std::vector<T> h(tot);
+    const double a0 = c->as_scaled[0];
+    for (int j = 0; j < c->dim; ++j)
+        for (int i = 0; i < c->P.n[j]; ++i) {
+            const double p = (i < (c->P.n[j] >> 1)) ? i * c->P.dp[j] : (i - c->P.n[j]) * c->P.dp[j];
+            const double ph = a0 * (.5 * p * p);
+            const double cn = cos(ph), sn = -sin(ph);
+            T &e = h[c->P.off[j] + i];
+            if constexpr (std::is_same<T, double2>::value) {
+                e.x = cn;
+                e.y = sn;
+            } else {
+                e.x = (float)cn;
+                e.y = (float)sn;
+                e.z = (float)(cn - (double)e.x);
+                e.w = (float)(sn - (double)e.y);
+            }
+        }
+    if (c->d_atab) HIPCK_P(c, hipFree(c->d_atab), rc);
+    c->d_atab = nullptr;
+    HIPCK_P(c, hipMalloc(&c->d_atab, tot * sizeof(T)), rc);
+    HIPCK_P(c, hipMemcpy(c->d_atab, h.data(), tot * sizeof(T), hipMemcpyHostToDevice), rc);
+    c->atab_gen = c->setup_gen;
+    c->atab_prec = c->prec;
+    return (const T *)c->d_atab;
 }
 
 // A pending observable record (partials of the previous axis-0 pass) is
@@ -958,6 +1005,73 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
     memset(&spec, 0, sizeof spec);
     spec.ncodes = ncodes;
     for (int k = 0; k < ncodes; ++k) spec.codes[k] = codes[k];
+    // Merged step boundary (2-D, one instance, length gauge): between two
+    // steps of this call, the last kinetic pass of step n saves the state
+    // (the record's state, before the owed mask) to a ring of side buffers,
+    // applies the axis-1 parts of the mask and of step n+1's first kinetic
+    // factor, and the first axis-0 pass of step n+1 applies their axis-0
+    // parts before its potential: the operators of the boundary pass and of
+    // the first kinetic block, regrouped by axis (parts on different axes
+    // commute).  The records of the saved states are computed by one batched
+    // observable pass per ring fill, off the per-step chain.  Two passes per
+    // step fewer.
+    const int side_slots_max = 64;
+    const size_t sbytes = (size_t)c->N * sizeof(C<R>);
+    int side_slots = 0;
+    bool merge = gauge == QND_GAUGE_LENGTH && d == 2 && !batched && (!want_obs || fused) && nsteps > 1;
+    if (merge && want_obs) {
+        side_slots = (int)std::min<int64_t>(side_slots_max, std::min<int64_t>(nsteps - 1,
+                                            (int64_t)(QND_SIDE_MAX_BYTES / sbytes)));
+        if (side_slots < 1) merge = false;
+    }
+    const typename TwT<R>::T *atab = nullptr;
+    Geo gside{};
+    if (merge) {
+        atab = atab_for<R>(c, &rc);
+        if (rc) return rc;
+        if (side_slots > 0) {
+            if (c->side_cap < (size_t)side_slots * sbytes) {
+                if (c->d_side) HIPCK(c, hipFree(c->d_side));
+                c->d_side = nullptr;
+                c->side_cap = 0;
+                HIPCK(c, hipMalloc(&c->d_side, (size_t)side_slots * sbytes));
+                c->side_cap = (size_t)side_slots * sbytes;
+            }
+            gside = geometry(c, 0, side_slots);
+            if ((rc = ensure_part(c, (size_t)std::max(g0.grid, gside.grid) * QND_NOBS))) return rc;
+        }
+    }
+    C<R> *side = reinterpret_cast<C<R> *>(c->d_side);
+    int side_n = 0;           // saved record states in the ring
+    int64_t side_first = 0;   // step of the first of them
+    // records of the saved states: one observable-only axis-0 pass over the
+    // ring (a batch of side_n instances), then one finalize block per record
+    auto flush_side = [&]() -> int {
+        if (side_n == 0) return 0;
+        const Geo gs = side_n == side_slots ? gside : geometry(c, 0, side_n);
+        PassArgs<R> a = base_args<R>(c, side, gs);
+        a.scale1 = inv_n0;
+        a.flags = PF_OBS | PF_MASK | PF_NOSTORE;
+        a.partials = c->d_part;
+        a.npart = QND_NOBS;
+        a.B.nb = side_n;
+        int r;
+        if ((r = launch<R, PK_POT, +1>(c, gs, a, PC_SMALL))) return r;
+        Finalize f;
+        memset(&f, 0, sizeof f);
+        f.part = c->d_part;
+        f.nwg = gs.grid / side_n;
+        f.rec = c->d_rec + (side_first - step0) * width;
+        f.spec = spec;
+        f.B.nb = side_n;
+        f.B.rec_stride = width;
+        f.B.finE = c->d_Eo + side_first * d;
+        f.B.finE_stride = d;
+        if ((r = launch_finalize(c, &f))) return r;
+        side_n = 0;
+        return 0;
+    };
+    bool merged_in = false;   // this step's boundary was done by the previous step's passes
     // per-instance field rows: [nb][ntime][s][d] (stage) and [nb][ntime][d] (t_n)
     const int64_t stage_stride = ntime * s * d, obs_stride = ntime * d;
     auto stage_rows = [&](const double *tab, int64_t it, int ia) {
@@ -989,7 +1103,7 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
         Finalize fin;
         memset(&fin, 0, sizeof fin);
         const Finalize *finp = nullptr;
-        if (want_obs && fused) {
+        if (want_obs && fused && !merged_in) {
             if (!batched)
                 for (int j = 0; j < d; ++j) spec.E[j] = Eo[j];
             fin.part = c->d_part;
@@ -1021,23 +1135,26 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
             a.B.fld_stride = fr.stride;
         };
         if (gauge == QND_GAUGE_LENGTH) {
-            // boundary pass: owed mask, observables
-            PassArgs<R> a = pot_args();
-            a.flags = PF_SECOND;
-            if (st.pend_mask) a.flags |= PF_MASK;
-            if (finp) {
-                a.flags |= PF_OBS;
-                a.partials = c->d_part;
-                a.npart = QND_NOBS;
+            if (!merged_in) {
+                // boundary pass: owed mask, observables
+                PassArgs<R> a = pot_args();
+                a.flags = PF_SECOND;
+                if (st.pend_mask) a.flags |= PF_MASK;
+                if (finp) {
+                    a.flags |= PF_OBS;
+                    a.partials = c->d_part;
+                    a.npart = QND_NOBS;
+                }
+                if (a.flags != PF_SECOND) {
+                    if ((rc = launch_pot<R>(c, g0, a))) return rc;
+                }
+                st.pend_mask = false;
+                if (d == 1 && (rc = finalize_now())) return rc;
             }
-            if (a.flags != PF_SECOND) {
-                if ((rc = launch_pot<R>(c, g0, a))) return rc;
-            }
-            st.pend_mask = false;
-            if (d == 1 && (rc = finalize_now())) return rc;
+            bool merged_out = false;
             for (int ia = 0; ia < s; ++ia) {
                 const double a0 = c->as_scaled[2 * ia], a1 = c->as_scaled[2 * ia + 1];
-                if (d > 1 && ia == 0) {
+                if (d > 1 && ia == 0 && !merged_in) {
                     if ((rc = kinetic_block<R>(c, psi, KM_PHASE, a0, nullptr, finp, nb))) return rc;
                     finp = nullptr;
                 }
@@ -1054,13 +1171,43 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
                     b.ck_pre = a0;
                     b.ck_post = a1;
                 }
+                if (ia == 0 && merged_in) {
+                    // the owed axis-0 mask and axis-0 part of K(a_0)
+                    b.flags |= PF_PREKX | (c->P.mask_on ? PF_MASK : 0u);
+                    b.atab = atab;
+                }
                 if ((rc = launch_pot<R>(c, g0, b))) return rc;
                 if (d > 1) {
                     double ck = a1 + (ia < s - 1 ? c->as_scaled[2 * ia + 2] : 0.0);
-                    if ((rc = kinetic_block<R>(c, psi, KM_PHASE, ck, nullptr, nullptr, nb))) return rc;
+                    if (ia == s - 1 && merge && it + 1 < step0 + nsteps) {
+                        // merged boundary: K(a_last), save the record's state,
+                        // axis-1 parts of the mask and of K(a_0)
+                        const Geo gk = geometry(c, 1, 1, PK_KIN);
+                        PassArgs<R> k = base_args<R>(c, psi, gk);
+                        k.flags = PF_KSPLIT | (c->P.mask_on ? PF_MASK : 0u);
+                        k.kmode = KM_PHASE;
+                        k.ck_mid = ck;
+                        k.scale2 = 1.0 / (double)c->nr[1];
+                        k.ktab = ktab_for<R>(c, ck, gk, &rc);
+                        if (rc) return rc;
+                        if (k.ktab) k.flags |= PF_KTAB;
+                        k.atab = atab;
+                        if (want_obs) {
+                            if (side_n == 0) side_first = it + 1;
+                            k.flags |= PF_SAVE;
+                            k.save = side + (size_t)side_n * c->N;
+                            ++side_n;
+                        }
+                        if ((rc = launch<R, PK_KIN, -1>(c, gk, k, PC_KIN))) return rc;
+                        merged_out = true;
+                    } else if ((rc = kinetic_block<R>(c, psi, KM_PHASE, ck, nullptr, nullptr, nb))) {
+                        return rc;
+                    }
                 }
             }
             st.pend_mask = c->P.mask_on != 0;
+            merged_in = merged_out;
+            if (side_n == side_slots && (rc = flush_side())) return rc;
         } else {
             // velocity gauge: boundary pass carries the owed last half-step,
             // the owed mask, the observables and the first half-step
@@ -1109,6 +1256,7 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
             st.pend_mask = c->P.mask_on != 0;
         }
     }
+    if ((rc = flush_side())) return rc;
     for (int b = 1; b < nb; ++b) c->slots[slot0 + b] = st;
     if (want_obs && fused) {
         // device records [nb][nsteps][width] -> obs_out [nb][nsteps][width]
@@ -1694,6 +1842,9 @@ void qnd_destroy(qnd_ctx *c)
     if (c->d_bAt) hipFree(c->d_bAt);
     if (c->d_bEo) hipFree(c->d_bEo);
     if (c->d_ktab) hipFree(c->d_ktab);
+    if (c->d_atab) hipFree(c->d_atab);
+    if (c->d_Eo) hipFree(c->d_Eo);
+    if (c->d_side) hipFree(c->d_side);
     if (c->d_kcoef) hipFree(c->d_kcoef);
     for (void *p : {c->d_adj_ckpt, c->d_adj_final, c->d_adj_save, (void *)c->d_adj_part})
         if (p) hipFree(p);
@@ -1873,6 +2024,12 @@ int qnd_set_field(qnd_ctx *c, int64_t ntime, const double *Et, const double *At,
     if (At) memcpy(c->At.data(), At, ns * sizeof(double));
     if (Eobs) memcpy(c->Eobs.data(), Eobs, no * sizeof(double));
     if (Aobs) memcpy(c->Aobs.data(), Aobs, no * sizeof(double));
+    // E(t_n) on the device: the field of the records that observable passes
+    // over saved states finalize
+    if (c->d_Eo) HIPCK(c, hipFree(c->d_Eo));
+    c->d_Eo = nullptr;
+    HIPCK(c, hipMalloc(&c->d_Eo, (no ? no : 1) * sizeof(double)));
+    if (no) HIPCK(c, hipMemcpy(c->d_Eo, c->Eobs.data(), no * sizeof(double), hipMemcpyHostToDevice));
     c->ntime = ntime;
     ++c->setup_gen;
     return 0;

@@ -750,6 +750,12 @@ enum PassFlags : unsigned {
     PF_GRAD = 1u << 12,      // partials of x_j Im(conj(u) phi) and dV/da Im(conj(u) phi), phi from `save`
     PF_SRC = 1u << 13,       // u += sum_j src_j (w_j - E_j) y + src_norm y (y from `save`), w = -grad V
     PF_KTAB = 1u << 14,      // kinetic factor read from `ktab` (tabulated e^{-i c K}, four values per thread)
+    // merged step boundary (2-D length gauge): K(a_last) | record | Mask | K(a_0)
+    PF_KSPLIT = 1u << 15,    // kinetic pass: after the factor and inverse, [store to `save`], [this axis's
+                             // mask], forward, this axis's factor from `atab`, inverse
+    PF_PREKX = 1u << 16,     // axis-0 pass: after the first transform, [axis-0 mask], forward, axis-0
+                             // factor from `atab`, inverse, then the r-space operators
+    PF_NOSTORE = 1u << 17,   // no store (observable-only pass)
 };
 enum KMode { KM_PHASE = 0, KM_VELOCITY = 1, KM_IMAG = 2 };
 enum VMode { VM_LENGTH = 0, VM_VELOCITY = 1, VM_IMAG = 2 };
@@ -795,6 +801,9 @@ template <typename R> struct PassArgs {
     // kinetic factors e^{-i c K} of this pass's coefficient at every element of
     // one instance (row-major), when tabulated (PF_KTAB)
     const typename TwT<R>::T *ktab;
+    // per-axis kinetic factors e^{-i c K_j} of one coefficient (the step's
+    // first, a_0), Sigma n_j entries at the per-axis offsets (PF_KSPLIT, PF_PREKX)
+    const typename TwT<R>::T *atab;
     // adjoint passes: forward states saved in position space, source weights
     C<R> *save;
     double src[QND_MAXD];
@@ -1138,6 +1147,23 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         // ---- first transform ----
         transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
 
+        if (KIND == PK_POT && (fl & PF_PREKX)) {
+            // the step's first kinetic factor along axis 0, after the owed
+            // axis-0 mask (the kinetic pass before applied the other axes'
+            // parts; the factors of different axes commute with each other
+            // and with the other axes' masks)
+            const bool mk0 = (fl & PF_MASK) && P.mask_on;
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                const R f = mk0 ? (R)(msk_at(m) * a.scale1) : (R)a.scale1;
+                x[m] = scl<R>(x[m], f);
+            }
+            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(a.atab[P.off[ax] + t + m * T], cs < 0.0));
+            transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+        }
+
         if (KIND == PK_KIN) {
             // all axes in k-space: the kinetic factor of this pass's axes
             // (reference order of the sums).  exp(-i c sum_j K_j) is a product
@@ -1295,7 +1321,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                         }
                     }
                 }
-                if ((fl & PF_MASK) && P.mask_on) {
+                if ((fl & PF_MASK) && !(fl & PF_PREKX) && P.mask_on) {
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
                         const int m = m0 + i;
@@ -1372,11 +1398,33 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                 for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], s2);
             }
         }
+        if (KIND == PK_KIN && (fl & PF_KSPLIT)) {
+            // merged step boundary: the state after the step's last kinetic
+            // factor is the record's state (saved for the observable pass);
+            // then this axis's part of the owed mask and of the next step's
+            // first kinetic factor
+            if (fl & PF_SAVE) {
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) a.save[lbase + (int64_t)(t + m * T) * estride] = x[m];
+            }
+            if ((fl & PF_MASK) && P.mask_on) {
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], (R)P.maskt[P.off[ax] + t + m * T]);
+            }
+            transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            const R s2 = (R)a.scale2;
+#pragma unroll
+            for (int m = 0; m < VPT; ++m)
+                x[m] = scl<R>(fac_mul<R>(x[m], tab_fac<R>(a.atab[P.off[ax] + t + m * T], cs < 0.0)), s2);
+            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+        }
     }
 
     // ---- store ----
+    if (!(POTK && (fl & PF_NOSTORE))) {
 #pragma unroll
-    for (int m = 0; m < VPT; ++m) data[lbase + (int64_t)(t + m * T) * estride] = x[m];
+        for (int m = 0; m < VPT; ++m) data[lbase + (int64_t)(t + m * T) * estride] = x[m];
+    }
 
     // ---- Gram-Schmidt dots on the stored representation ----
     if (POTK && (fl & PF_GS_DOTS)) {

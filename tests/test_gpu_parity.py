@@ -198,6 +198,55 @@ def test_real_steps_3d_geometries(gpu, prec, gauge, tun):
     g.close()
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_merged_boundary_matches_single_steps(gpu, prec):
+    """One call of many length-gauge steps merges each step boundary (the
+    record's state saved to a ring, the mask and the first kinetic factor
+    split by axis over the neighbouring passes, records from one batched pass
+    per ring fill; 150 steps wrap the 64-record ring twice).  One-step calls
+    take the unmerged boundary pass.  Both give the same states and records."""
+    nr, dr, pot = (32, 48), 0.4, "Ne_2D"
+    print(f"case merged-boundary prec={prec}", flush=True)
+    dt, nt = 0.05, 150
+    o = O.Oracle(nr, dr, pot, "BM4", dt, cap=(0.05, 1.0))
+    E, A, Eobs, Aobs = field_tables(o, nt, 2)
+    psi0 = smooth_state(o, 6)[0]
+    names = ["dipole_r", "norm", "dipole_a"]
+    runs = []
+    for split in (False, True):
+        g = q.Grid(nr, dr, prec, n_slots=1)
+        g.set_potential(pot, cap=(0.05, 1.0))
+        g.set_integrator("BM4", dt)
+        g.set_mask(0.1)
+        g.set_field(E, A, Eobs, Aobs)
+        g.upload(0, psi0)
+        if split:
+            obs = np.vstack([g.real_steps(0, "length", k, 1, names) for k in range(nt)])
+        else:
+            obs = g.real_steps(0, "length", 0, nt, names)
+        runs.append((g.download(0), obs))
+        g.close()
+    (p1, o1), (p2, o2) = runs
+    o.set_mask(0.1)
+    ref_psi, ref_obs = o.real_run(psi0, "length", E, A, Eobs, Aobs, names)
+    if prec == "c128":
+        assert rel(p1, p2) < TOL128, rel(p1, p2)
+        assert np.allclose(o1, o2, rtol=1e-11, atol=1e-12), np.abs(o1 - o2).max()
+        assert rel(p1, ref_psi) < TOL128, rel(p1, ref_psi)
+        assert np.allclose(o1, ref_obs, rtol=1e-11, atol=1e-12), np.abs(o1 - ref_obs).max()
+    else:
+        # two FP32 rounding sequences of the same operators: each within the
+        # complex64 error of the FP64 oracle, the merged one no worse
+        e1, e2 = rel(p1, ref_psi), rel(p2, ref_psi)
+        r1 = np.abs(o1 - ref_obs).max()
+        r2 = np.abs(o2 - ref_obs).max()
+        print(f"c64 vs oracle: merged {e1:.3e} (records {r1:.3e}), one-step calls {e2:.3e} (records {r2:.3e})")
+        assert e1 < 1e-4 and e2 < 1e-4, (e1, e2)
+        assert e1 < 2.0 * e2 + 1e-6, (e1, e2)
+        scale = np.abs(ref_obs).max()
+        assert r1 < 1e-3 * scale and r1 < 2.0 * r2 + 1e-7, (r1, r2, scale)
+
+
 @pytest.mark.parametrize("gauge", ["length", "velocity"])
 def test_real_unfused_observables(gpu, gauge):
     nr, dr, pot = (32, 32), 0.4, "H_2D"
