@@ -359,6 +359,20 @@ Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1)
     return g;
 }
 
+// The automatic geometry, ignoring the tuning overrides (which are chosen
+// for the passes of one state): for the batched observable pass over saved
+// record states.
+Geo geometry_auto(qnd_ctx *c, int axis, int nb)
+{
+    const int vo = c->vpt_override, t0 = c->tw_override[0], t1 = c->tw_override[1];
+    c->vpt_override = c->tw_override[0] = c->tw_override[1] = 0;
+    const Geo g = geometry(c, axis, nb);
+    c->vpt_override = vo;
+    c->tw_override[0] = t0;
+    c->tw_override[1] = t1;
+    return g;
+}
+
 template <typename R> PassArgs<R> base_args(qnd_ctx *c, C<R> *data, const Geo &g)
 {
     PassArgs<R> a;
@@ -1037,7 +1051,7 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
                 HIPCK(c, hipMalloc(&c->d_side, (size_t)side_slots * sbytes));
                 c->side_cap = (size_t)side_slots * sbytes;
             }
-            gside = geometry(c, 0, side_slots);
+            gside = geometry_auto(c, 0, side_slots);
             if ((rc = ensure_part(c, (size_t)std::max(g0.grid, gside.grid) * QND_NOBS))) return rc;
         }
     }
@@ -1048,7 +1062,7 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
     // ring (a batch of side_n instances), then one finalize block per record
     auto flush_side = [&]() -> int {
         if (side_n == 0) return 0;
-        const Geo gs = side_n == side_slots ? gside : geometry(c, 0, side_n);
+        const Geo gs = side_n == side_slots ? gside : geometry_auto(c, 0, side_n);
         PassArgs<R> a = base_args<R>(c, side, gs);
         a.scale1 = inv_n0;
         a.flags = PF_OBS | PF_MASK | PF_NOSTORE;
