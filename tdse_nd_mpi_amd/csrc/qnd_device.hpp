@@ -1134,6 +1134,25 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             }
         }
 
+        // merged step boundary: per-axis kinetic factors (and the kinetic
+        // pass's mask values) of this thread's elements; with four values per
+        // thread requested before the transform, else read where used
+        typename TwT<R>::T atv[VPT];
+        double amv[VPT];
+        const bool split = (KIND == PK_KIN && (fl & PF_KSPLIT)) || (KIND == PK_POT && (fl & PF_PREKX));
+        if constexpr (VPT == 4) {
+            if (split) {
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) atv[m] = a.atab[P.off[ax] + t + m * T];
+                if (KIND == PK_KIN && (fl & PF_MASK) && P.mask_on) {
+#pragma unroll
+                    for (int m = 0; m < VPT; ++m) amv[m] = P.maskt[P.off[ax] + t + m * T];
+                }
+            }
+        }
+        auto atab_at = [&](int m) { return VPT == 4 ? atv[m] : a.atab[P.off[ax] + t + m * T]; };
+        auto kmask_at = [&](int m) { return VPT == 4 ? amv[m] : P.maskt[P.off[ax] + t + m * T]; };
+
         if (KIND == PK_POT1D && (fl & PF_KPRE)) {
 #pragma unroll
             for (int m = 0; m < VPT; ++m) {
@@ -1160,7 +1179,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             }
             transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(a.atab[P.off[ax] + t + m * T], cs < 0.0));
+            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(atab_at(m), cs < 0.0));
             transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
         }
 
@@ -1409,13 +1428,13 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             }
             if ((fl & PF_MASK) && P.mask_on) {
 #pragma unroll
-                for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], (R)P.maskt[P.off[ax] + t + m * T]);
+                for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], (R)kmask_at(m));
             }
             transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
             const R s2 = (R)a.scale2;
 #pragma unroll
             for (int m = 0; m < VPT; ++m)
-                x[m] = scl<R>(fac_mul<R>(x[m], tab_fac<R>(a.atab[P.off[ax] + t + m * T], cs < 0.0)), s2);
+                x[m] = scl<R>(fac_mul<R>(x[m], tab_fac<R>(atab_at(m), cs < 0.0)), s2);
             transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
         }
     }
