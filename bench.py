@@ -63,7 +63,7 @@ def ground_state(q):
     return psi, E, nsteps, t_imag
 
 
-def real_run(q, field, prec, psi0, warmup, steps, Et, At, Eo, Ao, profile_steps=0, tune=True):
+def real_run(q, field, prec, psi0, warmup, steps, Et, At, Eo, Ao, profile_steps=0, tune=True, fixed=None):
     g = q.Grid(NR, DR, prec, n_slots=1)
     g.set_potential(POT)
     g.set_integrator("BM4", DT)
@@ -71,8 +71,13 @@ def real_run(q, field, prec, psi0, warmup, steps, Et, At, Eo, Ao, profile_steps=
     g.set_field(Et, At, Eo, Ao)
     g.upload(0, psi0)
     from tdse_nd_mpi_amd import tuning
-    best, table = (tuning.autotune_batch(g, 0, 1, "length", 0, nsteps=8)   # restores psi0
-                   if tune else ((0, 0, 0), []))
+    if fixed:
+        # a fixed launch geometry: the same kernels with and without a profiler
+        g.set_tuning(*fixed)
+        best, table = tuple(fixed), []
+    else:
+        best, table = (tuning.autotune_batch(g, 0, 1, "length", 0, nsteps=8)   # restores psi0
+                       if tune else ((0, 0, 0), []))
     names = ["dipole_r", "dipole_a", "norm"]
     obs_w = g.real_steps(0, "length", 0, warmup, names) if warmup else np.zeros((0, 5))
     import torch
@@ -435,6 +440,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for counter runs: a 256 MiB device copy (calibration), then 20 c64 steps; no JSON")
     ap.add_argument("--dump", default="", help="write the observable traces to this .npz")
+    ap.add_argument("--tuning", default="",
+                    help="fixed launch geometry 'vpt,tw_contig,tw_strided' for both precisions instead of the "
+                         "autotuner (profiled runs: the profiler distorts the autotuner's timings)")
     ap.add_argument("--only", choices=["eig1d", "big3d", "adjoint", "sweep"], default=None,
                     help="run one secondary workload alone and print its JSON (for kernel-trace profiles)")
     args = ap.parse_args()
@@ -485,8 +493,9 @@ def main():
     res = {}
     for prec in (["c64"] if args.skip_fp64 else ["c64", "c128"]):
         log(f"real time {prec}: warmup {args.warmup}, timed {args.steps} ...")
+        fixed = tuple(int(v) for v in args.tuning.split(",")) if args.tuning else None
         obs, wall, gpu_ms, prof, tuned = real_run(q, F, prec, psi0, args.warmup, args.steps, Et, At, Eo, Ao,
-                                                  profile_steps=args.profile_steps)
+                                                  profile_steps=args.profile_steps, fixed=fixed)
         res[prec] = dict(obs=obs, wall=wall, gpu_ms=gpu_ms, prof=prof, tuned=tuned)
         log(f"  {prec}: {wall * 1e3 / args.steps:.4f} ms/step wall, {gpu_ms / args.steps:.4f} ms/step gpu")
 

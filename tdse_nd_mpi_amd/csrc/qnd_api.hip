@@ -1404,93 +1404,152 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
     auto kin_adj = [&](C<R> *data, double ck) -> int {
         return kinetic_block<R>(c, data, KM_PHASE, ck, nullptr, nullptr, 1, FieldRows(), PF_CONJ);
     };
-    // recompute segment sg from its checkpoint on the second stream, saving y_n
-    // and phi_{n,i}; it may start once the adjoint that last read its buffer is done
-    auto recompute = [&](int64_t sg) -> int {
-        std::swap(c->stream, c->stream2);
-        struct Restore {
-            qnd_ctx *c;
-            ~Restore() { std::swap(c->stream, c->stream2); }
-        } restore{c};
-        if (sg + 2 < nseg) HIPCK(c, hipStreamWaitEvent(c->stream, ev_adj[sg + 2], 0));
-        const int64_t nA = sg * K, len = (nA + K <= nsteps) ? K : nsteps - nA;
-        C<R> *saves = saves_of(sg);
-        HIPCK(c, hipMemcpyAsync(W, reinterpret_cast<C<R> *>(c->d_adj_ckpt) + sg * c->N, sbytes,
+    // Recompute of segment sg from its checkpoint on the second stream
+    // (saving y_n and phi_{n,i}; it may start once the sweep that last read
+    // its buffer is done), and the costate sweep of segment sg on the first
+    // stream (once the recompute of sg is done).  The recompute of sg - 1 and
+    // the sweep of sg are enqueued step by step, interleaved, so that both
+    // streams have work queued together and run concurrently.
+    // The recompute merges its step boundaries as real_steps does (the first
+    // step of a segment starts from the checkpoint with the boundary pass):
+    // the last kinetic pass of step k saves the record's state y_{k+1} in
+    // the axis-0 Fourier representation, which the sweep's source pass turns
+    // into position space itself (ysrcm).
+    std::vector<std::vector<char>> ysrcm(nseg);
+    const typename TwT<R>::T *atab = atab_for<R>(c, &rc);
+    if (rc) return cleanup(), rc;
+    struct Seg {
+        int64_t sg = -1, nA = 0, len = 0, k = 0;
+        C<R> *saves = nullptr;
+        double *part = nullptr;
+        SlotState st;
+        bool merged_in = false;
+    };
+    auto seg_of = [&](int64_t sg) {
+        Seg g;
+        g.sg = sg;
+        g.nA = sg * K;
+        g.len = (g.nA + K <= nsteps) ? K : nsteps - g.nA;
+        g.saves = saves_of(sg);
+        g.part = part_of(sg);
+        g.st = c->adj_ckpt_state[sg];
+        return g;
+    };
+    struct OnStream2 {
+        qnd_ctx *c;
+        explicit OnStream2(qnd_ctx *cc) : c(cc) { std::swap(c->stream, c->stream2); }
+        ~OnStream2() { std::swap(c->stream, c->stream2); }
+    };
+    auto rec_begin = [&](Seg &g) -> int {
+        OnStream2 on2(c);
+        if (g.sg + 2 < nseg) HIPCK(c, hipStreamWaitEvent(c->stream, ev_adj[g.sg + 2], 0));
+        HIPCK(c, hipMemcpyAsync(W, reinterpret_cast<C<R> *>(c->d_adj_ckpt) + g.sg * c->N, sbytes,
                                 hipMemcpyDeviceToDevice, c->stream));
-        SlotState st = c->adj_ckpt_state[sg];
-        if (st.rep == REP_R) {
+        if (g.st.rep == REP_R) {
             if (int r = plain_pass<R>(c, W, 0, -1, 1.0)) return r;
-            st.rep = REP_M;
+            g.st.rep = REP_M;
         }
-        masked[sg].assign(len, 0);
-        for (int64_t k = 0; k < len; ++k) {
-            const int64_t n = nA + k;
-            const double *Et = &c->Et[(size_t)n * s * d];
-            PassArgs<R> b = pot(W);
-            b.flags |= PF_SAVE | PF_OBS | (st.pend_mask ? PF_MASK : 0u);
-            b.save = saves + (k * per_step) * c->N;
-            b.partials = c->d_part;
-            b.npart = QND_NOBS;
-            masked[sg][k] = st.pend_mask;
-            if (int r = launch<R, PK_POT, +1>(c, g0, b, PC_POT)) return r;
-            if (int r = kinetic_block<R>(c, W, KM_PHASE, kcoef(0), nullptr)) return r;
-            for (int ia = 0; ia < s; ++ia) {
-                PassArgs<R> p = pot(W);
-                p.flags |= PF_V1 | PF_SAVE;
-                p.vmode = VM_LENGTH;
-                p.cv1 = as[2 * ia] + as[2 * ia + 1];
-                for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
-                p.save = saves + (k * per_step + 1 + ia) * c->N;
-                if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
-                if (int r = kinetic_block<R>(c, W, KM_PHASE, kcoef(ia + 1), nullptr)) return r;
-            }
-            st.pend_mask = c->P.mask_on != 0;
-        }
-        HIPCK(c, hipEventRecord(ev_rec[sg], c->stream));
+        masked[g.sg].assign(g.len, 0);
+        ysrcm[g.sg].assign(g.len, 0);
         return 0;
     };
-    // adjoint sweep over segment sg on the first stream
-    auto sweep = [&](int64_t sg) -> int {
-        HIPCK(c, hipStreamWaitEvent(c->stream, ev_rec[sg], 0));
-        const int64_t nA = sg * K, len = (nA + K <= nsteps) ? K : nsteps - nA;
-        C<R> *saves = saves_of(sg);
-        double *part = part_of(sg);
-        for (int64_t k = len - 1; k >= 0; --k) {
-            const int64_t n = nA + k;
-            const double *Et = &c->Et[(size_t)n * s * d];
-            const double *Eo = &c->Eobs[(size_t)n * d];
-            if (int r = kin_adj(lam, kcoef(s))) return r;
-            for (int ia = s - 1; ia >= 0; --ia) {
-                PassArgs<R> p = pot(lam);
-                p.flags |= PF_V1 | PF_CONJ | PF_GRAD;
-                p.vmode = VM_LENGTH;
-                p.cv1 = as[2 * ia] + as[2 * ia + 1];
-                for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
-                p.save = saves + (k * per_step + 1 + ia) * c->N;
-                p.partials = part + (size_t)(k * per_step + 1 + ia) * nwg * QND_NOBS;
-                p.npart = QND_NOBS;
-                if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
-                if (int r = kin_adj(lam, kcoef(ia))) return r;
-            }
-            PassArgs<R> b = pot(lam);
-            b.flags |= PF_SRC | (masked[sg][k] ? PF_MASK : 0u);
-            for (int j = 0; j < d; ++j) {
-                b.src[j] = dL_da[n * d + j] * dNr * inv_n0;
-                b.Ev[j] = Eo[j];
-            }
-            b.save = saves + (k * per_step) * c->N;
-            b.partials = part + (size_t)(k * per_step) * nwg * QND_NOBS;
-            b.npart = QND_NOBS;
+    auto rec_step = [&](Seg &g) -> int {
+        OnStream2 on2(c);
+        const int64_t k = g.k++, n = g.nA + k;
+        const double *Et = &c->Et[(size_t)n * s * d];
+        masked[g.sg][k] = g.st.pend_mask;
+        const bool merged_in = g.merged_in;
+        if (!merged_in) {
+            PassArgs<R> b = pot(W);
+            b.flags |= PF_SAVE | (g.st.pend_mask ? PF_MASK : 0u);
+            b.save = g.saves + (k * per_step) * c->N;
             if (int r = launch<R, PK_POT, +1>(c, g0, b, PC_POT)) return r;
+            if (int r = kinetic_block<R>(c, W, KM_PHASE, kcoef(0), nullptr)) return r;
         }
+        g.merged_in = false;
+        for (int ia = 0; ia < s; ++ia) {
+            PassArgs<R> p = pot(W);
+            p.flags |= PF_V1 | PF_SAVE;
+            p.vmode = VM_LENGTH;
+            p.cv1 = as[2 * ia] + as[2 * ia + 1];
+            for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
+            p.save = g.saves + (k * per_step + 1 + ia) * c->N;
+            if (ia == 0 && merged_in) {
+                p.flags |= PF_PREKX | (g.st.pend_mask ? PF_MASK : 0u);
+                p.atab = atab;
+            }
+            if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
+            if (ia == s - 1 && k + 1 < g.len && d == 2) {
+                // merged boundary: K(a_last), save y_{k+1} (axis-0 Fourier), axis-1 parts
+                const Geo gk = geometry(c, 1, 1, PK_KIN);
+                PassArgs<R> q = base_args<R>(c, W, gk);
+                q.flags = PF_KSPLIT | PF_SAVE | (c->P.mask_on ? PF_MASK : 0u);
+                q.kmode = KM_PHASE;
+                q.ck_mid = kcoef(s);
+                q.scale2 = 1.0 / (double)c->nr[1];
+                int r = 0;
+                q.ktab = ktab_for<R>(c, q.ck_mid, gk, &r);
+                if (r) return r;
+                if (q.ktab) q.flags |= PF_KTAB;
+                q.atab = atab;
+                q.save = g.saves + ((k + 1) * per_step) * c->N;
+                if ((r = launch<R, PK_KIN, -1>(c, gk, q, PC_KIN))) return r;
+                ysrcm[g.sg][k + 1] = 1;
+                g.merged_in = true;
+            } else if (int r = kinetic_block<R>(c, W, KM_PHASE, kcoef(ia + 1), nullptr)) {
+                return r;
+            }
+        }
+        g.st.pend_mask = c->P.mask_on != 0;
+        return 0;
+    };
+    auto rec_end = [&](Seg &g) -> int {
+        OnStream2 on2(c);
+        HIPCK(c, hipEventRecord(ev_rec[g.sg], c->stream));
+        return 0;
+    };
+    auto sweep_begin = [&](Seg &g) -> int {
+        HIPCK(c, hipStreamWaitEvent(c->stream, ev_rec[g.sg], 0));
+        g.k = g.len - 1;
+        return 0;
+    };
+    auto sweep_step = [&](Seg &g) -> int {
+        const int64_t k = g.k--, n = g.nA + k;
+        const double *Et = &c->Et[(size_t)n * s * d];
+        const double *Eo = &c->Eobs[(size_t)n * d];
+        if (int r = kin_adj(lam, kcoef(s))) return r;
+        for (int ia = s - 1; ia >= 0; --ia) {
+            PassArgs<R> p = pot(lam);
+            p.flags |= PF_V1 | PF_CONJ | PF_GRAD;
+            p.vmode = VM_LENGTH;
+            p.cv1 = as[2 * ia] + as[2 * ia + 1];
+            for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
+            p.save = g.saves + (k * per_step + 1 + ia) * c->N;
+            p.partials = g.part + (size_t)(k * per_step + 1 + ia) * nwg * QND_NOBS;
+            p.npart = QND_NOBS;
+            if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
+            if (int r = kin_adj(lam, kcoef(ia))) return r;
+        }
+        PassArgs<R> b = pot(lam);
+        b.flags |= PF_SRC | (masked[g.sg][k] ? PF_MASK : 0u) | (ysrcm[g.sg][k] ? PF_SRCM : 0u);
+        for (int j = 0; j < d; ++j) {
+            b.src[j] = dL_da[n * d + j] * dNr * inv_n0;
+            b.Ev[j] = Eo[j];
+        }
+        b.save = g.saves + (k * per_step) * c->N;
+        b.partials = g.part + (size_t)(k * per_step) * nwg * QND_NOBS;
+        b.npart = QND_NOBS;
+        return launch<R, PK_POT, +1>(c, g0, b, PC_POT);
+    };
+    auto sweep_end = [&](Seg &g) -> int {
         {
             ProfScope ps(c, PC_SMALL);
-            const int npass = (int)(len * per_step);
+            const int npass = (int)(g.len * per_step);
             hipLaunchKernelGGL(k_sum_blocks, dim3((npass * QND_NOBS + 3) / 4), dim3(256), 0, c->stream,
-                               (const double *)part, npass, nwg, QND_NOBS, gsum_dev + nA * per_step * QND_NOBS);
+                               (const double *)g.part, npass, nwg, QND_NOBS, gsum_dev + g.nA * per_step * QND_NOBS);
             HIPCK(c, hipGetLastError());
         }
-        HIPCK(c, hipEventRecord(ev_adj[sg], c->stream));
+        HIPCK(c, hipEventRecord(ev_adj[g.sg], c->stream));
         return 0;
     };
 
@@ -1507,10 +1566,27 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
     }
     // the second stream must not start before the checkpoints are written
     HIPCK(c, hipStreamSynchronize(c->stream));
-    if ((rc = recompute(nseg - 1))) return cleanup(), rc;
+    {
+        Seg last = seg_of(nseg - 1);
+        if ((rc = rec_begin(last))) return cleanup(), rc;
+        while (last.k < last.len)
+            if ((rc = rec_step(last))) return cleanup(), rc;
+        if ((rc = rec_end(last))) return cleanup(), rc;
+    }
     for (int64_t sg = nseg - 1; sg >= 0; --sg) {
-        if (sg > 0 && (rc = recompute(sg - 1))) return cleanup(), rc;
-        if ((rc = sweep(sg))) return cleanup(), rc;
+        Seg sw = seg_of(sg), rc_seg;
+        const bool rec = sg > 0;
+        if (rec) {
+            rc_seg = seg_of(sg - 1);
+            if ((rc = rec_begin(rc_seg))) return cleanup(), rc;
+        }
+        if ((rc = sweep_begin(sw))) return cleanup(), rc;
+        while (sw.k >= 0 || (rec && rc_seg.k < rc_seg.len)) {
+            if (rec && rc_seg.k < rc_seg.len && (rc = rec_step(rc_seg))) return cleanup(), rc;
+            if (sw.k >= 0 && (rc = sweep_step(sw))) return cleanup(), rc;
+        }
+        if (rec && (rc = rec_end(rc_seg))) return cleanup(), rc;
+        if ((rc = sweep_end(sw))) return cleanup(), rc;
     }
     std::vector<double> gsum(ngsum);
     HIPCK(c, hipMemcpyAsync(gsum.data(), gsum_dev, ngsum * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -756,6 +756,8 @@ enum PassFlags : unsigned {
     PF_PREKX = 1u << 16,     // axis-0 pass: after the first transform, [axis-0 mask], forward, axis-0
                              // factor from `atab`, inverse, then the r-space operators
     PF_NOSTORE = 1u << 17,   // no store (observable-only pass)
+    PF_SRCM = 1u << 18,      // with PF_SRC: `save` holds the record's state in the axis-0 Fourier
+                             // representation (merged boundary); y = [Mask] F0^-1 save / n0
 };
 enum KMode { KM_PHASE = 0, KM_VELOCITY = 1, KM_IMAG = 2 };
 enum VMode { VM_LENGTH = 0, VM_VELOCITY = 1, VM_IMAG = 2 };
@@ -1166,6 +1168,21 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         // ---- first transform ----
         transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
 
+        // adjoint source of a merged boundary: the record's state y from its
+        // saved axis-0 Fourier representation (inverse transform, scale, mask)
+        C<R> ysv[VPT];
+        if (KIND == PK_POT && (fl & PF_SRCM)) {
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) ysv[m] = a.save[lbase + (int64_t)(t + m * T) * estride];
+            transform<R, DIR1, VPT>(ysv, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            const bool mky = (fl & PF_MASK) && P.mask_on;
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                const double f = mky ? msk_at(m) * mko * a.scale1 : a.scale1;
+                ysv[m] = mk<R>((R)((double)ysv[m].x * f), (R)((double)ysv[m].y * f));
+            }
+        }
+
         if (KIND == PK_POT && (fl & PF_PREKX)) {
             // the step's first kinetic factor along axis 0, after the owed
             // axis-0 mask (the kinetic pass before applied the other axes'
@@ -1287,7 +1304,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                     for (int i = 0; i < G; ++i) {
                         const int m = m0 + i;
                         // adjoint: u = costate here; phi / y = forward state saved here
-                        const C<R> f = a.save[lbase + (int64_t)(t + m * T) * estride];
+                        const C<R> f = (KIND == PK_POT && (fl & PF_SRCM)) ? ysv[m]
+                                                                           : a.save[lbase + (int64_t)(t + m * T) * estride];
                         double dV_da, dVdu_da;
                         softcore_derivs(P, rr[i], dV_da, dVdu_da);
                         if (fl & PF_GRAD) {
