@@ -819,19 +819,63 @@ int observables(qnd_ctx *c, int slot, const int *codes, int ncodes, const double
 }
 
 // Gram matrix of slots 0..nprev-1 in the M representation, into d_small.
+// G is Hermitian: the entries with i <= j are reduced, G_ji = conj(G_ij).
+// Up to QND_GRAM_MAX states one kernel reads each state once and sums every
+// pair; beyond that, one reduction per pair.
+template <typename R, int NP> void launch_gram(qnd_ctx *c, int grid)
+{
+    hipLaunchKernelGGL((k_gram<R, NP>), dim3(grid), dim3(256), 0, c->stream, state_ptr<R>(c, 0), c->N, c->N,
+                       c->d_part);
+}
 template <typename R> int gram(qnd_ctx *c, int nprev)
 {
     std::vector<double> G(2 * (size_t)nprev * nprev, 0.0);
     const double scale = c->P.dNr / (double)c->nr[0];
-    for (int i = 0; i < nprev; ++i)
-        for (int j = 0; j < nprev; ++j) {
-            double v[2];
-            int rc = reduce<R>(c, RM_DOT, state_ptr<R>(c, i), state_ptr<R>(c, j), nullptr, 0, 0, nullptr,
-                               2, scale, v);
-            if (rc) return rc;
-            G[2 * (i * nprev + j)] = v[0];
-            G[2 * (i * nprev + j) + 1] = v[1];
+    auto put = [&](int i, int j, const double *v) {
+        G[2 * (i * nprev + j)] = v[0];
+        G[2 * (i * nprev + j) + 1] = i == j ? 0.0 : v[1];
+        G[2 * (j * nprev + i)] = v[0];
+        G[2 * (j * nprev + i) + 1] = i == j ? 0.0 : -v[1];
+    };
+    if (nprev <= QND_GRAM_MAX) {
+        const int npair = nprev * (nprev + 1) / 2, nq = 2 * npair;
+        const int grid = flat_grid(c->N, 8, 512, 8192);
+        int rc = ensure_part(c, (size_t)grid * nq);
+        if (rc) return rc;
+        std::vector<double> v(nq);
+        {
+            ProfScope ps(c, PC_SMALL);
+            switch (nprev) {
+            case 1: launch_gram<R, 1>(c, grid); break;
+            case 2: launch_gram<R, 2>(c, grid); break;
+            case 3: launch_gram<R, 3>(c, grid); break;
+            case 4: launch_gram<R, 4>(c, grid); break;
+            case 5: launch_gram<R, 5>(c, grid); break;
+            case 6: launch_gram<R, 6>(c, grid); break;
+            case 7: launch_gram<R, 7>(c, grid); break;
+            default: launch_gram<R, 8>(c, grid); break;
+            }
+            HIPCK(c, hipGetLastError());
+            hipLaunchKernelGGL(k_sum_partials, dim3((nq + 3) / 4), dim3(256), 0, c->stream, c->d_part, grid, nq, nq,
+                               scale, c->d_small + SMALL_SUMS);
+            HIPCK(c, hipGetLastError());
         }
+        HIPCK(c, hipMemcpyAsync(v.data(), c->d_small + SMALL_SUMS, nq * sizeof(double), hipMemcpyDeviceToHost,
+                                c->stream));
+        HIPCK(c, hipStreamSynchronize(c->stream));
+        int p = 0;
+        for (int i = 0; i < nprev; ++i)
+            for (int j = i; j < nprev; ++j, ++p) put(i, j, &v[2 * p]);
+    } else {
+        for (int i = 0; i < nprev; ++i)
+            for (int j = i; j < nprev; ++j) {
+                double v[2];
+                int rc = reduce<R>(c, RM_DOT, state_ptr<R>(c, i), state_ptr<R>(c, j), nullptr, 0, 0, nullptr,
+                                   2, scale, v);
+                if (rc) return rc;
+                put(i, j, v);
+            }
+    }
     HIPCK(c, hipMemcpyAsync(c->d_small + SMALL_GRAM, G.data(), G.size() * sizeof(double),
                             hipMemcpyHostToDevice, c->stream));
     return 0;

@@ -1702,6 +1702,44 @@ __global__ void k_elem(int mode, C<R> *dst, const C<R> *src, int axis, double sh
     }
 }
 
+// Gram matrix of NP states (slots base + i * stride, i < NP) in one read of
+// each: per-workgroup FP64 partials of sum conj(a_i) a_j for the pairs
+// i <= j in row-major order of the upper triangle, (re, im) per pair, with
+// the arithmetic of the RM_DOT reduction below.
+#define QND_GRAM_MAX 8
+template <typename R, int NP>
+__global__ void __launch_bounds__(256) k_gram(const C<R> *__restrict__ base, int64_t stride, int64_t N,
+                                              double *partials)
+{
+    constexpr int NPAIR = NP * (NP + 1) / 2;
+    __shared__ double red[16 * 2 * NPAIR];
+    double acc[2 * NPAIR];
+#pragma unroll
+    for (int q = 0; q < 2 * NPAIR; ++q) acc[q] = 0.0;
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < N;
+         f += (int64_t)gridDim.x * blockDim.x) {
+        double vr[NP], vi[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const C<R> v = base[i * stride + f];
+            vr[i] = v.x;
+            vi[i] = v.y;
+        }
+        int p = 0;
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+#pragma unroll
+            for (int j = i; j < NP; ++j, ++p) {
+                acc[2 * p] += vr[i] * vr[j] + vi[i] * vi[j];
+                acc[2 * p + 1] += vr[i] * vi[j] - vi[i] * vr[j];
+            }
+    }
+    block_reduce_lds(acc, 2 * NPAIR, red);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 2 * NPAIR; ++q) partials[(int64_t)blockIdx.x * (2 * NPAIR) + q] = acc[q];
+}
+
 enum RedMode { RM_DOT = 0, RM_DOTX = 1, RM_PMOM = 2, RM_OBS = 3, RM_HCOMB = 4 };
 
 // Per-workgroup partial sums (FP64).  Layout per workgroup: npart doubles.

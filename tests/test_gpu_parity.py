@@ -90,6 +90,29 @@ def test_imag_steps_gram_schmidt(gpu, nr, dr, pot, integ):
     g.close()
 
 
+def test_gram_schmidt_ten_states(gpu):
+    """Ten states: the Gram matrix of up to eight previous states comes from one
+    pass over them, of nine from one reduction per pair; both against the oracle."""
+    nr, dr, pot, nst, nsteps = (16, 32), 0.5, "H_2D", 10, 4
+    print("case imag ten states", flush=True)
+    o = O.Oracle(nr, dr, pot, "BM4", 0.05)
+    g = q.Grid(nr, dr, "c128", n_slots=nst)
+    g.set_potential(pot)
+    g.set_integrator("BM4", 0.05)
+    states = smooth_state(o, 13, nst)
+    ref = states.copy()
+    for j in range(nst):
+        g.upload(j, states[j])
+        for _ in range(nsteps):
+            ref[j] = o.step_imag(ref[j])
+            rc, ref = o.gram_schmidt(ref, j)
+            assert rc == 0
+        g.imag_steps(j, nsteps, gram_schmidt=True)
+    for j in range(nst):
+        assert rel(g.download(j), ref[j]) < TOL128, (j, rel(g.download(j), ref[j]))
+    g.close()
+
+
 @pytest.mark.parametrize("nr,dr,pot", [((128,), 0.3, "H_1D"), ((32, 64), 0.4, "Ne_2D"), ((16, 8, 16), 0.5, "H_3D")])
 def test_observables_match_oracle(gpu, nr, dr, pot):
     print(f"case observables nr={nr} pot={pot}", flush=True)
