@@ -1256,7 +1256,7 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
                             k.save = side + (size_t)side_n * c->N;
                             ++side_n;
                         }
-                        if ((rc = launch<R, PK_KIN, -1>(c, gk, k, PC_KIN))) return rc;
+                        if ((rc = launch<R, PK_KINS, -1>(c, gk, k, PC_KIN))) return rc;
                         merged_out = true;
                     } else if ((rc = kinetic_block<R>(c, psi, KM_PHASE, ck, nullptr, nullptr, nb))) {
                         return rc;
@@ -1537,7 +1537,7 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
                 if (q.ktab) q.flags |= PF_KTAB;
                 q.atab = atab;
                 q.save = g.saves + ((k + 1) * per_step) * c->N;
-                if ((r = launch<R, PK_KIN, -1>(c, gk, q, PC_KIN))) return r;
+                if ((r = launch<R, PK_KINS, -1>(c, gk, q, PC_KIN))) return r;
                 ysrcm[g.sg][k + 1] = 1;
                 g.merged_in = true;
             } else if (int r = kinetic_block<R>(c, W, KM_PHASE, kcoef(ia + 1), nullptr)) {

@@ -732,7 +732,9 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
 // ---------------------------------------------------------------------
 // pass arguments
 // ---------------------------------------------------------------------
-enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3 };
+// PK_KINS is the kinetic pass of a merged step boundary (PF_KSPLIT): a kernel
+// of its own, so that its second transform's registers stay out of PK_KIN.
+enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
     PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
@@ -751,7 +753,7 @@ enum PassFlags : unsigned {
     PF_SRC = 1u << 13,       // u += sum_j src_j (w_j - E_j) y + src_norm y (y from `save`), w = -grad V
     PF_KTAB = 1u << 14,      // kinetic factor read from `ktab` (tabulated e^{-i c K}, four values per thread)
     // merged step boundary (2-D length gauge): K(a_last) | record | Mask | K(a_0)
-    PF_KSPLIT = 1u << 15,    // kinetic pass: after the factor and inverse, [store to `save`], [this axis's
+    PF_KSPLIT = 1u << 15,    // PK_KINS pass: after the factor and inverse, [store to `save`], [this axis's
                              // mask], forward, this axis's factor from `atab`, inverse
     PF_PREKX = 1u << 16,     // axis-0 pass: after the first transform, [axis-0 mask], forward, axis-0
                              // factor from `atab`, inverse, then the r-space operators
@@ -994,6 +996,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     __shared__ double red[16 * QND_NOBS];
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
     constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D);
+    constexpr bool KINK = (KIND == PK_KIN || KIND == PK_KINS);
 
     // workgroups after the nblk line tiles finalize the previous pass's
     // observable record (one per instance) and do no line work, so the
@@ -1114,7 +1117,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         // requested here so that the load overlaps the transform
         double fv[QND_MAXD];
         {
-            const bool use_a = (KIND == PK_KIN) || a.kmode == KM_VELOCITY;
+            const bool use_a = KINK || a.kmode == KM_VELOCITY;
             if (a.B.fld) {
                 const double *src = a.B.fld + (L / a.B.lpi) * a.B.fld_stride;
 #pragma unroll
@@ -1128,7 +1131,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         // tabulated kinetic factors of this thread's elements (contiguous
         // lines of the last axis), requested before the transform
         typename TwT<R>::T ktv[VPT];
-        if constexpr (KIND == PK_KIN && VPT == 4) {
+        if constexpr (KINK && VPT == 4) {
             if (fl & PF_KTAB) {
                 const int64_t e0 = (L % a.B.lpi) * n + t;
 #pragma unroll
@@ -1141,12 +1144,12 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         // thread requested before the transform, else read where used
         typename TwT<R>::T atv[VPT];
         double amv[VPT];
-        const bool split = (KIND == PK_KIN && (fl & PF_KSPLIT)) || (KIND == PK_POT && (fl & PF_PREKX));
+        const bool split = KIND == PK_KINS || (KIND == PK_POT && (fl & PF_PREKX));
         if constexpr (VPT == 4) {
             if (split) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) atv[m] = a.atab[P.off[ax] + t + m * T];
-                if (KIND == PK_KIN && (fl & PF_MASK) && P.mask_on) {
+                if (KIND == PK_KINS && (fl & PF_MASK) && P.mask_on) {
 #pragma unroll
                     for (int m = 0; m < VPT; ++m) amv[m] = P.maskt[P.off[ax] + t + m * T];
                 }
@@ -1200,7 +1203,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
         }
 
-        if (KIND == PK_KIN) {
+        if (KINK) {
             // all axes in k-space: the kinetic factor of this pass's axes
             // (reference order of the sums).  exp(-i c sum_j K_j) is a product
             // of commuting per-axis factors, so in 3-D and above each pass of
@@ -1216,7 +1219,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             }
             const double Aax = fv[ax];
             // one loop per mode (mode tests outside the loop over the points)
-            if (KIND == PK_KIN && VPT == 4 && (fl & PF_KTAB)) {
+            if (KINK && VPT == 4 && (fl & PF_KTAB)) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(ktv[m], cs < 0.0));
             } else if (a.kmode == KM_IMAG) {
@@ -1416,7 +1419,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         }
 
         // ---- second transform ----
-        if (KIND == PK_KIN || (fl & PF_SECOND)) {
+        if (KINK || (fl & PF_SECOND)) {
             transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
             const R s2 = (R)a.scale2;
             if (KIND == PK_POT1D && (fl & PF_KPOST)) {
@@ -1435,7 +1438,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                 for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], s2);
             }
         }
-        if (KIND == PK_KIN && (fl & PF_KSPLIT)) {
+        if (KIND == PK_KINS) {
             // merged step boundary: the state after the step's last kinetic
             // factor is the record's state (saved for the observable pass);
             // then this axis's part of the owed mask and of the next step's
