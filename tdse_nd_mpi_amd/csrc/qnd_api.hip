@@ -270,9 +270,13 @@ template <typename R> int blocks_per_cu(qnd_ctx *c, int kind, int vpt, int threa
 // Launch geometry of a pass along `axis` over nb consecutive slots (a batch
 // of instances: an extra, untransformed leading axis).  `kind` is the pass
 // kernel whose occupancy sizes strided tiles (-1: the potential pass on axis
-// 0 of a grid of dim >= 2, else the plain pass).
-Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1)
+// 0 of a grid of dim >= 2, else the plain pass).  `tuned` false ignores the
+// tuning overrides, which are chosen for the passes of the run's own slots
+// (the batched observable pass over saved record states takes the
+// automatic geometry).
+Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1, bool tuned = true)
 {
+    const int vpt_ov = tuned ? c->vpt_override : 0;
     Geo g{};
     g.axis = axis;
     g.n = (int)c->nr[axis];
@@ -290,7 +294,7 @@ Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1)
         g.log2n = ilog2(g.n);
         g.vpt = 8;
         const bool few_waves = g.nlines * g.n / 8 / 64 < 4096;
-        if ((c->vpt_override == 4 || (c->vpt_override == 0 && few_waves)) && g.n / 4 <= QND_PASS_THREADS)
+        if ((vpt_ov == 4 || (vpt_ov == 0 && few_waves)) && g.n / 4 <= QND_PASS_THREADS)
             g.vpt = 4;
     }
     g.T = g.n / g.vpt;
@@ -310,7 +314,7 @@ Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1)
         if (tw > twmaxseg) tw = twmaxseg;
         while (tw > 1 && (g.S % tw) != 0) tw /= 2;
     }
-    const int ov = c->tw_override[g.S == 1 ? 0 : 1];
+    const int ov = tuned ? c->tw_override[g.S == 1 ? 0 : 1] : 0;
     if (ov > 0) tw = pow2floor(ov);
     int thrcap = QND_PASS_THREADS / g.T;
     if (tw > thrcap) tw = thrcap;
@@ -356,20 +360,6 @@ Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1)
     g.grid = (int)(g.nlines / tw);
     g.ldsStride = g.n + (tw > 1 ? 1 : 0);
     g.lds = 2 * (size_t)tw * g.ldsStride * c->esz;
-    return g;
-}
-
-// The automatic geometry, ignoring the tuning overrides (which are chosen
-// for the passes of one state): for the batched observable pass over saved
-// record states.
-Geo geometry_auto(qnd_ctx *c, int axis, int nb)
-{
-    const int vo = c->vpt_override, t0 = c->tw_override[0], t1 = c->tw_override[1];
-    c->vpt_override = c->tw_override[0] = c->tw_override[1] = 0;
-    const Geo g = geometry(c, axis, nb);
-    c->vpt_override = vo;
-    c->tw_override[0] = t0;
-    c->tw_override[1] = t1;
     return g;
 }
 
@@ -1095,7 +1085,7 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
                 HIPCK(c, hipMalloc(&c->d_side, (size_t)side_slots * sbytes));
                 c->side_cap = (size_t)side_slots * sbytes;
             }
-            gside = geometry_auto(c, 0, side_slots);
+            gside = geometry(c, 0, side_slots, -1, false);
             if ((rc = ensure_part(c, (size_t)std::max(g0.grid, gside.grid) * QND_NOBS))) return rc;
         }
     }
@@ -1106,7 +1096,7 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
     // ring (a batch of side_n instances), then one finalize block per record
     auto flush_side = [&]() -> int {
         if (side_n == 0) return 0;
-        const Geo gs = side_n == side_slots ? gside : geometry_auto(c, 0, side_n);
+        const Geo gs = side_n == side_slots ? gside : geometry(c, 0, side_n, -1, false);
         PassArgs<R> a = base_args<R>(c, side, gs);
         a.scale1 = inv_n0;
         a.flags = PF_OBS | PF_MASK | PF_NOSTORE;
@@ -1242,7 +1232,7 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
                         // axis-1 parts of the mask and of K(a_0)
                         const Geo gk = geometry(c, 1, 1, PK_KIN);
                         PassArgs<R> k = base_args<R>(c, psi, gk);
-                        k.flags = PF_KSPLIT | (c->P.mask_on ? PF_MASK : 0u);
+                        k.flags = c->P.mask_on ? PF_MASK : 0u;
                         k.kmode = KM_PHASE;
                         k.ck_mid = ck;
                         k.scale2 = 1.0 / (double)c->nr[1];
@@ -1527,7 +1517,7 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
                 // merged boundary: K(a_last), save y_{k+1} (axis-0 Fourier), axis-1 parts
                 const Geo gk = geometry(c, 1, 1, PK_KIN);
                 PassArgs<R> q = base_args<R>(c, W, gk);
-                q.flags = PF_KSPLIT | PF_SAVE | (c->P.mask_on ? PF_MASK : 0u);
+                q.flags = PF_SAVE | (c->P.mask_on ? PF_MASK : 0u);
                 q.kmode = KM_PHASE;
                 q.ck_mid = kcoef(s);
                 q.scale2 = 1.0 / (double)c->nr[1];

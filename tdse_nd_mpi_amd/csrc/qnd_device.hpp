@@ -732,8 +732,11 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
 // ---------------------------------------------------------------------
 // pass arguments
 // ---------------------------------------------------------------------
-// PK_KINS is the kinetic pass of a merged step boundary (PF_KSPLIT): a kernel
-// of its own, so that its second transform's registers stay out of PK_KIN.
+// PK_KINS is the kinetic pass of a merged step boundary: after the factor and
+// the inverse transform it [stores the state to `save` (PF_SAVE)], [applies
+// its axis's mask (PF_MASK)], and runs a forward transform, its axis's factor
+// from `atab` and the inverse.  A kernel of its own, so that the second
+// transform's registers stay out of PK_KIN.
 enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
@@ -752,9 +755,8 @@ enum PassFlags : unsigned {
     PF_GRAD = 1u << 12,      // partials of x_j Im(conj(u) phi) and dV/da Im(conj(u) phi), phi from `save`
     PF_SRC = 1u << 13,       // u += sum_j src_j (w_j - E_j) y + src_norm y (y from `save`), w = -grad V
     PF_KTAB = 1u << 14,      // kinetic factor read from `ktab` (tabulated e^{-i c K}, four values per thread)
-    // merged step boundary (2-D length gauge): K(a_last) | record | Mask | K(a_0)
-    PF_KSPLIT = 1u << 15,    // PK_KINS pass: after the factor and inverse, [store to `save`], [this axis's
-                             // mask], forward, this axis's factor from `atab`, inverse
+    // merged step boundary (2-D length gauge): K(a_last) | record | Mask | K(a_0);
+    // its kinetic pass is the kernel kind PK_KINS
     PF_PREKX = 1u << 16,     // axis-0 pass: after the first transform, [axis-0 mask], forward, axis-0
                              // factor from `atab`, inverse, then the r-space operators
     PF_NOSTORE = 1u << 17,   // no store (observable-only pass)
@@ -806,7 +808,7 @@ template <typename R> struct PassArgs {
     // one instance (row-major), when tabulated (PF_KTAB)
     const typename TwT<R>::T *ktab;
     // per-axis kinetic factors e^{-i c K_j} of one coefficient (the step's
-    // first, a_0), Sigma n_j entries at the per-axis offsets (PF_KSPLIT, PF_PREKX)
+    // first, a_0), Sigma n_j entries at the per-axis offsets (PK_KINS, PF_PREKX)
     const typename TwT<R>::T *atab;
     // adjoint passes: forward states saved in position space, source weights
     C<R> *save;
