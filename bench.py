@@ -395,6 +395,26 @@ def measured_traffic(kernel="void qnd::k_pass<float, 2, 1, 4>"):
             "source": os.path.relpath(TRAFFIC_PROFILE, ROOT) + " (default launch geometry)"}
 
 
+def copy_bandwidth(reps=10):
+    """Measured device-copy bandwidth (GB/s, read + write) of a 1 GiB FP32
+    copy, timed with events on torch's stream, where the copies run."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def spectrum(a, dt):
     w = np.hanning(len(a))
     return np.abs(np.fft.rfft((a - a.mean()) * w)) ** 2
@@ -552,6 +572,7 @@ def main():
                        "traffic": (tr["bytes"] if (tr := measured_traffic()) else None),
                        "traffic_detail": tr,
                        "avg_launch_us": avg_ms * 1e3, "bytes_per_launch": bytes_launch,
+                       "copy_GBs": (cbw := copy_bandwidth()), "frac_of_copy": ach / cbw,
                        "launches_per_step": {names[i]: float(cnt[i]) / args.profile_steps for i in range(4)},
                        "class_ms_per_step": {names[i]: float(ms[i]) / args.profile_steps for i in range(4)}}
     if not args.skip_cpu:
