@@ -1403,17 +1403,33 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
         HIPCK(c, hipMalloc(&c->d_adj_save, sbytes * 2 * nsave));
         c->adj_save_cap = 2 * nsave;
     }
+    // partials of the sweep's passes, their sums; the boundary source pass's
+    // partials and sums; the per-step source weights
+    const int64_t lpi0 = c->N / c->nr[0];
     const size_t npart = 2 * nsave * nwg * QND_NOBS, ngsum = (size_t)nsteps * per_step * QND_NOBS;
-    if (c->adj_part_cap < npart + ngsum) {
+    const size_t npartT = (size_t)K * lpi0 * QND_NOBS, ngsumT = (size_t)nsteps * QND_NOBS, nsrc = (size_t)nsteps * d;
+    const size_t nadj = npart + ngsum + npartT + ngsumT + nsrc;
+    if (c->adj_part_cap < nadj) {
         if (c->d_adj_part) HIPCK(c, hipFree(c->d_adj_part));
         c->d_adj_part = nullptr;
-        HIPCK(c, hipMalloc(&c->d_adj_part, (npart + ngsum) * sizeof(double)));
-        c->adj_part_cap = npart + ngsum;
+        HIPCK(c, hipMalloc(&c->d_adj_part, nadj * sizeof(double)));
+        c->adj_part_cap = nadj;
     }
     double *gsum_dev = c->d_adj_part + npart;
+    double *partT = gsum_dev + ngsum, *gsumT_dev = partT + npartT, *src_dev = gsumT_dev + ngsumT;
+    {
+        // source weights of the records, dL/da_{n,j} dNr / n0, per step
+        std::vector<double> sw(nsrc);
+        for (size_t q = 0; q < nsrc; ++q) sw[q] = dL_da[q] * dNr * inv_n0;
+        HIPCK(c, hipMemcpy(src_dev, sw.data(), nsrc * sizeof(double), hipMemcpyHostToDevice));
+    }
     C<R> *lam = reinterpret_cast<C<R> *>(c->d_scr[0]);
     C<R> *W = reinterpret_cast<C<R> *>(c->d_scr[1]);
+    // a segment's saves: the record states y_k (or, at merged boundaries, their
+    // axis-0 Fourier representation) in K contiguous slots, then phi_{k,i}
     auto saves_of = [&](int64_t sg) { return reinterpret_cast<C<R> *>(c->d_adj_save) + (sg & 1) * nsave * c->N; };
+    auto yslot = [&](C<R> *saves, int64_t k) { return saves + (size_t)k * c->N; };
+    auto pslot = [&](C<R> *saves, int64_t k, int ia) { return saves + (size_t)(K + k * s + ia) * c->N; };
     auto part_of = [&](int64_t sg) { return c->d_adj_part + (sg & 1) * nsave * nwg * QND_NOBS; };
     std::vector<hipEvent_t> ev_rec(nseg, nullptr), ev_adj(nseg, nullptr);
     for (int64_t k = 0; k < nseg; ++k) {
@@ -1496,7 +1512,7 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
         if (!merged_in) {
             PassArgs<R> b = pot(W);
             b.flags |= PF_SAVE | (g.st.pend_mask ? PF_MASK : 0u);
-            b.save = g.saves + (k * per_step) * c->N;
+            b.save = yslot(g.saves, k);
             if (int r = launch<R, PK_POT, +1>(c, g0, b, PC_POT)) return r;
             if (int r = kinetic_block<R>(c, W, KM_PHASE, kcoef(0), nullptr)) return r;
         }
@@ -1507,7 +1523,7 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
             p.vmode = VM_LENGTH;
             p.cv1 = as[2 * ia] + as[2 * ia + 1];
             for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
-            p.save = g.saves + (k * per_step + 1 + ia) * c->N;
+            p.save = pslot(g.saves, k, ia);
             if (ia == 0 && merged_in) {
                 p.flags |= PF_PREKX | (g.st.pend_mask ? PF_MASK : 0u);
                 p.atab = atab;
@@ -1526,7 +1542,7 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
                 if (r) return r;
                 if (q.ktab) q.flags |= PF_KTAB;
                 q.atab = atab;
-                q.save = g.saves + ((k + 1) * per_step) * c->N;
+                q.save = yslot(g.saves, k + 1);
                 if ((r = launch<R, PK_KINS, -1>(c, gk, q, PC_KIN))) return r;
                 ysrcm[g.sg][k + 1] = 1;
                 g.merged_in = true;
@@ -1542,35 +1558,97 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
         HIPCK(c, hipEventRecord(ev_rec[g.sg], c->stream));
         return 0;
     };
+    // The sweep crosses a merged boundary (records saved in the axis-0 Fourier
+    // representation, 2-D) in two passes, the adjoints of the forward's two
+    // boundary passes: the adjoint B_k^dagger = F0 Mask F0^-1 + T_k of the
+    // record's pass is linear in the costate plus the source
+    // T_k = F0 Mask S_k(y_k), which does not depend on the costate.  Before the
+    // segment's steps, one batched pass turns the saved states of its merged
+    // steps into T_k in place (PF_ZERO: source alone), with the explicit
+    // soft-core partials of those records.  Then
+    //   pass A (axis 0): P_0^dagger, the conjugated axis-0 part of K(a_0), the
+    //                    axis-0 mask (PF_POSTKX);
+    //   pass B (axis 1): the conjugated axis-1 part of K(a_0), the axis-1 mask,
+    //                    + T_k, then K(a_last)^dagger of step k-1 (PK_KINR).
     auto sweep_begin = [&](Seg &g) -> int {
         HIPCK(c, hipStreamWaitEvent(c->stream, ev_rec[g.sg], 0));
         g.k = g.len - 1;
+        int64_t k1 = 1;
+        while (k1 < g.len && !ysrcm[g.sg][k1]) ++k1;
+        const int64_t ni = g.len - k1;  // merged steps k1 .. len-1
+        for (int64_t k = k1; k < g.len; ++k)
+            if (!ysrcm[g.sg][k] || masked[g.sg][k] != masked[g.sg][k1]) return fail(c, QND_EINVAL, "adjoint: mixed boundaries");
+        if (ni > 0) {
+            const Geo gt = geometry(c, 0, (int)ni);
+            PassArgs<R> a = base_args<R>(c, yslot(g.saves, k1), gt);
+            a.scale1 = inv_n0;
+            a.flags = PF_SRC | PF_SRCM | PF_ZERO | PF_SECOND | (masked[g.sg][k1] ? PF_MASK : 0u);
+            a.save = yslot(g.saves, k1);
+            a.partials = partT;
+            a.npart = QND_NOBS;
+            a.B.nb = (int)ni;
+            a.B.fld = c->d_Eo + (g.nA + k1) * d;
+            a.B.fld_stride = d;
+            a.B.srcr = src_dev + (g.nA + k1) * d;
+            a.B.srcr_stride = d;
+            if (int r = launch<R, PK_POT, +1>(c, gt, a, PC_POT)) return r;
+            ProfScope ps(c, PC_SMALL);
+            hipLaunchKernelGGL(k_sum_blocks, dim3(((int)ni * QND_NOBS + 3) / 4), dim3(256), 0, c->stream,
+                               (const double *)partT, (int)ni, (int)(gt.grid / ni), QND_NOBS,
+                               gsumT_dev + (g.nA + k1) * QND_NOBS);
+            HIPCK(c, hipGetLastError());
+        }
         return 0;
     };
     auto sweep_step = [&](Seg &g) -> int {
         const int64_t k = g.k--, n = g.nA + k;
         const double *Et = &c->Et[(size_t)n * s * d];
         const double *Eo = &c->Eobs[(size_t)n * d];
-        if (int r = kin_adj(lam, kcoef(s))) return r;
+        // K(a_last)^dagger of this step, unless the merged boundary after it
+        // (the next step's pass B) applied it
+        if (k == g.len - 1 || !ysrcm[g.sg][k + 1])
+            if (int r = kin_adj(lam, kcoef(s))) return r;
+        const bool merged = ysrcm[g.sg][k] != 0;
         for (int ia = s - 1; ia >= 0; --ia) {
             PassArgs<R> p = pot(lam);
             p.flags |= PF_V1 | PF_CONJ | PF_GRAD;
             p.vmode = VM_LENGTH;
             p.cv1 = as[2 * ia] + as[2 * ia + 1];
             for (int j = 0; j < d; ++j) p.Ev[j] = Et[ia * d + j];
-            p.save = g.saves + (k * per_step + 1 + ia) * c->N;
+            p.save = pslot(g.saves, k, ia);
             p.partials = g.part + (size_t)(k * per_step + 1 + ia) * nwg * QND_NOBS;
             p.npart = QND_NOBS;
+            if (ia == 0 && merged) {
+                p.flags |= PF_POSTKX | (masked[g.sg][k] ? PF_MASK : 0u);
+                p.atab = atab;
+            }
             if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
-            if (int r = kin_adj(lam, kcoef(ia))) return r;
+            if (ia > 0 || !merged)
+                if (int r = kin_adj(lam, kcoef(ia))) return r;
+        }
+        if (merged) {
+            // pass B: axis-1 parts of K(a_0) and of the mask, + T_k, K(a_last)^dagger of step k-1
+            const Geo gk = geometry(c, 1, 1, PK_KIN);
+            PassArgs<R> q = base_args<R>(c, lam, gk);
+            q.flags = PF_CONJ | (masked[g.sg][k] ? PF_MASK : 0u);
+            q.kmode = KM_PHASE;
+            q.ck_mid = kcoef(s);
+            q.scale2 = 1.0 / (double)c->nr[1];
+            int r = 0;
+            q.ktab = ktab_for<R>(c, q.ck_mid, gk, &r);
+            if (r) return r;
+            if (q.ktab) q.flags |= PF_KTAB;
+            q.atab = atab;
+            q.save = yslot(g.saves, k);
+            return launch<R, PK_KINR, -1>(c, gk, q, PC_KIN);
         }
         PassArgs<R> b = pot(lam);
-        b.flags |= PF_SRC | (masked[g.sg][k] ? PF_MASK : 0u) | (ysrcm[g.sg][k] ? PF_SRCM : 0u);
+        b.flags |= PF_SRC | (masked[g.sg][k] ? PF_MASK : 0u);
         for (int j = 0; j < d; ++j) {
             b.src[j] = dL_da[n * d + j] * dNr * inv_n0;
             b.Ev[j] = Eo[j];
         }
-        b.save = g.saves + (k * per_step) * c->N;
+        b.save = yslot(g.saves, k);
         b.partials = g.part + (size_t)(k * per_step) * nwg * QND_NOBS;
         b.npart = QND_NOBS;
         return launch<R, PK_POT, +1>(c, g0, b, PC_POT);
@@ -1622,15 +1700,21 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
         if (rec && (rc = rec_end(rc_seg))) return cleanup(), rc;
         if ((rc = sweep_end(sw))) return cleanup(), rc;
     }
-    std::vector<double> gsum(ngsum);
+    std::vector<double> gsum(ngsum), gsumT(ngsumT);
     HIPCK(c, hipMemcpyAsync(gsum.data(), gsum_dev, ngsum * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipMemcpyAsync(gsumT.data(), gsumT_dev, ngsumT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream));
     HIPCK(c, hipStreamSynchronize(c->stream2));
     cleanup();
+    std::vector<char> merged_rec(nsteps, 0);
+    for (int64_t sg = 0; sg < nseg; ++sg)
+        for (size_t k = 0; k < ysrcm[sg].size(); ++k) merged_rec[sg * K + k] = ysrcm[sg][k];
     for (int64_t k = 0; k < nsteps * s * d; ++k) dL_dEt[k] = 0.0;
     double dsoft = 0.0;
     for (int64_t n = 0; n < nsteps; ++n) {
-        const double *bsum = &gsum[(size_t)(n * per_step) * QND_NOBS];
+        // the record's explicit partials: from its boundary pass, or from the
+        // source pass of a merged boundary
+        const double *bsum = merged_rec[n] ? &gsumT[(size_t)n * QND_NOBS] : &gsum[(size_t)(n * per_step) * QND_NOBS];
         dsoft += n0d * bsum[0];  // explicit dependence of a_n on the soft-core length
         const double nrm = c->adj_obs[n * (d + 1) + d];
         for (int j = 0; j < d; ++j) dL_dEobs[n * d + j] = -dL_da[n * d + j] * nrm;

@@ -737,7 +737,11 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
 // its axis's mask (PF_MASK)], and runs a forward transform, its axis's factor
 // from `atab` and the inverse.  A kernel of its own, so that the second
 // transform's registers stay out of PK_KIN.
-enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4 };
+// PK_KINR is its adjoint in reverse order (the costate sweep): forward
+// transform, its axis's conjugated factor from `atab`, inverse, [its axis's
+// mask], plus the boundary's source term from `save` (axis-0 Fourier
+// representation), then the ordinary kinetic pass.
+enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
     PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
@@ -762,6 +766,9 @@ enum PassFlags : unsigned {
     PF_NOSTORE = 1u << 17,   // no store (observable-only pass)
     PF_SRCM = 1u << 18,      // with PF_SRC: `save` holds the record's state in the axis-0 Fourier
                              // representation (merged boundary); y = [Mask] F0^-1 save / n0
+    PF_ZERO = 1u << 19,      // with PF_SRC: the costate is zero (the pass forms the source term alone)
+    PF_POSTKX = 1u << 20,    // axis-0 pass: after the second transform, the conjugated axis-0 factor
+                             // from `atab`, inverse, [axis-0 mask], forward (adjoint of PF_PREKX)
 };
 enum KMode { KM_PHASE = 0, KM_VELOCITY = 1, KM_IMAG = 2 };
 enum VMode { VM_LENGTH = 0, VM_VELOCITY = 1, VM_IMAG = 2 };
@@ -787,6 +794,8 @@ struct Batch {
     const double *finE;      // E(t_n) of instance 0 for the record being finalized
     int64_t finE_stride;
     int64_t rec_stride;      // doubles between the records of two instances
+    const double *srcr;      // PF_SRC: per-instance source weights [dim] (nullptr: PassArgs::src)
+    int64_t srcr_stride;
 };
 
 template <typename R> struct PassArgs {
@@ -998,7 +1007,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     __shared__ double red[16 * QND_NOBS];
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
     constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D);
-    constexpr bool KINK = (KIND == PK_KIN || KIND == PK_KINS);
+    constexpr bool KINK = (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR);
 
     // workgroups after the nblk line tiles finalize the previous pass's
     // observable record (one per instance) and do no line work, so the
@@ -1146,12 +1155,12 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         // thread requested before the transform, else read where used
         typename TwT<R>::T atv[VPT];
         double amv[VPT];
-        const bool split = KIND == PK_KINS || (KIND == PK_POT && (fl & PF_PREKX));
+        const bool split = KIND == PK_KINS || KIND == PK_KINR || (KIND == PK_POT && (fl & (PF_PREKX | PF_POSTKX)));
         if constexpr (VPT == 4) {
             if (split) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) atv[m] = a.atab[P.off[ax] + t + m * T];
-                if (KIND == PK_KINS && (fl & PF_MASK) && P.mask_on) {
+                if ((KIND == PK_KINS || KIND == PK_KINR) && (fl & PF_MASK) && P.mask_on) {
 #pragma unroll
                     for (int m = 0; m < VPT; ++m) amv[m] = P.maskt[P.off[ax] + t + m * T];
                 }
@@ -1167,6 +1176,25 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                 const double K = .5 * p * p;
                 if (a.kmode == KM_IMAG) x[m] = scl<R>(x[m], real_exp<R>(-a.ck_pre * K));
                 else x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_pre * K));
+            }
+        }
+
+        if constexpr (KIND == PK_KINR) {
+            // adjoint of the merged boundary's kinetic pass, first half: this
+            // axis's part of K(a_0) conjugated, this axis's mask, then the
+            // boundary's source term (axis-0 Fourier representation)
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], (R)a.scale2);
+            transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(atab_at(m), cs < 0.0));
+            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            const bool mk = (fl & PF_MASK) && P.mask_on;
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                const C<R> src = a.save[lbase + (int64_t)(t + m * T) * estride];
+                const C<R> v = mk ? scl<R>(x[m], (R)kmask_at(m)) : x[m];
+                x[m] = add<R>(v, src);
             }
         }
 
@@ -1186,6 +1214,10 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                 const double f = mky ? msk_at(m) * mko * a.scale1 : a.scale1;
                 ysv[m] = mk<R>((R)((double)ysv[m].x * f), (R)((double)ysv[m].y * f));
             }
+        }
+        if (KIND == PK_POT && (fl & PF_ZERO)) {
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) x[m] = mk<R>((R)0, (R)0);
         }
 
         if (KIND == PK_POT && (fl & PF_PREKX)) {
@@ -1321,12 +1353,13 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                                 if (j < P.dim) obs[1 + j] += ((j == ax) ? xc[i] : xo[j]) * im;
                         } else {
                             double wsum = a.src_norm, dwsum = 0.0;
+                            const double *srow = a.B.srcr ? a.B.srcr + (L / a.B.lpi) * a.B.srcr_stride : a.src;
 #pragma unroll
                             for (int j = 0; j < QND_MAXD; ++j) {
                                 if (j < P.dim) {
                                     const double xj = (j == ax) ? xc[i] : xo[j];
-                                    wsum += a.src[j] * (-2.0 * xj * dVdu[i] - fv[j]);
-                                    dwsum += a.src[j] * (-2.0 * xj * dVdu_da);
+                                    wsum += srow[j] * (-2.0 * xj * dVdu[i] - fv[j]);
+                                    dwsum += srow[j] * (-2.0 * xj * dVdu_da);
                                 }
                             }
                             v[i] = mk<R>((R)((double)v[i].x + wsum * (double)f.x),
@@ -1363,7 +1396,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                         }
                     }
                 }
-                if ((fl & PF_MASK) && !(fl & PF_PREKX) && P.mask_on) {
+                if ((fl & PF_MASK) && !(fl & (PF_PREKX | PF_POSTKX)) && P.mask_on) {
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
                         const int m = m0 + i;
@@ -1439,6 +1472,20 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], s2);
             }
+        }
+        if (KIND == PK_POT && (fl & PF_POSTKX)) {
+            // adjoint of PF_PREKX: the conjugated axis-0 part of K(a_0), then
+            // the axis-0 mask, on the state just returned to axis-0 Fourier space
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(atab_at(m), cs < 0.0));
+            transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            const bool mk0 = (fl & PF_MASK) && P.mask_on;
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                const R f = mk0 ? (R)(msk_at(m) * a.scale1) : (R)a.scale1;
+                x[m] = scl<R>(x[m], f);
+            }
+            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
         }
         if (KIND == PK_KINS) {
             // merged step boundary: the state after the step's last kinetic
