@@ -270,6 +270,59 @@ def test_merged_boundary_matches_single_steps(gpu, prec):
         assert r1 < 1e-3 * scale and r1 < 2.0 * r2 + 1e-7, (r1, r2, scale)
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_full_size_512_matches_oracle(gpu, prec):
+    """The benchmark's grid and operators at full size: Ne_2D 512^2, DR 0.2,
+    BM4 length gauge with the 800 nm field and the outer-10 % mask, 6 steps in
+    one call (merged boundaries) from a smooth state, against the FP64 oracle."""
+    nr, dr, pot, dt, nt = (512, 512), 0.2, "Ne_2D", 0.05, 6
+    print(f"case full-size 512^2 prec={prec}", flush=True)
+    o = O.Oracle(nr, dr, pot, "BM4", dt)
+    o.set_mask(0.1)
+    E, A, Eobs, Aobs = field_tables(o, nt, 2, ellip=0.0)
+    psi0 = smooth_state(o, 21)[0]
+    names = ["dipole_r", "dipole_a", "norm"]
+    ref_psi, ref_obs = o.real_run(psi0, "length", E, A, Eobs, Aobs, names)
+    g = q.Grid(nr, dr, prec, n_slots=1)
+    g.set_potential(pot)
+    g.set_integrator("BM4", dt)
+    g.set_mask(0.1)
+    g.set_field(E, A, Eobs, Aobs)
+    g.upload(0, psi0)
+    got_obs = g.real_steps(0, "length", 0, nt, names)
+    got = g.download(0)
+    g.close()
+    if prec == "c128":
+        assert rel(got, ref_psi) < TOL128, rel(got, ref_psi)
+        assert np.allclose(got_obs, ref_obs, rtol=1e-11, atol=1e-12), np.abs(got_obs - ref_obs).max()
+    else:
+        assert rel(got, ref_psi) < TOL64, rel(got, ref_psi)
+        assert np.allclose(got_obs, ref_obs, rtol=TOL64, atol=TOL64 * np.abs(ref_obs).max())
+
+
+def test_unitarity_256_cube(gpu):
+    """A size-independent property at 3-D size: without CAP or mask the
+    complex64 real-time step is unitary, so the norm record stays constant
+    (H_3D 256^3, BM4 length gauge under a field, 8 steps)."""
+    n, dr, dt, nt = 256, 0.3, 0.05, 8
+    print("case unitarity 256^3 c64", flush=True)
+    x = -0.5 * n * dr + np.arange(n) * dr
+    gx = np.exp(-(x * x) / 8.0)
+    psi0 = (gx[:, None, None] * gx[None, :, None] * gx[None, None, :]).astype(np.complex128)
+    psi0 = psi0 * np.exp(0.2j * x)[None, None, :]
+    psi0 /= np.sqrt(np.sum(np.abs(psi0) ** 2) * dr ** 3)
+    from tdse_nd_mpi_amd import field as F
+    g = q.Grid((n, n, n), dr, "c64", n_slots=1)
+    g.set_potential("H_3D")
+    g.set_integrator("BM4", dt)
+    g.set_field(*F.tables([F.Mode(intensity=1e14, cycles=2)], "BM4", dt, nt + 2, 3))
+    g.upload(0, psi0)
+    rec = g.real_steps(0, "length", 0, nt, ["norm"])[:, 0]
+    g.close()
+    assert abs(rec[0] - 1.0) < 1e-6, rec[0]
+    assert np.abs(rec - rec[0]).max() < 1e-6, np.abs(rec - rec[0]).max()
+
+
 @pytest.mark.parametrize("gauge", ["length", "velocity"])
 def test_real_unfused_observables(gpu, gauge):
     nr, dr, pot = (32, 32), 0.4, "H_2D"
