@@ -430,6 +430,7 @@ int launch(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, int cls)
 template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R> &a)
 {
     if (c->dim == 1) return launch<R, PK_POT1D, +1>(c, g, a, PC_POT);
+    if (a.flags == (PF_V1 | PF_SECOND) && a.vmode == VM_LENGTH) return launch<R, PK_POTL, +1>(c, g, a, PC_POT);
     return launch<R, PK_POT, +1>(c, g, a, PC_POT);
 }
 

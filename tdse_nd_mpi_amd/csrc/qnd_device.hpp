@@ -741,7 +741,10 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
 // transform, its axis's conjugated factor from `atab`, inverse, [its axis's
 // mask], plus the boundary's source term from `save` (axis-0 Fourier
 // representation), then the ordinary kinetic pass.
-enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5 };
+// PK_POTL is the potential pass of a length-gauge stage (flags PF_V1 |
+// PF_SECOND only): PK_POT with its flags and potential mode known at compile
+// time, so that none of the other operators' code is in its kernel.
+enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5, PK_POTL = 6 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
     PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
@@ -1000,13 +1003,15 @@ __host__ __device__ inline unsigned pass_geo(int log2n, int log2T, int log2TW, i
 
 template <typename R, int KIND, int DIR1, int VPT>
 __global__ void __launch_bounds__(QND_PASS_THREADS)
-    k_pass(C<R> *data, const typename TwT<R>::T *tw, int64_t S, unsigned geo, unsigned fl, int nblk,
+    k_pass(C<R> *data, const typename TwT<R>::T *tw, int64_t S, unsigned geo, unsigned fl_arg, int nblk,
            const PassArgs<R> a)
 {
+    const unsigned fl = KIND == PK_POTL ? (unsigned)(PF_V1 | PF_SECOND) : fl_arg;
+    const int vmode = KIND == PK_POTL ? (int)VM_LENGTH : a.vmode;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[16 * QND_NOBS];
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
-    constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D);
+    constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL);
     constexpr bool KINK = (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR);
 
     // workgroups after the nblk line tiles finalize the previous pass's
@@ -1369,12 +1374,12 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                     }
                 }
                 if (fl & PF_V1) {
-                    if (a.vmode == VM_IMAG) {
+                    if (vmode == VM_IMAG) {
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
                             v[i] = scl<R>(v[i], real_exp<R>(-a.cv1 * V[i]));
                         }
-                    } else if (a.vmode == VM_LENGTH) {
+                    } else if (vmode == VM_LENGTH) {
                         // without a CAP the damping argument is the constant 0 (no exponential)
                         if (P.cap_on) {
 #pragma unroll
@@ -1419,7 +1424,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                     }
                 }
                 if (fl & PF_V2) {
-                    if (a.vmode == VM_IMAG) {
+                    if (vmode == VM_IMAG) {
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
                             v[i] = scl<R>(v[i], real_exp<R>(-a.cv2 * V[i]));
