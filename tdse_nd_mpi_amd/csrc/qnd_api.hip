@@ -625,7 +625,11 @@ int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A,
             a.B.finE_stride = fin->B.finE_stride;
             a.B.rec_stride = fin->B.rec_stride;
         }
-        if ((rc = launch<R, PK_KIN, -1>(c, g, a, PC_KIN))) return rc;
+        if (kmode == KM_PHASE && !(a.flags & PF_FINALIZE)) {
+            if ((rc = launch<R, PK_KINP, -1>(c, g, a, PC_KIN))) return rc;
+        } else if ((rc = launch<R, PK_KIN, -1>(c, g, a, PC_KIN))) {
+            return rc;
+        }
     }
     return 0;
 }

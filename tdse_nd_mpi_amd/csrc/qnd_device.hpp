@@ -744,7 +744,10 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
 // PK_POTL is the potential pass of a length-gauge stage (flags PF_V1 |
 // PF_SECOND only): PK_POT with its flags and potential mode known at compile
 // time, so that none of the other operators' code is in its kernel.
-enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5, PK_POTL = 6 };
+// PK_KINP is the kinetic pass of the phase mode (length gauge) that finalizes
+// no record: PK_KIN with its mode known at compile time.
+enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5, PK_POTL = 6,
+                PK_KINP = 7 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
     PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
@@ -1008,11 +1011,12 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
 {
     const unsigned fl = KIND == PK_POTL ? (unsigned)(PF_V1 | PF_SECOND) : fl_arg;
     const int vmode = KIND == PK_POTL ? (int)VM_LENGTH : a.vmode;
+    const int kmode = KIND == PK_KINP ? (int)KM_PHASE : a.kmode;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[16 * QND_NOBS];
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
     constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL);
-    constexpr bool KINK = (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR);
+    constexpr bool KINK = (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR || KIND == PK_KINP);
 
     // workgroups after the nblk line tiles finalize the previous pass's
     // observable record (one per instance) and do no line work, so the
@@ -1133,7 +1137,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         // requested here so that the load overlaps the transform
         double fv[QND_MAXD];
         {
-            const bool use_a = KINK || a.kmode == KM_VELOCITY;
+            const bool use_a = KINK || kmode == KM_VELOCITY;
             if (a.B.fld) {
                 const double *src = a.B.fld + (L / a.B.lpi) * a.B.fld_stride;
 #pragma unroll
@@ -1179,7 +1183,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             for (int m = 0; m < VPT; ++m) {
                 const double p = coord_p(P, ax, t + m * T);
                 const double K = .5 * p * p;
-                if (a.kmode == KM_IMAG) x[m] = scl<R>(x[m], real_exp<R>(-a.ck_pre * K));
+                if (kmode == KM_IMAG) x[m] = scl<R>(x[m], real_exp<R>(-a.ck_pre * K));
                 else x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_pre * K));
             }
         }
@@ -1261,14 +1265,14 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             if (KINK && VPT == 4 && (fl & PF_KTAB)) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(ktv[m], cs < 0.0));
-            } else if (a.kmode == KM_IMAG) {
+            } else if (kmode == KM_IMAG) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) {
                     const double p = coord_p(P, ax, t + m * T);
                     const double K = Ko + .5 * p * p;
                     x[m] = scl<R>(x[m], real_exp<R>(-a.ck_mid * K));
                 }
-            } else if (a.kmode == KM_VELOCITY) {
+            } else if (kmode == KM_VELOCITY) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) {
                     const double p = coord_p(P, ax, t + m * T);
@@ -1468,8 +1472,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                     const double p = coord_p(P, ax, t + m * T);
                     const double K = .5 * p * p;
                     const C<R> v = scl<R>(x[m], s2);
-                    if (a.kmode == KM_IMAG) x[m] = scl<R>(v, real_exp<R>(-a.ck_post * K));
-                    else if (a.kmode == KM_VELOCITY)
+                    if (kmode == KM_IMAG) x[m] = scl<R>(v, real_exp<R>(-a.ck_post * K));
+                    else if (kmode == KM_VELOCITY)
                         x[m] = fac_mul<R>(v, phase_factor<R>(0.0, cs * a.ck_post * (K + (p + .5 * fv[ax]) * fv[ax])));
                     else x[m] = fac_mul<R>(v, phase_factor<R>(0.0, cs * a.ck_post * K));
                 }
