@@ -431,6 +431,7 @@ template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R>
 {
     if (c->dim == 1) return launch<R, PK_POT1D, +1>(c, g, a, PC_POT);
     if (a.flags == (PF_V1 | PF_SECOND) && a.vmode == VM_LENGTH) return launch<R, PK_POTL, +1>(c, g, a, PC_POT);
+    if (a.flags == (PF_V1 | PF_V2 | PF_SECOND) && a.vmode == VM_IMAG) return launch<R, PK_POTI, +1>(c, g, a, PC_POT);
     return launch<R, PK_POT, +1>(c, g, a, PC_POT);
 }
 
@@ -627,6 +628,8 @@ int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A,
         }
         if (kmode == KM_PHASE && !(a.flags & PF_FINALIZE)) {
             if ((rc = launch<R, PK_KINP, -1>(c, g, a, PC_KIN))) return rc;
+        } else if (kmode == KM_IMAG && !(a.flags & PF_FINALIZE)) {
+            if ((rc = launch<R, PK_KINI, -1>(c, g, a, PC_KIN))) return rc;
         } else if ((rc = launch<R, PK_KIN, -1>(c, g, a, PC_KIN))) {
             return rc;
         }

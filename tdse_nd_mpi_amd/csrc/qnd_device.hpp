@@ -745,9 +745,11 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
 // PF_SECOND only): PK_POT with its flags and potential mode known at compile
 // time, so that none of the other operators' code is in its kernel.
 // PK_KINP is the kinetic pass of the phase mode (length gauge) that finalizes
-// no record: PK_KIN with its mode known at compile time.
+// no record: PK_KIN with its mode known at compile time.  PK_POTI and PK_KINI
+// are the same for imaginary time: the potential pass between two stages
+// (PF_V1 | PF_V2 | PF_SECOND) and the kinetic pass.
 enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5, PK_POTL = 6,
-                PK_KINP = 7 };
+                PK_KINP = 7, PK_POTI = 8, PK_KINI = 9 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
     PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
@@ -1009,14 +1011,17 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     k_pass(C<R> *data, const typename TwT<R>::T *tw, int64_t S, unsigned geo, unsigned fl_arg, int nblk,
            const PassArgs<R> a)
 {
-    const unsigned fl = KIND == PK_POTL ? (unsigned)(PF_V1 | PF_SECOND) : fl_arg;
-    const int vmode = KIND == PK_POTL ? (int)VM_LENGTH : a.vmode;
-    const int kmode = KIND == PK_KINP ? (int)KM_PHASE : a.kmode;
+    const unsigned fl = KIND == PK_POTL   ? (unsigned)(PF_V1 | PF_SECOND)
+                        : KIND == PK_POTI ? (unsigned)(PF_V1 | PF_V2 | PF_SECOND)
+                                          : fl_arg;
+    const int vmode = KIND == PK_POTL ? (int)VM_LENGTH : KIND == PK_POTI ? (int)VM_IMAG : a.vmode;
+    const int kmode = KIND == PK_KINP ? (int)KM_PHASE : KIND == PK_KINI ? (int)KM_IMAG : a.kmode;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[16 * QND_NOBS];
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
-    constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL);
-    constexpr bool KINK = (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR || KIND == PK_KINP);
+    constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL || KIND == PK_POTI);
+    constexpr bool KINK =
+        (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR || KIND == PK_KINP || KIND == PK_KINI);
 
     // workgroups after the nblk line tiles finalize the previous pass's
     // observable record (one per instance) and do no line work, so the
