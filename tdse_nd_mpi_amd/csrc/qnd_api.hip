@@ -432,6 +432,8 @@ template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R>
     if (c->dim == 1) return launch<R, PK_POT1D, +1>(c, g, a, PC_POT);
     if (a.flags == (PF_V1 | PF_SECOND) && a.vmode == VM_LENGTH) return launch<R, PK_POTL, +1>(c, g, a, PC_POT);
     if (a.flags == (PF_V1 | PF_V2 | PF_SECOND) && a.vmode == VM_IMAG) return launch<R, PK_POTI, +1>(c, g, a, PC_POT);
+    if ((a.flags & ~PF_MASK) == (PF_V1 | PF_SECOND | PF_PREKX) && a.vmode == VM_LENGTH)
+        return launch<R, PK_POTX, +1>(c, g, a, PC_POT);
     return launch<R, PK_POT, +1>(c, g, a, PC_POT);
 }
 

@@ -748,8 +748,10 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
 // no record: PK_KIN with its mode known at compile time.  PK_POTI and PK_KINI
 // are the same for imaginary time: the potential pass between two stages
 // (PF_V1 | PF_V2 | PF_SECOND) and the kinetic pass.
+// PK_POTX is the first potential pass after a merged boundary
+// (PF_V1 | PF_SECOND | PF_PREKX, [PF_MASK]), length gauge.
 enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5, PK_POTL = 6,
-                PK_KINP = 7, PK_POTI = 8, PK_KINI = 9 };
+                PK_KINP = 7, PK_POTI = 8, PK_KINI = 9, PK_POTX = 10 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
     PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
@@ -1013,13 +1015,17 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
 {
     const unsigned fl = KIND == PK_POTL   ? (unsigned)(PF_V1 | PF_SECOND)
                         : KIND == PK_POTI ? (unsigned)(PF_V1 | PF_V2 | PF_SECOND)
+                        : KIND == PK_POTX ? (unsigned)(PF_V1 | PF_SECOND | PF_PREKX) | (fl_arg & PF_MASK)
                                           : fl_arg;
-    const int vmode = KIND == PK_POTL ? (int)VM_LENGTH : KIND == PK_POTI ? (int)VM_IMAG : a.vmode;
-    const int kmode = KIND == PK_KINP ? (int)KM_PHASE : KIND == PK_KINI ? (int)KM_IMAG : a.kmode;
+    const int vmode = (KIND == PK_POTL || KIND == PK_POTX) ? (int)VM_LENGTH : KIND == PK_POTI ? (int)VM_IMAG : a.vmode;
+    const int kmode = (KIND == PK_KINP || KIND == PK_KINS || KIND == PK_KINR) ? (int)KM_PHASE
+                      : KIND == PK_KINI                                       ? (int)KM_IMAG
+                                                                              : a.kmode;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[16 * QND_NOBS];
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
-    constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL || KIND == PK_POTI);
+    constexpr bool POTK =
+        (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL || KIND == PK_POTI || KIND == PK_POTX);
     constexpr bool KINK =
         (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR || KIND == PK_KINP || KIND == PK_KINI);
 
@@ -1169,7 +1175,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         // thread requested before the transform, else read where used
         typename TwT<R>::T atv[VPT];
         double amv[VPT];
-        const bool split = KIND == PK_KINS || KIND == PK_KINR || (KIND == PK_POT && (fl & (PF_PREKX | PF_POSTKX)));
+        const bool split = KIND == PK_KINS || KIND == PK_KINR || KIND == PK_POTX ||
+                           (KIND == PK_POT && (fl & (PF_PREKX | PF_POSTKX)));
         if constexpr (VPT == 4) {
             if (split) {
 #pragma unroll
@@ -1234,7 +1241,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             for (int m = 0; m < VPT; ++m) x[m] = mk<R>((R)0, (R)0);
         }
 
-        if (KIND == PK_POT && (fl & PF_PREKX)) {
+        if ((KIND == PK_POT || KIND == PK_POTX) && (fl & PF_PREKX)) {
             // the step's first kinetic factor along axis 0, after the owed
             // axis-0 mask (the kinetic pass before applied the other axes'
             // parts; the factors of different axes commute with each other
