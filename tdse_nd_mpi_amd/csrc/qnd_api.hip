@@ -1114,7 +1114,7 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
         a.npart = QND_NOBS;
         a.B.nb = side_n;
         int r;
-        if ((r = launch<R, PK_POT, +1>(c, gs, a, PC_SMALL))) return r;
+        if ((r = launch<R, PK_POTO, +1>(c, gs, a, PC_SMALL))) return r;
         Finalize f;
         memset(&f, 0, sizeof f);
         f.part = c->d_part;

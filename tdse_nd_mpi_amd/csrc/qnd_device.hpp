@@ -749,9 +749,10 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
 // are the same for imaginary time: the potential pass between two stages
 // (PF_V1 | PF_V2 | PF_SECOND) and the kinetic pass.
 // PK_POTX is the first potential pass after a merged boundary
-// (PF_V1 | PF_SECOND | PF_PREKX, [PF_MASK]), length gauge.
+// (PF_V1 | PF_SECOND | PF_PREKX, [PF_MASK]), length gauge; PK_POTO the
+// batched record pass (PF_OBS | PF_NOSTORE, [PF_MASK]).
 enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5, PK_POTL = 6,
-                PK_KINP = 7, PK_POTI = 8, PK_KINI = 9, PK_POTX = 10 };
+                PK_KINP = 7, PK_POTI = 8, PK_KINI = 9, PK_POTX = 10, PK_POTO = 11 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
     PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
@@ -1016,6 +1017,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     const unsigned fl = KIND == PK_POTL   ? (unsigned)(PF_V1 | PF_SECOND)
                         : KIND == PK_POTI ? (unsigned)(PF_V1 | PF_V2 | PF_SECOND)
                         : KIND == PK_POTX ? (unsigned)(PF_V1 | PF_SECOND | PF_PREKX) | (fl_arg & PF_MASK)
+                        : KIND == PK_POTO ? (unsigned)(PF_OBS | PF_NOSTORE) | (fl_arg & PF_MASK)
                                           : fl_arg;
     const int vmode = (KIND == PK_POTL || KIND == PK_POTX) ? (int)VM_LENGTH : KIND == PK_POTI ? (int)VM_IMAG : a.vmode;
     const int kmode = (KIND == PK_KINP || KIND == PK_KINS || KIND == PK_KINR) ? (int)KM_PHASE
@@ -1024,8 +1026,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[16 * QND_NOBS];
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
-    constexpr bool POTK =
-        (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL || KIND == PK_POTI || KIND == PK_POTX);
+    constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL || KIND == PK_POTI ||
+                           KIND == PK_POTX || KIND == PK_POTO);
     constexpr bool KINK =
         (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR || KIND == PK_KINP || KIND == PK_KINI);
 
