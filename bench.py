@@ -368,7 +368,7 @@ def big3d_run(q, F, imag_steps, real_steps):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_passes_ne2d_512x512_c64.txt")
 
 
-def measured_traffic(kernel="void qnd::k_pass<float, 2, 1, 4>"):
+def measured_traffic(kernel="void qnd::k_pass<float, 6, 1, 4>"):
     """HBM bytes per launch of `kernel` from the committed counter profile
     (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, of `bench.py --traffic-probe`):
     each counter's KiB per dispatch scaled by its reading for the calibration
@@ -578,7 +578,7 @@ def main():
                                               np.linalg.norm(fp64["obs"][:, :2]))
     # roofline of the dominant kernel class (HIP events around its launches)
     cnt, ms = mixed["prof"]
-    names = ["axis0_potential_pass", "kinetic_row_pass", "plain_fft_pass", "reductions"]
+    names = ["axis0_potential_pass", "kinetic_row_pass", "plain_fft_pass", "reductions", "boundary_passes"]
     k = int(np.argmax(ms[:2]))
     avg_ms = ms[k] / max(cnt[k], 1)
     bytes_launch = 2 * N * 8
@@ -589,8 +589,8 @@ def main():
                        "traffic_detail": tr,
                        "avg_launch_us": avg_ms * 1e3, "bytes_per_launch": bytes_launch,
                        "copy_GBs": (cbw := copy_bandwidth()), "frac_of_copy": ach / cbw,
-                       "launches_per_step": {names[i]: float(cnt[i]) / args.profile_steps for i in range(4)},
-                       "class_ms_per_step": {names[i]: float(ms[i]) / args.profile_steps for i in range(4)}}
+                       "launches_per_step": {names[i]: float(cnt[i]) / args.profile_steps for i in range(5)},
+                       "class_ms_per_step": {names[i]: float(ms[i]) / args.profile_steps for i in range(5)}}
     if not args.skip_cpu:
         log("cpu baseline (FP64 oracle on the host cores) ...")
         done, t_cpu, threads, ref_obs = cpu_baseline(psi0, Et, At, Eo, Ao, args.cpu_budget)

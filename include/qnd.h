@@ -195,7 +195,8 @@ int qnd_synchronize(qnd_ctx *ctx);
 /* Per-kernel-class profiling: when enabled, every launch of a pass class is
  * bracketed by events; qnd_profile_read returns the count and total ms per
  * class (classes: 0 potential/axis-0 pass, 1 kinetic/last-axis pass,
- * 2 plain FFT pass, 3 small reduction kernels). */
+ * 2 plain FFT pass, 3 small reduction kernels and record passes, 4 the two
+ * passes of a merged step boundary). */
 int qnd_profile_enable(qnd_ctx *ctx, int enable);
 int qnd_profile_read(qnd_ctx *ctx, int64_t *count, double *ms, int nclass);
 

@@ -286,9 +286,9 @@ class Grid:
         self._ck(library().qnd_profile_enable(self.h, 1 if enable else 0))
 
     def profile_read(self):
-        cnt = np.zeros(4, dtype=np.int64)
-        ms = np.zeros(4)
-        self._ck(library().qnd_profile_read(self.h, cnt.ctypes.data_as(C.POINTER(C.c_int64)), _d(ms), 4))
+        cnt = np.zeros(5, dtype=np.int64)
+        ms = np.zeros(5)
+        self._ck(library().qnd_profile_read(self.h, cnt.ctypes.data_as(C.POINTER(C.c_int64)), _d(ms), 5))
         return cnt, ms
 
 

@@ -42,7 +42,7 @@ struct AxisPlan {
     void *tw = nullptr;  // device twiddle table, C<R>[n]
 };
 
-enum ProfClass { PC_POT = 0, PC_KIN = 1, PC_PLAIN = 2, PC_SMALL = 3, PC_N = 4 };
+enum ProfClass { PC_POT = 0, PC_KIN = 1, PC_PLAIN = 2, PC_SMALL = 3, PC_BND = 4, PC_N = 5 };
 
 }  // namespace
 
@@ -433,7 +433,7 @@ template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R>
     if (a.flags == (PF_V1 | PF_SECOND) && a.vmode == VM_LENGTH) return launch<R, PK_POTL, +1>(c, g, a, PC_POT);
     if (a.flags == (PF_V1 | PF_V2 | PF_SECOND) && a.vmode == VM_IMAG) return launch<R, PK_POTI, +1>(c, g, a, PC_POT);
     if ((a.flags & ~PF_MASK) == (PF_V1 | PF_SECOND | PF_PREKX) && a.vmode == VM_LENGTH)
-        return launch<R, PK_POTX, +1>(c, g, a, PC_POT);
+        return launch<R, PK_POTX, +1>(c, g, a, PC_BND);
     return launch<R, PK_POT, +1>(c, g, a, PC_POT);
 }
 
@@ -1256,7 +1256,7 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
                             k.save = side + (size_t)side_n * c->N;
                             ++side_n;
                         }
-                        if ((rc = launch<R, PK_KINS, -1>(c, gk, k, PC_KIN))) return rc;
+                        if ((rc = launch<R, PK_KINS, -1>(c, gk, k, PC_BND))) return rc;
                         merged_out = true;
                     } else if ((rc = kinetic_block<R>(c, psi, KM_PHASE, ck, nullptr, nullptr, nb))) {
                         return rc;
@@ -1537,8 +1537,10 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
             if (ia == 0 && merged_in) {
                 p.flags |= PF_PREKX | (g.st.pend_mask ? PF_MASK : 0u);
                 p.atab = atab;
+                if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
+            } else if (int r = launch<R, PK_POTS, +1>(c, g0, p, PC_POT)) {
+                return r;
             }
-            if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
             if (ia == s - 1 && k + 1 < g.len && d == 2) {
                 // merged boundary: K(a_last), save y_{k+1} (axis-0 Fourier), axis-1 parts
                 const Geo gk = geometry(c, 1, 1, PK_KIN);
@@ -1631,8 +1633,10 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
             if (ia == 0 && merged) {
                 p.flags |= PF_POSTKX | (masked[g.sg][k] ? PF_MASK : 0u);
                 p.atab = atab;
+                if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
+            } else if (int r = launch<R, PK_POTG, +1>(c, g0, p, PC_POT)) {
+                return r;
             }
-            if (int r = launch<R, PK_POT, +1>(c, g0, p, PC_POT)) return r;
             if (ia > 0 || !merged)
                 if (int r = kin_adj(lam, kcoef(ia))) return r;
         }
