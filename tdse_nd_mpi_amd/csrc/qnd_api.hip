@@ -432,6 +432,8 @@ template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R>
     if (c->dim == 1) return launch<R, PK_POT1D, +1>(c, g, a, PC_POT);
     if (a.flags == (PF_V1 | PF_SECOND) && a.vmode == VM_LENGTH) return launch<R, PK_POTL, +1>(c, g, a, PC_POT);
     if (a.flags == (PF_V1 | PF_V2 | PF_SECOND) && a.vmode == VM_IMAG) return launch<R, PK_POTI, +1>(c, g, a, PC_POT);
+    if (a.flags == (PF_V1 | PF_V2 | PF_SECOND) && a.vmode == VM_VELOCITY)
+        return launch<R, PK_POTV, +1>(c, g, a, PC_POT);
     if ((a.flags & ~PF_MASK) == (PF_V1 | PF_SECOND | PF_PREKX) && a.vmode == VM_LENGTH)
         return launch<R, PK_POTX, +1>(c, g, a, PC_BND);
     return launch<R, PK_POT, +1>(c, g, a, PC_POT);
@@ -632,6 +634,8 @@ int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A,
             if ((rc = launch<R, PK_KINP, -1>(c, g, a, PC_KIN))) return rc;
         } else if (kmode == KM_IMAG && !(a.flags & PF_FINALIZE)) {
             if ((rc = launch<R, PK_KINI, -1>(c, g, a, PC_KIN))) return rc;
+        } else if (kmode == KM_VELOCITY && !(a.flags & PF_FINALIZE)) {
+            if ((rc = launch<R, PK_KINV, -1>(c, g, a, PC_KIN))) return rc;
         } else if ((rc = launch<R, PK_KIN, -1>(c, g, a, PC_KIN))) {
             return rc;
         }

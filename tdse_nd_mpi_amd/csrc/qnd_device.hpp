@@ -754,8 +754,11 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
 // PK_POTS and PK_POTG are the adjoint's length-gauge stage passes: the
 // recompute's (PF_V1 | PF_SAVE | PF_SECOND) and the costate sweep's
 // (PF_V1 | PF_CONJ | PF_GRAD | PF_SECOND).
+// PK_POTV and PK_KINV are the velocity gauge's stage potential pass
+// (PF_V1 | PF_V2 | PF_SECOND) and kinetic pass.
 enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5, PK_POTL = 6,
-                PK_KINP = 7, PK_POTI = 8, PK_KINI = 9, PK_POTX = 10, PK_POTO = 11, PK_POTS = 12, PK_POTG = 13 };
+                PK_KINP = 7, PK_POTI = 8, PK_KINI = 9, PK_POTX = 10, PK_POTO = 11, PK_POTS = 12, PK_POTG = 13,
+                PK_POTV = 14, PK_KINV = 15 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
     PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
@@ -1018,7 +1021,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
            const PassArgs<R> a)
 {
     const unsigned fl = KIND == PK_POTL   ? (unsigned)(PF_V1 | PF_SECOND)
-                        : KIND == PK_POTI ? (unsigned)(PF_V1 | PF_V2 | PF_SECOND)
+                        : (KIND == PK_POTI || KIND == PK_POTV) ? (unsigned)(PF_V1 | PF_V2 | PF_SECOND)
                         : KIND == PK_POTX ? (unsigned)(PF_V1 | PF_SECOND | PF_PREKX) | (fl_arg & PF_MASK)
                         : KIND == PK_POTO ? (unsigned)(PF_OBS | PF_NOSTORE) | (fl_arg & PF_MASK)
                         : KIND == PK_POTS ? (unsigned)(PF_V1 | PF_SAVE | PF_SECOND)
@@ -1026,17 +1029,20 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                                           : fl_arg;
     const int vmode = (KIND == PK_POTL || KIND == PK_POTX || KIND == PK_POTS || KIND == PK_POTG) ? (int)VM_LENGTH
                       : KIND == PK_POTI                                                        ? (int)VM_IMAG
+                      : KIND == PK_POTV                                                        ? (int)VM_VELOCITY
                                                                                                : a.vmode;
     const int kmode = (KIND == PK_KINP || KIND == PK_KINS || KIND == PK_KINR) ? (int)KM_PHASE
                       : KIND == PK_KINI                                       ? (int)KM_IMAG
+                      : KIND == PK_KINV                                       ? (int)KM_VELOCITY
                                                                               : a.kmode;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[16 * QND_NOBS];
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
     constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL || KIND == PK_POTI ||
-                           KIND == PK_POTX || KIND == PK_POTO || KIND == PK_POTS || KIND == PK_POTG);
-    constexpr bool KINK =
-        (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR || KIND == PK_KINP || KIND == PK_KINI);
+                           KIND == PK_POTX || KIND == PK_POTO || KIND == PK_POTS || KIND == PK_POTG ||
+                           KIND == PK_POTV);
+    constexpr bool KINK = (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR || KIND == PK_KINP ||
+                           KIND == PK_KINI || KIND == PK_KINV);
 
     // workgroups after the nblk line tiles finalize the previous pass's
     // observable record (one per instance) and do no line work, so the
