@@ -429,7 +429,11 @@ int launch(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, int cls)
 // axis-0 pass with r-space operators (IFFT first)
 template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R> &a)
 {
-    if (c->dim == 1) return launch<R, PK_POT1D, +1>(c, g, a, PC_POT);
+    if (c->dim == 1) {
+        if (a.flags == (PF_V1 | PF_V2 | PF_SECOND | PF_KPOST) && a.vmode == VM_IMAG && a.kmode == KM_IMAG)
+            return launch<R, PK_P1DI, +1>(c, g, a, PC_POT);
+        return launch<R, PK_POT1D, +1>(c, g, a, PC_POT);
+    }
     if (a.flags == (PF_V1 | PF_SECOND) && a.vmode == VM_LENGTH) return launch<R, PK_POTL, +1>(c, g, a, PC_POT);
     if (a.flags == (PF_V1 | PF_V2 | PF_SECOND) && a.vmode == VM_IMAG) return launch<R, PK_POTI, +1>(c, g, a, PC_POT);
     if (a.flags == (PF_V1 | PF_V2 | PF_SECOND) && a.vmode == VM_VELOCITY)

@@ -756,9 +756,11 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
 // (PF_V1 | PF_CONJ | PF_GRAD | PF_SECOND).
 // PK_POTV and PK_KINV are the velocity gauge's stage potential pass
 // (PF_V1 | PF_V2 | PF_SECOND) and kinetic pass.
+// PK_P1DI is the 1-D imaginary-time stage pass (PF_V1 | PF_V2 | PF_SECOND |
+// PF_KPOST, both modes imaginary).
 enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5, PK_POTL = 6,
                 PK_KINP = 7, PK_POTI = 8, PK_KINI = 9, PK_POTX = 10, PK_POTO = 11, PK_POTS = 12, PK_POTG = 13,
-                PK_POTV = 14, PK_KINV = 15 };
+                PK_POTV = 14, PK_KINV = 15, PK_P1DI = 16 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
     PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
@@ -1022,17 +1024,18 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
 {
     const unsigned fl = KIND == PK_POTL   ? (unsigned)(PF_V1 | PF_SECOND)
                         : (KIND == PK_POTI || KIND == PK_POTV) ? (unsigned)(PF_V1 | PF_V2 | PF_SECOND)
+                        : KIND == PK_P1DI ? (unsigned)(PF_V1 | PF_V2 | PF_SECOND | PF_KPOST)
                         : KIND == PK_POTX ? (unsigned)(PF_V1 | PF_SECOND | PF_PREKX) | (fl_arg & PF_MASK)
                         : KIND == PK_POTO ? (unsigned)(PF_OBS | PF_NOSTORE) | (fl_arg & PF_MASK)
                         : KIND == PK_POTS ? (unsigned)(PF_V1 | PF_SAVE | PF_SECOND)
                         : KIND == PK_POTG ? (unsigned)(PF_V1 | PF_CONJ | PF_GRAD | PF_SECOND)
                                           : fl_arg;
     const int vmode = (KIND == PK_POTL || KIND == PK_POTX || KIND == PK_POTS || KIND == PK_POTG) ? (int)VM_LENGTH
-                      : KIND == PK_POTI                                                        ? (int)VM_IMAG
+                      : (KIND == PK_POTI || KIND == PK_P1DI)                                   ? (int)VM_IMAG
                       : KIND == PK_POTV                                                        ? (int)VM_VELOCITY
                                                                                                : a.vmode;
     const int kmode = (KIND == PK_KINP || KIND == PK_KINS || KIND == PK_KINR) ? (int)KM_PHASE
-                      : KIND == PK_KINI                                       ? (int)KM_IMAG
+                      : (KIND == PK_KINI || KIND == PK_P1DI)                  ? (int)KM_IMAG
                       : KIND == PK_KINV                                       ? (int)KM_VELOCITY
                                                                               : a.kmode;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1040,7 +1043,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
     constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL || KIND == PK_POTI ||
                            KIND == PK_POTX || KIND == PK_POTO || KIND == PK_POTS || KIND == PK_POTG ||
-                           KIND == PK_POTV);
+                           KIND == PK_POTV || KIND == PK_P1DI);
     constexpr bool KINK = (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR || KIND == PK_KINP ||
                            KIND == PK_KINI || KIND == PK_KINV);
 
@@ -1205,7 +1208,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         auto atab_at = [&](int m) { return VPT == 4 ? atv[m] : a.atab[P.off[ax] + t + m * T]; };
         auto kmask_at = [&](int m) { return VPT == 4 ? amv[m] : P.maskt[P.off[ax] + t + m * T]; };
 
-        if (KIND == PK_POT1D && (fl & PF_KPRE)) {
+        if ((KIND == PK_POT1D || KIND == PK_P1DI) && (fl & PF_KPRE)) {
 #pragma unroll
             for (int m = 0; m < VPT; ++m) {
                 const double p = coord_p(P, ax, t + m * T);
@@ -1493,7 +1496,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         if (KINK || (fl & PF_SECOND)) {
             transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
             const R s2 = (R)a.scale2;
-            if (KIND == PK_POT1D && (fl & PF_KPOST)) {
+            if ((KIND == PK_POT1D || KIND == PK_P1DI) && (fl & PF_KPOST)) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) {
                     const double p = coord_p(P, ax, t + m * T);
