@@ -348,8 +348,13 @@ def big3d_run(q, F, imag_steps, real_steps):
              F.Mode(intensity=1e13, wavelength=400.0, cep=ph, envelope="cos2", cycles=8, angle=np.deg2rad(20.0))]
     g.set_integrator("BM4", 0.02)
     g.set_mask(0.1)
-    nt = real_steps + 2
+    nt = max(real_steps, 8) + 2
     g.set_field(*F.tables(modes, "BM4", 0.02, nt, 3))
+    # launch geometry of the real-time step: contiguous and strided lines per
+    # workgroup timed over 6 steps each (the tuner restores the state)
+    from tdse_nd_mpi_amd import tuning
+    cands = [(0, c, s) for c in (0, 1, 2) for s in (0, 4, 8)]
+    best, table = tuning.autotune_batch(g, 0, 1, "length", 0, nsteps=6, cands=cands)
     g.real_steps(0, "length", 0, 1, ["dipole_r", "dipole_a", "norm"])
     g.synchronize()
     t0 = time.perf_counter()
@@ -361,6 +366,8 @@ def big3d_run(q, F, imag_steps, real_steps):
             "imag_grid_point_steps_per_s": N / t_imag,
             "real_BM4_ms_per_step": t_real * 1e3, "real_grid_point_steps_per_s": N / t_real,
             "real_step_roofline_frac": 14 * 2 * 8 * (N / t_real) / (HBM_PEAK_GBS * 1e9),
+            "real_tuned_vpt_twc_tws": list(best),
+            "real_tuning_ms_per_step": {str(list(k)): round(v, 3) for k, v in table},
             "second_mode_phase": ph, "finite": bool(np.isfinite(obs).all()),
             "steps_timed": {"imag": imag_steps, "real": real_steps}}
 
@@ -472,7 +479,7 @@ def main():
                     help="steps of the adjoint-gradient workload (0 skips it)")
     ap.add_argument("--fd-samples", type=int, default=16)
     ap.add_argument("--skip-1d", action="store_true", help="skip the 1-D eigenstate workload")
-    ap.add_argument("--big3d-steps", type=int, default=3, help="timed steps of the 512^3 workload (0 skips it)")
+    ap.add_argument("--big3d-steps", type=int, default=6, help="timed steps of the 512^3 workload (0 skips it)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for counter runs: a 256 MiB device copy (calibration), then 20 c64 steps; no JSON")
     ap.add_argument("--dump", default="", help="write the observable traces to this .npz")
