@@ -325,7 +325,7 @@ def eig1d_run(q):
 # W/cm^2 at 20 degrees, seeded relative phase, with dipole, acceleration and
 # the cos^(1/8) mask every step.  Few steps (the per-step cost is stationary).
 # ---------------------------------------------------------------------
-def big3d_run(q, F, imag_steps, real_steps):
+def big3d_run(q, F, imag_steps, real_steps, fixed=None):
     nr, dr = (512, 512, 512), 0.2
     N = 512 ** 3
     g = q.Grid(nr, dr, "c64", n_slots=8)
@@ -352,9 +352,14 @@ def big3d_run(q, F, imag_steps, real_steps):
     g.set_field(*F.tables(modes, "BM4", 0.02, nt, 3))
     # launch geometry of the real-time step: contiguous and strided lines per
     # workgroup timed over 6 steps each (the tuner restores the state)
+    # (`fixed`: that geometry, untimed, for profiled runs)
     from tdse_nd_mpi_amd import tuning
-    cands = [(0, c, s) for c in (0, 1, 2) for s in (0, 4, 8)]
-    best, table = tuning.autotune_batch(g, 0, 1, "length", 0, nsteps=6, cands=cands)
+    if fixed:
+        best, table = tuple(fixed), []
+        g.set_tuning(*best)
+    else:
+        cands = [(0, c, s) for c in (0, 1, 2) for s in (0, 4, 8)]
+        best, table = tuning.autotune_batch(g, 0, 1, "length", 0, nsteps=6, cands=cands)
     g.real_steps(0, "length", 0, 1, ["dipole_r", "dipole_a", "norm"])
     g.synchronize()
     t0 = time.perf_counter()
@@ -480,6 +485,8 @@ def main():
     ap.add_argument("--fd-samples", type=int, default=16)
     ap.add_argument("--skip-1d", action="store_true", help="skip the 1-D eigenstate workload")
     ap.add_argument("--big3d-steps", type=int, default=6, help="timed steps of the 512^3 workload (0 skips it)")
+    ap.add_argument("--tuning3d", default="",
+                    help="vpt,tw_contig,tw_strided of the 512^3 real-time step instead of the autotuner")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for counter runs: a 256 MiB device copy (calibration), then 20 c64 steps; no JSON")
     ap.add_argument("--dump", default="", help="write the observable traces to this .npz")
@@ -503,9 +510,10 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
     N = int(np.prod(NR))
+    fixed3d = tuple(int(v) for v in args.tuning3d.split(",")) if args.tuning3d else None
     if args.only:
         runs = {"eig1d": lambda: eig1d_run(q),
-                "big3d": lambda: big3d_run(q, F, max(args.big3d_steps, 1), max(args.big3d_steps, 1)),
+                "big3d": lambda: big3d_run(q, F, max(args.big3d_steps, 1), max(args.big3d_steps, 1), fixed3d),
                 "adjoint": lambda: adjoint_run(q, args.adjoint_steps, args.fd_samples),
                 "sweep": lambda: sweep_run(q, F, args.sweep_steps, 10)}
         print(json.dumps({"workload": args.only, "result": runs[args.only]()}))
@@ -617,7 +625,7 @@ def main():
         log(f"  max |E_gpu - E_oracle| = {out['eig1d_4096']['seeded_gaussian']['max_abs_diff_Ha']:.2e} Ha")
     if args.big3d_steps > 0:
         log(f"3-D 512^3: {args.big3d_steps} imaginary and real steps ...")
-        out["big3d_512"] = big3d_run(q, F, args.big3d_steps, args.big3d_steps)
+        out["big3d_512"] = big3d_run(q, F, args.big3d_steps, args.big3d_steps, fixed3d)
         log(f"  real {out['big3d_512']['real_BM4_ms_per_step']:.2f} ms/step")
     if args.adjoint_steps > 0:
         log(f"adjoint gradient: 256^2, {args.adjoint_steps} steps, FD on {args.fd_samples} samples ...")
