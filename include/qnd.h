@@ -144,13 +144,17 @@ int qnd_real_steps(qnd_ctx *ctx, int slot, int gauge, int64_t step0,
  * the conjugated passes, each checkpoint segment is recomputed once, and it
  * returns dL/dEt [nsteps][s][dim] (the stage fields of qnd_set_field),
  * dL/dEobs [nsteps][dim] (the E(t_n) inside a_n) and dL/d(soft-core length)
- * (the initial state held fixed).  Changing the potential, integrator, mask
- * or field between the two calls makes qnd_adjoint fail with QND_ESTATE. */
+ * (the initial state held fixed).  nsteps is the row count of the caller's
+ * arrays and must equal the forward run's (else QND_EINVAL).  qnd_adjoint
+ * always differentiates the context's last checkpointed forward run; changing
+ * the potential, integrator, mask or field between the two calls makes it
+ * fail with QND_ESTATE. */
 int qnd_forward_checkpointed(qnd_ctx *ctx, int slot, int64_t nsteps,
                              int64_t every, double *obs_out,
                              double *norm_final);
-int qnd_adjoint(qnd_ctx *ctx, const double *dL_da, double dL_dnorm,
-                double *dL_dEt, double *dL_dEobs, double *dL_dsoft);
+int qnd_adjoint(qnd_ctx *ctx, int64_t nsteps, const double *dL_da,
+                double dL_dnorm, double *dL_dEt, double *dL_dEobs,
+                double *dL_dsoft);
 
 /* Launch tuning (no reference counterpart: FFTW's planner plays this role,
  * src/meshgrid.c:166-167 with FFTW_MEASURE).  vpt = values per thread of a

@@ -37,8 +37,18 @@
  *                            of the FP64 loop (1001 for T=100, dt=0.1).
  *   or_real_run              unitarity; agreement with or_step_* sequence.
  *   or_field_vectors         closed forms; E = -dA/dt by central differences.
+ *   or_field_vectors_angle   angle 0 equals or_field_vectors bit for bit; a
+ *                            rotated mode equals the rotation matrix applied
+ *                            to the unrotated mode's vectors (and |E|, |A|
+ *                            unchanged); E = -dA/dt by central differences.
  *   or_final_observables     PMD header counts and order on tiny grids (numpy
- *                            restatement), PED sum rule.
+ *                            restatement).  PED: the energy sum rule
+ *                            sum_j S_j de = (2 pi sigma^2)^((1-d)/2)
+ *                            sum_inside |psi_p|^2 (the Gaussian kernel's
+ *                            Riemann sum, Poisson error ~ e^{-2 pi^2}), its
+ *                            first moment = <K> and its variance = Var(K) +
+ *                            sigma^2, with <K>, Var(K) of a boosted Gaussian
+ *                            packet from numpy's FFT and from closed forms.
  * The end-to-end composition of the reference (its exact observables.real
  * traces and output files) is not pinned by any reference-held vector: the
  * reference ships none and cannot be built here (FFTW is absent).
@@ -72,6 +82,7 @@ struct or_ctx {
     double as[2 * OR_MAX_STAGES], as_scaled[2 * OR_MAX_STAGES];
     cplx *in, *out;              /* scratch, like grid->in / grid->out */
     double mask_frac;
+    double dt;                   /* DT as given (the imaginary-time loop steps by it) */
     double *mask;                /* per point, extension */
 };
 
@@ -337,6 +348,7 @@ or_ctx *or_create(int dim, const int64_t *nr, const double *dr,
         c->off[j + 1] = c->off[j] + nr[j];
     }
     c->s = or_integrator_coefficients(integrator, dt, c->as, c->as_scaled);
+    c->dt = dt;
     if (c->s < 0) {
         if (err) snprintf(err, errlen, "Unknown integrator: %s", integrator);
         free(c);
@@ -792,7 +804,7 @@ int or_imag_run(or_ctx *c, int n_states, double T, int guess_mode,
                 uint64_t seed, double *psi_out, double *E_out, double *Lz_out)
 {
     int64_t N = c->N;
-    double dt = c->as_scaled[0] / c->as[0];
+    double dt = c->dt;
     double alpha[OR_MAX_DIM] = {0};
     or_initial_guess(c, n_states, guess_mode, seed, psi_out);
     for (int st = 0; st < n_states; st++) {
@@ -878,6 +890,14 @@ int or_field_vectors(int dim, int nmodes, const double *freq, const double *amp,
                      const double *cep, const double *delay, const int *env,
                      double t, double *A, double *E)
 {
+    return or_field_vectors_angle(dim, nmodes, freq, amp, ellip, duration, cep, delay, env, NULL, t, A, E);
+}
+
+int or_field_vectors_angle(int dim, int nmodes, const double *freq, const double *amp,
+                           const double *ellip, const double *duration,
+                           const double *cep, const double *delay, const int *env,
+                           const double *angle, double t, double *A, double *E)
+{
     double f = 0., df = 0.;
     for (int j = 0; j < dim; j++) { E[j] = 0.; A[j] = 0.; }
     for (int j = 0; j < nmodes; j++) {
@@ -904,10 +924,26 @@ int or_field_vectors(int dim, int nmodes, const double *freq, const double *amp,
         } else {
             return 1;
         }
-        E[0] += -F0 * (df * cos(w * ts + ph) - f * w * sin(w * ts + ph)) / w;
-        if (dim >= 2) E[1] += -F0 * xi * (df * sin(w * ts + ph) + f * w * cos(w * ts + ph)) / w;
-        A[0] += F0 * f * cos(w * ts + ph) / w;
-        if (dim >= 2) A[1] += F0 * xi * f * sin(w * ts + ph) / w;
+        if (!angle) {  /* the reference's accumulation, term by term */
+            E[0] += -F0 * (df * cos(w * ts + ph) - f * w * sin(w * ts + ph)) / w;
+            if (dim >= 2) E[1] += -F0 * xi * (df * sin(w * ts + ph) + f * w * cos(w * ts + ph)) / w;
+            A[0] += F0 * f * cos(w * ts + ph) / w;
+            if (dim >= 2) A[1] += F0 * xi * f * sin(w * ts + ph) / w;
+        } else {
+            /* extension: the mode's (x, y) vectors rotated by angle[j] in the
+             * xy plane (major axis at angle[j] from x) */
+            const double ex = -F0 * (df * cos(w * ts + ph) - f * w * sin(w * ts + ph)) / w;
+            const double ey = -F0 * xi * (df * sin(w * ts + ph) + f * w * cos(w * ts + ph)) / w;
+            const double ax = F0 * f * cos(w * ts + ph) / w;
+            const double ay = F0 * xi * f * sin(w * ts + ph) / w;
+            const double ca = cos(angle[j]), sa = sin(angle[j]);
+            E[0] += ca * ex - sa * ey;
+            A[0] += ca * ax - sa * ay;
+            if (dim >= 2) {
+                E[1] += sa * ex + ca * ey;
+                A[1] += sa * ax + ca * ay;
+            }
+        }
     }
     return 0;
 }
@@ -918,12 +954,22 @@ int or_field_tables(int dim, int nmodes, const double *freq, const double *amp,
                     const double *as_scaled, int s, const double *time,
                     int64_t ntime, double *A, double *E)
 {
+    return or_field_tables_angle(dim, nmodes, freq, amp, ellip, duration, cep, delay, env, NULL, as_scaled, s,
+                                 time, ntime, A, E);
+}
+
+int or_field_tables_angle(int dim, int nmodes, const double *freq, const double *amp,
+                          const double *ellip, const double *duration,
+                          const double *cep, const double *delay, const int *env,
+                          const double *angle, const double *as_scaled, int s,
+                          const double *time, int64_t ntime, double *A, double *E)
+{
     for (int64_t i = 0; i < ntime; i++) {
         double t = time[i] + as_scaled[0];
         int64_t ind = i * dim * s;
         for (int j = 0; j < s; j++) {
-            if (or_field_vectors(dim, nmodes, freq, amp, ellip, duration, cep, delay,
-                                 env, t, &A[ind + j * dim], &E[ind + j * dim]))
+            if (or_field_vectors_angle(dim, nmodes, freq, amp, ellip, duration, cep, delay,
+                                       env, angle, t, &A[ind + j * dim], &E[ind + j * dim]))
                 return 1;
             if (j < s - 1) t += as_scaled[2 * j + 1] + as_scaled[2 * j + 2];
         }

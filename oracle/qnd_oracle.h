@@ -113,12 +113,24 @@ int or_field_vectors(int dim, int nmodes, const double *freq, const double *amp,
                      const double *ellip, const double *duration,
                      const double *cep, const double *delay, const int *env,
                      double t, double *A, double *E);
+/* Extension: each mode's (x, y) field vectors rotated by angle[j] radians in
+ * the xy plane (the reference fixes the major axis on x,
+ * src/parameters.c:942-947).  angle == NULL is or_field_vectors. */
+int or_field_vectors_angle(int dim, int nmodes, const double *freq, const double *amp,
+                           const double *ellip, const double *duration,
+                           const double *cep, const double *delay, const int *env,
+                           const double *angle, double t, double *A, double *E);
 /* Stage tables (src/integrators.c:74-129) [ntime][s][dim]. */
 int or_field_tables(int dim, int nmodes, const double *freq, const double *amp,
                     const double *ellip, const double *duration,
                     const double *cep, const double *delay, const int *env,
                     const double *as_scaled, int s, const double *time,
                     int64_t ntime, double *A, double *E);
+int or_field_tables_angle(int dim, int nmodes, const double *freq, const double *amp,
+                          const double *ellip, const double *duration,
+                          const double *cep, const double *delay, const int *env,
+                          const double *angle, const double *as_scaled, int s,
+                          const double *time, int64_t ntime, double *A, double *E);
 
 /* Final observables (src/io_functions.c:136-287): mask, FFT to unitary
  * p-space, PED [Ne] and the PMD selection (density, phase) pairs in native

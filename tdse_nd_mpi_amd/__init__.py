@@ -71,7 +71,7 @@ def library():
     L.qnd_set_field_batch.argtypes = [vp, C.c_int, C.c_int64, dp, dp, dp]
     L.qnd_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.qnd_forward_checkpointed.argtypes = [vp, C.c_int, C.c_int64, C.c_int64, dp, dp]
-    L.qnd_adjoint.argtypes = [vp, dp, C.c_double, dp, dp, dp]
+    L.qnd_adjoint.argtypes = [vp, C.c_int64, dp, C.c_double, dp, dp, dp]
     L.qnd_real_steps_batch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
     L.qnd_final_observables.argtypes = [vp, C.c_int, dp, dp, C.c_int, C.c_double, dp, dp]
     L.qnd_event_record.argtypes = [vp, C.c_int]
@@ -189,13 +189,24 @@ class Grid:
         return out[:width]
 
     # -- real time ----------------------------------------------------
+    def _check_shape(self, name, a, shape):
+        if tuple(a.shape) != tuple(shape):
+            raise QNDError(f"{name} has shape {tuple(a.shape)}, expected {tuple(shape)} "
+                           f"(the integrator has {self.s} stages)")
+
     def set_field(self, Et, At, Eobs=None, Aobs=None):
         """Field tables [ntime][s][dim] (src/integrators.c:74-129)."""
+        if not self.s:
+            raise QNDError("integrator not set (set_integrator) before the field tables")
         Et = np.ascontiguousarray(Et, dtype=np.float64)
         At = np.ascontiguousarray(At, dtype=np.float64)
         nt = Et.shape[0]
         Eo = np.zeros((nt, self.dim)) if Eobs is None else np.ascontiguousarray(Eobs, dtype=np.float64)
         Ao = np.zeros((nt, self.dim)) if Aobs is None else np.ascontiguousarray(Aobs, dtype=np.float64)
+        self._check_shape("Et", Et, (nt, self.s, self.dim))
+        self._check_shape("At", At, (nt, self.s, self.dim))
+        self._check_shape("Eobs", Eo, (nt, self.dim))
+        self._check_shape("Aobs", Ao, (nt, self.dim))
         self._ck(library().qnd_set_field(self.h, nt, _d(Et), _d(At), _d(Eo), _d(Ao)))
         self.ntime = nt
 
@@ -231,20 +242,27 @@ class Grid:
         last forward_checkpointed run: (dL/dEt [nsteps][s][dim],
         dL/dEobs [nsteps][dim], dL/d soft-core length)."""
         dL_da = np.ascontiguousarray(dL_da, dtype=np.float64)
+        if dL_da.ndim != 2 or dL_da.shape[1] != self.dim:
+            raise QNDError(f"dL_da has shape {dL_da.shape}, expected (nsteps, {self.dim})")
         nsteps = dL_da.shape[0]
         dEt = np.zeros((nsteps, self.s, self.dim))
         dEo = np.zeros((nsteps, self.dim))
         ds = C.c_double(0.0)
-        self._ck(library().qnd_adjoint(self.h, _d(dL_da), float(dL_dnorm), _d(dEt), _d(dEo),
+        self._ck(library().qnd_adjoint(self.h, nsteps, _d(dL_da), float(dL_dnorm), _d(dEt), _d(dEo),
                                        C.cast(C.pointer(ds), C.POINTER(C.c_double))))
         return dEt, dEo, ds.value
 
     def set_field_batch(self, Et, At, Eobs=None):
         """Per-instance field tables: Et, At [nb][ntime][s][dim], Eobs [nb][ntime][dim]."""
+        if not self.s:
+            raise QNDError("integrator not set (set_integrator) before the field tables")
         Et = np.ascontiguousarray(Et, dtype=np.float64)
         At = np.ascontiguousarray(At, dtype=np.float64)
         nb, nt = Et.shape[0], Et.shape[1]
         Eo = np.zeros((nb, nt, self.dim)) if Eobs is None else np.ascontiguousarray(Eobs, dtype=np.float64)
+        self._check_shape("Et", Et, (nb, nt, self.s, self.dim))
+        self._check_shape("At", At, (nb, nt, self.s, self.dim))
+        self._check_shape("Eobs", Eo, (nb, nt, self.dim))
         self._ck(library().qnd_set_field_batch(self.h, nb, nt, _d(Et), _d(At), _d(Eo)))
         self.batch = nb
 

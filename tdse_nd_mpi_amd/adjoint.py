@@ -24,6 +24,7 @@ class Spec:
         self.precision, self.mask, self.every, self.slot = precision, mask, every, slot
         self.grid = None
         self.softcore = None
+        self.runs = 0          # forward runs made on the grid; the last one is what qnd_adjoint differentiates
 
     def ensure(self, softcore):
         if self.grid is None:
@@ -56,20 +57,35 @@ def forward_numpy(spec, E, softcore):
     g = spec.ensure(float(softcore))
     g.set_field(*field_tables(np.asarray(E, dtype=np.float64), g.s, g.dim))
     g.upload(spec.slot, spec.psi0)
-    return g.forward_checkpointed(spec.slot, len(E), spec.every)
+    out = g.forward_checkpointed(spec.slot, len(E), spec.every)
+    spec.runs += 1
+    return out
 
 
 class Propagate(torch.autograd.Function):
+    """The library keeps the checkpoints of one forward run per grid.  Each
+    application records which run it was; a backward whose run is no longer
+    the grid's last (another application ran in between) runs its own
+    forward again before the adjoint, so every gradient belongs to its own
+    field and soft-core length."""
+
     @staticmethod
     def forward(ctx, E, softcore, spec):
-        a, _, nf = forward_numpy(spec, E.detach().cpu().numpy(), float(softcore))
+        En = E.detach().cpu().numpy().astype(np.float64)
+        a, _, nf = forward_numpy(spec, En, float(softcore))
         ctx.spec = spec
+        ctx.run = spec.runs
+        ctx.E = En.copy()
+        ctx.softcore = float(softcore)
         return torch.from_numpy(a.copy()), torch.tensor(nf, dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, grad_a, grad_norm):
+        if ctx.spec.runs != ctx.run:
+            forward_numpy(ctx.spec, ctx.E, ctx.softcore)
+            ctx.run = ctx.spec.runs
         g = ctx.spec.grid
-        ga = np.zeros((grad_a.shape[0], g.dim)) if grad_a is None else grad_a.detach().cpu().numpy()
+        ga = np.zeros((len(ctx.E), g.dim)) if grad_a is None else grad_a.detach().cpu().numpy()
         gn = 0.0 if grad_norm is None else float(grad_norm)
         dEt, dEo, ds = g.adjoint(ga, gn)
         dE = dEt[:, :, 0].sum(axis=1) + dEo[:, 0]

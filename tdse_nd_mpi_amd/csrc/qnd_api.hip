@@ -83,7 +83,9 @@ struct qnd_ctx {
     AxisPlan plan[QND_MAXD];
     std::vector<double> Et, At, Eobs, Aobs;
     int64_t ntime = 0;
+    int field_s = 0;             // stage count the field tables were laid out for
     double *d_bEt = nullptr, *d_bAt = nullptr, *d_bEo = nullptr;  // batch field tables (device)
+    int bfield_s = 0;            // stage count of the batch tables
     // adjoint: checkpoints, saved position-space states, partials
     void *d_adj_ckpt = nullptr, *d_adj_final = nullptr, *d_adj_save = nullptr;
     double *d_adj_part = nullptr;
@@ -93,6 +95,8 @@ struct qnd_ctx {
     int adj_slot = -1, adj_prec = -1;
     int64_t setup_gen = 0, adj_gen = -1;  // potential/integrator/mask/field changes; the forward's value
     int64_t adj_nsteps = 0, adj_K = 0;
+    double *h_gs = nullptr;               // pinned: Gram-Schmidt flag, norm and step, two chunks
+    hipEvent_t ev_gs[2] = {nullptr, nullptr};
     // tabulated kinetic factors (length gauge, four values per thread): one
     // table of N entries per distinct kinetic coefficient of a step
     void *d_ktab = nullptr;
@@ -677,6 +681,20 @@ template <typename R> int ensure_rep(qnd_ctx *c, int slot, int want)
     return 0;
 }
 
+// Pay every slot's owed operators (velocity-gauge last V half-step, mask)
+// with the current set-up.  Called before the set-up changes, so that an
+// operator owed by the last real-time steps is the one those steps used.
+int flush_owed(qnd_ctx *c)
+{
+    for (int k = 0; k < c->n_slots; ++k) {
+        const SlotState &st = c->slots[k];
+        if (!st.pend_mask && !st.pend_vend) continue;
+        const int rc = c->prec == QND_C64 ? ensure_rep<float>(c, k, REP_R) : ensure_rep<double>(c, k, REP_R);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 // Reduction helper: launch k_red and sum partials into d_small[SMALL_SUMS..]
 template <typename R>
 int reduce(qnd_ctx *c, int mode, const C<R> *a, C<R> *b, const C<R> *cc, int axis, double fs,
@@ -903,12 +921,25 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
     if (gs && nprev > 0 && (rc = gram<R>(c, nprev))) return rc;
     C<R> *psi = state_ptr<R>(c, slot);
     double *coef = c->d_small + SMALL_COEF;
-    HIPCK(c, hipMemsetAsync(coef, 0, (2 * nprev + 3) * sizeof(double), c->stream));
+    HIPCK(c, hipMemsetAsync(coef, 0, (2 * nprev + 4) * sizeof(double), c->stream));
     const Geo g0 = geometry(c, 0);
     const int npart = 2 * (nprev + 1);
     if (gs && (rc = ensure_part(c, (size_t)g0.grid * npart))) return rc;
     const double inv_n0 = 1.0 / (double)c->nr[0];
     bool pending = false;
+    // Gram-Schmidt collapse (src/QuantumND_imag.c:297-304): the coefficient
+    // kernel records the step of the first collapse.  Every GS_CHUNK steps the
+    // flag is copied to pinned host memory behind an event; the host reads the
+    // previous chunk's copy after enqueuing the next chunk, so the check never
+    // drains the queue, and a call stops at most two chunks after the step.
+    const int64_t GS_CHUNK = 64;
+    auto collapse_error = [&](const double *h) {
+        return fail(c, QND_EGS_COLLAPSE,
+                    "ERROR: Wavefunction norm decayed to zero during Gram-Schmidt for state %d "
+                    "(step %lld of this call, norm %e, tolerance 1e-12).",
+                    slot, (long long)h[2], h[1]);
+    };
+    int64_t chunks_out = 0;  // flag copies enqueued
     for (int64_t it = 0; it < nsteps; ++it) {
         for (int ia = 0; ia < s; ++ia) {
             PassArgs<R> a = base_args<R>(c, psi, g0);
@@ -954,9 +985,23 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
             ProfScope ps(c, PC_SMALL);
             hipLaunchKernelGGL(k_gs_coeffs, dim3(1), dim3(reduce_threads(g0.grid)), 0, c->stream, c->d_part,
                                g0.grid, npart,
-                               nprev, c->P.dNr * inv_n0, c->d_small + SMALL_GRAM, coef);
+                               nprev, c->P.dNr * inv_n0, c->d_small + SMALL_GRAM, coef, (double)it);
             HIPCK(c, hipGetLastError());
             pending = true;
+            if ((it + 1) % GS_CHUNK == 0 && it + 1 < nsteps) {
+                const int b = (int)(chunks_out & 1);
+                HIPCK(c, hipMemcpyAsync(c->h_gs + 4 * b, coef + 2 * nprev + 1, 3 * sizeof(double),
+                                        hipMemcpyDeviceToHost, c->stream));
+                HIPCK(c, hipEventRecord(c->ev_gs[b], c->stream));
+                if (chunks_out >= 1) {  // the previous chunk's copy; this chunk is already queued
+                    HIPCK(c, hipEventSynchronize(c->ev_gs[1 - b]));
+                    if (c->h_gs[4 * (1 - b)] != 0.0) {
+                        HIPCK(c, hipStreamSynchronize(c->stream));
+                        return collapse_error(c->h_gs + 4 * (1 - b));
+                    }
+                }
+                ++chunks_out;
+            }
         }
     }
     if (pending) {
@@ -968,13 +1013,10 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
         HIPCK(c, hipGetLastError());
     }
     if (gs) {
-        double flag = 0;
-        HIPCK(c, hipMemcpyAsync(&flag, coef + 2 * nprev + 1, sizeof(double), hipMemcpyDeviceToHost,
-                                c->stream));
+        double h[3] = {0, 0, 0};
+        HIPCK(c, hipMemcpyAsync(h, coef + 2 * nprev + 1, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCK(c, hipStreamSynchronize(c->stream));
-        if (flag != 0.0)
-            return fail(c, QND_EGS_COLLAPSE,
-                        "ERROR: Wavefunction norm decayed to zero during Gram-Schmidt for state %d.", slot);
+        if (h[0] != 0.0) return collapse_error(h);
     }
     prof_flush(c);
     return 0;
@@ -1035,6 +1077,10 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
     const bool batched = nb > 1;
     const int64_t ntime = batched ? c->bntime : c->ntime;
     int rc;
+    // the tables are [ntime][s][dim] for the stage count they were set with
+    if (ntime > 0 && (batched ? c->bfield_s : c->field_s) != s)
+        return fail(c, QND_ESTATE, "the field tables were laid out for %d stages but the integrator has %d "
+                    "(set the field again after qnd_set_integrator)", batched ? c->bfield_s : c->field_s, s);
     if (step0 < 0 || step0 + nsteps > ntime)
         return fail(c, QND_EINVAL, "steps [%lld, %lld) outside the field table of %lld steps",
                     (long long)step0, (long long)(step0 + nsteps), (long long)ntime);
@@ -1062,6 +1108,11 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
             break;
         }
     }
+    // the length-gauge step starts from the plain state: a velocity-gauge V
+    // half-step owed by earlier steps is paid now, at its own time
+    if (gauge == QND_GAUGE_LENGTH && c->slots[slot0].pend_vend)
+        for (int k = 0; k < nb; ++k)
+            if ((rc = ensure_rep<R>(c, slot0 + k, REP_R))) return rc;
     SlotState &st = c->slots[slot0];
     C<R> *psi = state_ptr<R>(c, slot0);
     const Geo g0 = geometry(c, 0, nb);
@@ -2043,6 +2094,10 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
         }
     }
     for (int k = 0; k < 16; ++k) hipEventCreate(&c->ev[k]);
+    if (hipHostMalloc((void **)&c->h_gs, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_gs[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_gs[1], hipEventDisableTiming) != hipSuccess)
+        return bail(fail(nullptr, QND_EHIP, "pinned host buffer or events for the Gram-Schmidt check failed"));
     *out = c;
     return 0;
 }
@@ -2080,6 +2135,9 @@ void qnd_destroy(qnd_ctx *c)
         if (p) hipFree(p);
     for (int k = 0; k < 16; ++k)
         if (c->ev[k]) hipEventDestroy(c->ev[k]);
+    for (int k = 0; k < 2; ++k)
+        if (c->ev_gs[k]) hipEventDestroy(c->ev_gs[k]);
+    if (c->h_gs) hipHostFree(c->h_gs);
     if (c->stream2) {
         hipStreamSynchronize(c->stream2);
         hipStreamDestroy(c->stream2);
@@ -2091,6 +2149,7 @@ void qnd_destroy(qnd_ctx *c)
 int qnd_set_potential(qnd_ctx *c, const char *name, double softcore, const double *cap)
 {
     if (!c || !name) return QND_EINVAL;
+    if (int rc = flush_owed(c)) return rc;  // owed operators keep the set-up they were owed under
     ++c->setup_gen;
     int kind;
     double b, c2;
@@ -2120,6 +2179,7 @@ int qnd_set_potential(qnd_ctx *c, const char *name, double softcore, const doubl
 int qnd_set_integrator(qnd_ctx *c, const char *name, double dt)
 {
     if (!c || !name) return QND_EINVAL;
+    if (int rc = flush_owed(c)) return rc;  // owed operators keep the set-up they were owed under
     ++c->setup_gen;
     static const double bm4[6] = {.0792036964311957, .1303114101821663, .2228614958676077,
                                   -.3667132690474257, .3246481886897062, .1096884778767498};
@@ -2149,6 +2209,7 @@ int qnd_set_integrator(qnd_ctx *c, const char *name, double dt)
 int qnd_set_mask(qnd_ctx *c, double frac)
 {
     if (!c) return QND_EINVAL;
+    if (int rc = flush_owed(c)) return rc;  // owed operators keep the set-up they were owed under
     ++c->setup_gen;
     // m_j(x) = cos(pi/2 (|x| - x0)/(L - x0))^(1/8) for |x| > x0 = (1-frac) L,
     // L = n_j dr_j / 2; the mask is the product over axes
@@ -2261,6 +2322,7 @@ int qnd_set_field(qnd_ctx *c, int64_t ntime, const double *Et, const double *At,
     HIPCK(c, hipMalloc(&c->d_Eo, (no ? no : 1) * sizeof(double)));
     if (no) HIPCK(c, hipMemcpy(c->d_Eo, c->Eobs.data(), no * sizeof(double), hipMemcpyHostToDevice));
     c->ntime = ntime;
+    c->field_s = c->s;
     ++c->setup_gen;
     return 0;
 }
@@ -2286,15 +2348,23 @@ int qnd_forward_checkpointed(qnd_ctx *c, int slot, int64_t nsteps, int64_t every
     if (c->dim < 2) return fail(c, QND_EUNSUPPORTED, "the adjoint needs dim >= 2");
     if (nsteps < 1 || every < 1) return fail(c, QND_EINVAL, "nsteps and the checkpoint interval must be >= 1");
     if (nsteps > c->ntime) return fail(c, QND_EINVAL, "%lld steps exceed the field table", (long long)nsteps);
+    if (c->field_s != c->s)
+        return fail(c, QND_ESTATE, "the field tables were laid out for %d stages but the integrator has %d "
+                    "(set the field again after qnd_set_integrator)", c->field_s, c->s);
     return DISPATCH(c, adj_forward, c, slot, nsteps, every, obs_out, norm_final);
 }
 
-int qnd_adjoint(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dEt, double *dL_dEobs,
-                double *dL_dsoft)
+int qnd_adjoint(qnd_ctx *c, int64_t nsteps, const double *dL_da, double dL_dnorm, double *dL_dEt,
+                double *dL_dEobs, double *dL_dsoft)
 {
     if (!c || !dL_da || !dL_dEt || !dL_dEobs) return QND_EINVAL;
     if (c->adj_nsteps < 1 || c->adj_prec != c->prec)
         return fail(c, QND_ESTATE, "no checkpointed forward run (qnd_forward_checkpointed)");
+    // the caller's arrays are sized for nsteps rows: they must be the
+    // forward run's, or the sweep would read and write past them
+    if (nsteps != c->adj_nsteps)
+        return fail(c, QND_EINVAL, "the gradient arrays hold %lld steps but the checkpointed forward run has %lld",
+                    (long long)nsteps, (long long)c->adj_nsteps);
     if (c->adj_gen != c->setup_gen)
         return fail(c, QND_ESTATE, "potential, integrator, mask or field changed since the checkpointed forward run");
     return DISPATCH(c, adj_backward, c, dL_da, dL_dnorm, dL_dEt, dL_dEobs, dL_dsoft);
@@ -2334,6 +2404,7 @@ int qnd_set_field_batch(qnd_ctx *c, int nb, int64_t ntime, const double *Et, con
     if (Eobs && no) HIPCK(c, hipMemcpy(c->d_bEo, Eobs, no * sizeof(double), hipMemcpyHostToDevice));
     c->bnb = nb;
     c->bntime = ntime;
+    c->bfield_s = c->s;
     return 0;
 }
 

@@ -1618,10 +1618,12 @@ __global__ void __launch_bounds__(1024) k_obs_finalize(const double *__restrict_
 // representation (scale = dNr / n0 for the axis-0 Fourier representation):
 // b_j = <prev_j|v>, nv = <v|v>; c_j = b_j - sum_{i<j} c_i G_{j i};
 // |v'|^2 = nv - 2 Re sum_j c_j conj(b_j) + sum_{ij} conj(c_i) c_j G_{ij}.
-// coef = [c_0 .. c_{N-1} (re,im), invnorm, flag, norm]
+// coef = [c_0 .. c_{N-1} (re,im), invnorm, flag, norm, step of the first collapse]
+// `step` is the caller's index of the step whose dots these are: the first
+// collapse (norm below 1e-12) records it beside the sticky flag.
 __global__ void __launch_bounds__(1024) k_gs_coeffs(const double *__restrict__ part, int nwg, int npart,
                                                     int nprev, double scale, const double *__restrict__ gram,
-                                                    double *coef)
+                                                    double *coef, double step)
 {
     __shared__ double b[2 * 65 + 8];
     __shared__ double cre[64], cim[64];
@@ -1660,9 +1662,17 @@ __global__ void __launch_bounds__(1024) k_gs_coeffs(const double *__restrict__ p
             coef[2 * j] = cre[j];
             coef[2 * j + 1] = cim[j];
         }
-        if (n2 < 1.e-12) { coef[2 * nprev] = 0.0; coef[2 * nprev + 1] = 1.0; }  // sticky flag
-        else coef[2 * nprev] = 1.0 / sqrt(n2);
-        coef[2 * nprev + 2] = n2;
+        if (n2 < 1.e-12) {  // sticky flag; the norm and step of the first collapse are kept
+            if (coef[2 * nprev + 1] == 0.0) {
+                coef[2 * nprev + 2] = n2;
+                coef[2 * nprev + 3] = step;
+            }
+            coef[2 * nprev] = 0.0;
+            coef[2 * nprev + 1] = 1.0;
+        } else {
+            coef[2 * nprev] = 1.0 / sqrt(n2);
+            if (coef[2 * nprev + 1] == 0.0) coef[2 * nprev + 2] = n2;
+        }
     }
 }
 

@@ -90,6 +90,7 @@ int integrator_coefficients(const char *name, double dt, double *as, double *as_
 void gaussian_stream(uint64_t seed, int64_t n, double *out);
 
 /* I/O (src/io_functions.c) */
+void softcore_side_file(const char *output_imag, char *buf, size_t len);
 void Write_output_imag(const char *filename, double **Psi, double **Ej, const Parameters *P, int64_t N);
 void Write_observables(FILE *f, double time, const double *values, int nvalues);
 void Write_wavefunction_to_file(const char *filename, const double *Psi, const Parameters *P, int64_t N);

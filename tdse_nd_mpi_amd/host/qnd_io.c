@@ -2,7 +2,9 @@
  * qnd_io.c -- file formats of the QuantumND drop-in, byte-compatible with
  * src/io_functions.c for a single process: output.imag (text header + FP64
  * complex binary), observables.real, wavefunction.real, PED.real, PMD.real,
- * and the centred read-back of output.imag.
+ * and the centred read-back of output.imag.  The soft-core extension travels
+ * in a side file (<output.imag>.softcore), so the reference can read every
+ * output.imag this code writes.
  */
 #include "qnd_host.h"
 
@@ -10,9 +12,31 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Name of the side file that carries the soft-core extension next to an
+ * output.imag file: <output.imag>.softcore.  The header itself keeps the
+ * reference's keys only, because the reference reader stops at an unknown
+ * key (src/io_functions.c:377-385) and then cannot find the PSI marker. */
+void softcore_side_file(const char *output_imag, char *buf, size_t len)
+{
+    snprintf(buf, len, "%s.softcore", output_imag);
+}
+
 /* src/io_functions.c:57-106 */
 void Write_output_imag(const char *filename, double **Psi, double **Ej, const Parameters *P, int64_t N)
 {
+    char side[MAX_OUTPUT_FILES_CHAR + 16];
+    softcore_side_file(filename, side, sizeof side);
+    if (P->softcore >= 0) {
+        FILE *s = fopen(side, "w");
+        if (!s) {
+            perror("Error opening the soft-core side file");
+            qnd_abort();
+        }
+        fprintf(s, "SOFTCORE\t%16.15f\n", P->softcore);
+        fclose(s);
+    } else {
+        remove(side);  /* a stale side file from an earlier run must not apply */
+    }
     FILE *f = fopen(filename, "w");
     if (!f) {
         perror("Error opening the file in write_output_imag");
@@ -25,7 +49,6 @@ void Write_output_imag(const char *filename, double **Psi, double **Ej, const Pa
     fprintf(f, "\nDR");
     for (int j = 0; j < P->dim; j++) fprintf(f, "\t%16.15f", P->dr[j]);
     fprintf(f, "\nPOTENTIAL\t%s\n", P->potential);
-    if (P->softcore >= 0) fprintf(f, "SOFTCORE\t%16.15f\n", P->softcore);  /* extension only */
     fprintf(f, "E\t");
     for (int j = 0; j < P->N_states; j++) fprintf(f, "\t%16.15f", Ej[j][0]);
     fprintf(f, "\nDE");

@@ -488,7 +488,7 @@ void Initialize_parameters_real(const char *filename, Parameters *P, Parameters 
                     perror("The potential name of OUPUT_IMAG is undetermined");
                     local_error = 1;
                 }
-            } else if (strcmp(key, "SOFTCORE") == 0) {  /* extension line */
+            } else if (strcmp(key, "SOFTCORE") == 0) {  /* header line of older files of this code */
                 if (fscanf(f, "%lf", &P->softcore) != 1) local_error = 1;
             } else {
                 printf("\nUnknown parameter: %s\n", key);
@@ -502,6 +502,18 @@ void Initialize_parameters_real(const char *filename, Parameters *P, Parameters 
         local_error = 1;
     }
     for (j = 0; j < dim; j++) P_imag->dr[j] = P->dr[j];
+    {   /* soft-core extension: the side file written next to OUTPUT_IMAG */
+        char side[MAX_OUTPUT_FILES_CHAR + 16];
+        softcore_side_file(P->output_imag, side, sizeof side);
+        FILE *s = fopen(side, "r");
+        if (s) {
+            if (fscanf(s, "%31s", key) != 1 || strcmp(key, "SOFTCORE") != 0 || fscanf(s, "%lf", &P->softcore) != 1) {
+                printf("\nMalformed soft-core side file: %s\n", side);
+                local_error = 1;
+            }
+            fclose(s);
+        }
+    }
 
     /* full read of params.real (:468-706) */
     f = fopen(filename, "r");

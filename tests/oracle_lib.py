@@ -60,6 +60,10 @@ def lib():
                                        ip, C.c_double, dp, dp]
         L.or_field_tables.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, ip,
                                       dp, C.c_int, dp, C.c_int64, dp, dp]
+        L.or_field_vectors_angle.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp, dp,
+                                             ip, dp, C.c_double, dp, dp]
+        L.or_field_tables_angle.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, ip, dp,
+                                            dp, C.c_int, dp, C.c_int64, dp, dp]
         L.or_final_observables.restype = C.c_int64
         L.or_final_observables.argtypes = [C.c_void_p, dp, dp, dp, C.c_int,
                                            C.c_double, C.c_double, dp, dp, i64p]
@@ -106,13 +110,17 @@ def imag_step_count(T, dt):
 
 
 def field_vectors(dim, modes, t):
-    """modes: dict of arrays freq, amp, ellip, duration, cep, delay, env."""
+    """modes: dict of arrays freq, amp, ellip, duration, cep, delay, env and,
+    optionally, angle (the polarisation-angle extension, radians)."""
     m = _modes(modes)
     A = np.zeros(dim)
     E = np.zeros(dim)
-    rc = lib().or_field_vectors(dim, m["n"], *[_d(m[k]) for k in
-                                               ("freq", "amp", "ellip", "duration", "cep", "delay")],
-                                _i(m["env"]), float(t), _d(A), _d(E))
+    args = [_d(m[k]) for k in ("freq", "amp", "ellip", "duration", "cep", "delay")]
+    if m["angle"] is None:
+        rc = lib().or_field_vectors(dim, m["n"], *args, _i(m["env"]), float(t), _d(A), _d(E))
+    else:
+        rc = lib().or_field_vectors_angle(dim, m["n"], *args, _i(m["env"]), _d(m["angle"]), float(t),
+                                          _d(A), _d(E))
     if rc:
         raise ValueError("bad envelope")
     return A, E
@@ -124,10 +132,13 @@ def field_tables(dim, modes, as_scaled, s, time):
     nt = len(time)
     A = np.zeros(nt * s * dim)
     E = np.zeros(nt * s * dim)
-    rc = lib().or_field_tables(dim, m["n"], *[_d(m[k]) for k in
-                                              ("freq", "amp", "ellip", "duration", "cep", "delay")],
-                               _i(m["env"]), _d(np.ascontiguousarray(as_scaled, dtype=np.float64)),
-                               s, _d(time), nt, _d(A), _d(E))
+    args = [_d(m[k]) for k in ("freq", "amp", "ellip", "duration", "cep", "delay")]
+    asc = _d(np.ascontiguousarray(as_scaled, dtype=np.float64))
+    if m["angle"] is None:
+        rc = lib().or_field_tables(dim, m["n"], *args, _i(m["env"]), asc, s, _d(time), nt, _d(A), _d(E))
+    else:
+        rc = lib().or_field_tables_angle(dim, m["n"], *args, _i(m["env"]), _d(m["angle"]), asc, s, _d(time),
+                                         nt, _d(A), _d(E))
     if rc:
         raise ValueError("bad envelope")
     return A.reshape(nt, s, dim), E.reshape(nt, s, dim)
@@ -138,6 +149,8 @@ def _modes(modes):
     for k in ("freq", "amp", "ellip", "duration", "cep", "delay"):
         out[k] = np.ascontiguousarray(modes[k], dtype=np.float64)
     out["env"] = np.ascontiguousarray(modes["env"], dtype=np.int32)
+    out["angle"] = (np.ascontiguousarray(modes["angle"], dtype=np.float64)
+                    if modes.get("angle") is not None else None)
     out["n"] = len(out["freq"])
     return out
 

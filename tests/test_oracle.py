@@ -412,3 +412,146 @@ def test_gaussian_stream_splitmix_boxmuller():
     assert np.allclose(got, ref, rtol=1e-14, atol=1e-15)
     big = O.gaussian(7, 200000)
     assert abs(big.mean()) < 0.01 and abs(big.std() - 1) < 0.01
+
+
+# ---------------------------------------------------------------- PED pinned by mathematics
+def _boosted_packet(n, dr, w, k0):
+    x = -0.5 * n * dr + np.arange(n) * dr
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    return np.exp(-(X ** 2 + Y ** 2) / (2 * w * w)) * np.exp(1j * k0 * X)
+
+
+def test_ped_sum_rule_and_moments():
+    """The PED (src/io_functions.c:172-199) is the unitary p-space density
+    smoothed by a normalised Gaussian of width sigma = DE and sampled at
+    E_j = j DE.  For a boosted Gaussian packet whose energies lie well inside
+    [0, (Ne-1) DE], the Riemann sums of that kernel are exact up to a Poisson
+    term ~ e^{-2 pi^2}, which fixes three properties independently of the
+    oracle's formula:
+      sum_j S_j DE         = (2 pi sigma^2)^((1-d)/2) sum |psi|^2   (normalisation, exponent d/2)
+      sum_j E_j S_j / sum  = <K>                                     (kernel centred at j DE)
+      variance of S        = Var(K) + sigma^2                        (kernel width sigma, not sigma^2)
+    <K> and Var(K) come from numpy's FFT of the packet and, to the grid's
+    accuracy, from the closed forms of a Gaussian in k-space."""
+    n, dr, w, k0, d = 256, 0.5, 4.0, 1.5, 2
+    o = O.Oracle((n, n), dr, "H_2D", "BM4", 0.1)
+    psi = _boosted_packet(n, dr, w, k0)
+    Ne, de = 400, 0.01
+    # fm = 1 everywhere (every radius exceeds R1 < 0); every momentum inside
+    _, ped, _, _ = o.final_observables(psi, (-2.0, -1.0), (100.0, 100.0), Ne, de)
+    sigma = de
+    E = np.arange(Ne) * de
+    total = np.sum(np.abs(psi) ** 2)            # Parseval: the unitary p-space state has the same sum
+    lhs = np.sum(ped) * de
+    rhs = (2 * np.pi * sigma * sigma) ** ((1 - d) / 2) * total
+    assert abs(lhs / rhs - 1) < 1e-8, (lhs, rhs)
+    # moments of K from numpy's FFT
+    p = 2 * np.pi * np.fft.fftfreq(n, dr)
+    K = 0.5 * (p[:, None] ** 2 + p[None, :] ** 2)
+    dens = np.abs(np.fft.fftn(psi)) ** 2
+    mK = np.sum(dens * K) / np.sum(dens)
+    vK = np.sum(dens * (K - mK) ** 2) / np.sum(dens)
+    mS = np.sum(E * ped) / np.sum(ped)
+    vS = np.sum((E - mS) ** 2 * ped) / np.sum(ped)
+    assert abs(mS - mK) < 1e-8 * mK, (mS, mK)
+    assert abs(vS - (vK + sigma ** 2)) < 1e-6 * vK, (vS, vK + sigma ** 2)
+    # closed forms: |psi_p(k)|^2 ~ exp(-(k - k0)^2 w^2), variance s2 = 1/(2 w^2) per axis
+    s2 = 1.0 / (2 * w * w)
+    assert abs(mK - 0.5 * (k0 * k0 + d * s2)) < 1e-9, mK
+    vK_closed = (4 * k0 * k0 * s2 + 2 * s2 * s2) / 4 + (d - 1) * (2 * s2 * s2) / 4
+    assert abs(vK - vK_closed) < 1e-9, (vK, vK_closed)
+
+
+def test_ped_restricted_momenta_and_mask():
+    """P_MAX restricts the PED to the points of the PMD selection, and R_MASK
+    zeroes the inner region: with both the sum rule holds on the selected,
+    masked density (numpy FFT of the masked packet, selected by |p_j|)."""
+    n, dr, w, k0 = 128, 0.5, 3.0, 1.2
+    o = O.Oracle((n, n), dr, "H_2D", "BM4", 0.1)
+    psi = _boosted_packet(n, dr, w, k0)
+    Rm, pm, Ne, de = (1.0, 4.0), (1.6, 0.3), 500, 0.01
+    _, ped, _, _ = o.final_observables(psi, Rm, pm, Ne, de)
+    x = -0.5 * n * dr + np.arange(n) * dr
+    R = np.sqrt(x[:, None] ** 2 + x[None, :] ** 2)
+    fm = np.where((R >= Rm[0]) & (R <= Rm[1]), np.sin(0.5 * np.pi * (R - Rm[0]) / (Rm[1] - Rm[0])) ** 2,
+                  np.where(R > Rm[1], 1.0, 0.0))
+    pp = np.abs(np.fft.fftn(fm * psi)) ** 2 / psi.size
+    p = 2 * np.pi * np.fft.fftfreq(n, dr)
+    inside = (np.abs(p)[:, None] <= pm[0]) & (np.abs(p)[None, :] <= pm[1])
+    K = 0.5 * (p[:, None] ** 2 + p[None, :] ** 2)
+    assert K[inside].max() < (Ne - 1) * de - 8 * de and K[inside].min() >= 0
+    lhs = np.sum(ped) * de
+    # points with K < 8 sigma lose part of the kernel below E = 0: sum them exactly
+    kern = np.exp(-0.5 * (K[inside][:, None] - np.arange(Ne)[None, :] * de) ** 2 / de ** 2).sum(axis=1)
+    rhs = np.sum(pp[inside] * kern) * de / (2 * np.pi * de * de)
+    assert abs(lhs / rhs - 1) < 1e-12, (lhs, rhs)
+    far = K[inside] > 8 * de
+    rhs_far = (2 * np.pi * de * de) ** (-0.5) * np.sum(pp[inside][far])
+    lhs_far = np.sum(pp[inside][far] * kern[far]) * de / (2 * np.pi * de * de)
+    assert abs(lhs_far / rhs_far - 1) < 1e-8, (lhs_far, rhs_far)
+
+
+# ---------------------------------------------------------------- polarisation angle (extension)
+def _two_colour(angle2=None):
+    w1 = 45.5637639947958 / 800.0
+    m = dict(freq=[w1, 2 * w1], amp=[5.33803e-9 * np.sqrt(1e14), 5.33803e-9 * np.sqrt(1e13)],
+             ellip=[0.0, 0.3], duration=[4 * 2 * np.pi / w1, 8 * 2 * np.pi / (2 * w1)], cep=[0.0, 1.1],
+             delay=[0.0, 5.0], env=[O.ENV["cos2"], O.ENV["cos2"]])
+    if angle2 is not None:
+        m["angle"] = [0.0, angle2]
+    return m
+
+
+def test_field_angle_zero_is_reference():
+    for t in np.linspace(0.0, 450.0, 37):
+        A0, E0 = O.field_vectors(3, _two_colour(), t)
+        A1, E1 = O.field_vectors(3, _two_colour(0.0), t)
+        assert np.allclose(A1, A0, rtol=1e-15, atol=1e-300) and np.allclose(E1, E0, rtol=1e-15, atol=1e-300)
+        assert A1[2] == 0.0 and E1[2] == 0.0     # no z component (src/parameters.c:942-947)
+
+
+def test_field_angle_is_a_rotation():
+    """A rotated mode is the rotation matrix applied to the unrotated mode's
+    (x, y) vectors; modes add linearly; E = -dA/dt still holds."""
+    th = np.deg2rad(20.0)
+    Rot = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    single = lambda m, k: {key: [v[k]] for key, v in m.items()}
+    m = _two_colour(th)
+    for t in np.linspace(1.0, 440.0, 23):
+        A, E = O.field_vectors(2, m, t)
+        A1, E1 = O.field_vectors(2, single(_two_colour(), 0), t)
+        A2, E2 = O.field_vectors(2, single(_two_colour(), 1), t)
+        assert np.allclose(A, A1 + Rot @ A2, rtol=1e-13, atol=1e-15)
+        assert np.allclose(E, E1 + Rot @ E2, rtol=1e-13, atol=1e-15)
+        A2r, E2r = O.field_vectors(2, single(m, 1), t)
+        assert abs(np.linalg.norm(E2r) - np.linalg.norm(E2)) < 1e-15 + 1e-13 * np.linalg.norm(E2)
+        h = 1e-4
+        Ap, _ = O.field_vectors(2, m, t + h)
+        Am, _ = O.field_vectors(2, m, t - h)
+        assert np.allclose(E, -(Ap - Am) / (2 * h), atol=1e-9)
+
+
+def test_field_py_matches_oracle():
+    """tdse_nd_mpi_amd.field (the tables the benchmark and several GPU tests
+    propagate under) against the oracle's stage tables, every envelope, with
+    and without the polarisation angle."""
+    from tdse_nd_mpi_amd import field as F
+    dt, nt = 0.05, 900
+    for env in ("constant", "trapz", "cos2", "cos4"):
+        for integ in ("BM4", "BM6"):
+            modes = [F.Mode(intensity=1e14, wavelength=800.0, ellipticity=0.2, cep=0.4, delay=3.0,
+                            envelope=env, cycles=2),
+                     F.Mode(intensity=1e13, wavelength=400.0, ellipticity=0.0, cep=1.1, delay=0.0,
+                            envelope=env, cycles=3, angle=np.deg2rad(20.0))]
+            Et, At, Eo, Ao = F.tables(modes, integ, dt, nt, 2)
+            om = dict(freq=[m.w for m in modes], amp=[m.F0 for m in modes], ellip=[m.xi for m in modes],
+                      duration=[m.tau for m in modes], cep=[m.phi for m in modes],
+                      delay=[m.delay for m in modes], env=[O.ENV[env]] * 2, angle=[m.angle for m in modes])
+            s, _, asc = O.integrator_coefficients(integ, dt)
+            time = np.arange(nt) * dt
+            A, E = O.field_tables(2, om, asc, s, time)
+            scale = max(np.abs(E).max(), 1e-300)
+            assert np.allclose(Et, E, rtol=0, atol=1e-13 * scale), (env, integ, np.abs(Et - E).max())
+            assert np.allclose(At, A, rtol=0, atol=1e-13 * np.abs(A).max()), (env, integ)
+            Eo_ref = np.array([O.field_vectors(2, om, t)[1] for t in time[::50]])
+            assert np.allclose(Eo[::50], Eo_ref, rtol=0, atol=1e-13 * scale)
