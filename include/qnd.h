@@ -115,6 +115,24 @@ int qnd_energy_moments(qnd_ctx *ctx, int slot, int M, double *E);
 int qnd_observables(qnd_ctx *ctx, int slot, const int *codes, int ncodes,
                     const double *alpha, const double *Efield, double *out);
 
+/* Allocation-free variant of qnd_observables, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Computes the fused position-space codes QND_OBS_NORM, QND_OBS_DIPOLE_R and
+ * QND_OBS_DIPOLE_A (others: QND_EUNSUPPORTED) of the state psi_dev, a device
+ * array of N elements of the context's precision (float2 / double2,
+ * row-major, position space), into out_dev, a device array of the codes'
+ * widths (as qnd_observables).  Efield (host, dim values or NULL) is E(t) of
+ * dipole_a.  It calls no allocating routine: its reduction partials and sums
+ * live in a fixed device array of QND_NOALLOC_ROWS (8192) partial rows per
+ * GPU, held until the process ends; a grid needing more rows is refused
+ * (QND_EINVAL).  The call is ordered on the context's stream and returns
+ * without waiting: out_dev is complete once the stream reaches it (e.g.
+ * after qnd_synchronize).  Calls that overlap on one GPU share that array
+ * and race; the caller must order them. */
+int qnd_observables_noalloc(qnd_ctx *ctx, const void *psi_dev, const int *codes,
+                            int ncodes, const double *Efield, double *out_dev);
+
 /* Field tables of Initialize_field_vectors (src/integrators.c:74-129):
  * Et, At [ntime][s][dim] at the stage times; Eobs, Aobs [ntime][dim] at the
  * step times t_n (Field_vectors of src/QuantumND_real.c:173).  Copied. */

@@ -29,7 +29,7 @@ EXPORTS = ("qnd_create", "qnd_destroy", "qnd_last_error", "qnd_set_potential", "
            "qnd_real_steps", "qnd_final_observables", "qnd_event_record", "qnd_event_elapsed",
            "qnd_synchronize", "qnd_profile_enable", "qnd_profile_read", "qnd_version",
            "qnd_set_field_batch", "qnd_real_steps_batch", "qnd_set_tuning", "qnd_forward_checkpointed",
-           "qnd_adjoint")
+           "qnd_adjoint", "qnd_observables_noalloc")
 
 _lib = None
 
@@ -66,6 +66,7 @@ def library():
     L.qnd_transform.argtypes = [vp, C.c_int, C.c_int]
     L.qnd_energy_moments.argtypes = [vp, C.c_int, C.c_int, dp]
     L.qnd_observables.argtypes = [vp, C.c_int, ip, C.c_int, dp, dp, dp]
+    L.qnd_observables_noalloc.argtypes = [vp, vp, ip, C.c_int, dp, vp]
     L.qnd_set_field.argtypes = [vp, C.c_int64, dp, dp, dp, dp]
     L.qnd_real_steps.argtypes = [vp, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
     L.qnd_set_field_batch.argtypes = [vp, C.c_int, C.c_int64, dp, dp, dp]
@@ -187,6 +188,16 @@ class Grid:
         self._ck(library().qnd_observables(self.h, slot, codes.ctypes.data_as(C.POINTER(C.c_int)),
                                            len(codes), _opt_d(al), _opt_d(Ef), _d(out)))
         return out[:width]
+
+    def observables_noalloc(self, psi_dev, names, out_dev, Efield=None):
+        """qnd_observables_noalloc: norm / dipole_r / dipole_a of the device
+        state at address psi_dev into the device array at out_dev, without
+        allocating; ordered on the context's stream (synchronize() to wait)."""
+        codes = np.array([OBS[n] for n in names], dtype=np.int32)
+        Ef = None if Efield is None else np.ascontiguousarray(Efield, dtype=np.float64)
+        self._ck(library().qnd_observables_noalloc(self.h, C.c_void_p(int(psi_dev)),
+                                                   codes.ctypes.data_as(C.POINTER(C.c_int)), len(codes),
+                                                   _opt_d(Ef), C.c_void_p(int(out_dev))))
 
     # -- real time ----------------------------------------------------
     def _check_shape(self, name, a, shape):

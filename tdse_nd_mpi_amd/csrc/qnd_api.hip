@@ -20,6 +20,7 @@
 #include <cstring>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -124,6 +125,37 @@ struct qnd_ctx {
 };
 
 static std::string g_create_err;
+
+// ---------------------------------------------------------------------
+// Scratch of the allocation-free entry point (qnd_observables_noalloc): the
+// per-workgroup FP64 partials of its reduction and their column sums, in a
+// file-scope device array that each GPU holds until the process ends.  In
+// host code `noalloc_scratch` names a host-side stand-in, not the device
+// array: the device address comes from hipGetSymbolAddress, once per device
+// (qnd_create, with that device current), and only that address is used.
+// ---------------------------------------------------------------------
+#define QND_NOALLOC_ROWS 8192                                   // partial rows (workgroups) at most
+#define QND_NOALLOC_DOUBLES (QND_NOALLOC_ROWS * QND_NOBS + 64)  // + the column sums
+static __device__ __align__(256) double noalloc_scratch[QND_NOALLOC_DOUBLES];
+#define QND_MAX_DEVICES 64
+static std::mutex g_noalloc_mu;
+static double *g_noalloc_ptr[QND_MAX_DEVICES] = {};
+static hipError_t g_noalloc_err[QND_MAX_DEVICES] = {};
+static bool g_noalloc_done[QND_MAX_DEVICES] = {};
+
+// The device address of this device's scratch, looked up on the device's
+// first context (the caller has made the device current).
+static void noalloc_setup(int device)
+{
+    if (device < 0 || device >= QND_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_done[device]) return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch));
+    g_noalloc_err[device] = e;
+    g_noalloc_ptr[device] = e == hipSuccess ? (double *)ptr : nullptr;
+    g_noalloc_done[device] = true;
+}
 
 namespace {
 
@@ -1933,6 +1965,52 @@ template <typename R> int make_twiddles(qnd_ctx *c, int axis)
     return 0;
 }
 
+// qnd_observables_noalloc: the fused position-space observables (norm,
+// dipole_r, dipole_a) of a caller's device state with the reduction of
+// qnd_observables (same kernel, same partition into workgroups, same column
+// sums), its partials and sums in this device's scratch array, and the record
+// assembled on the device into the caller's output.  Nothing is allocated.
+__global__ void k_obs_assemble(const double *__restrict__ sums, ObsSpec spec, int dim, double *out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int off = 0;
+    for (int k = 0; k < spec.ncodes; ++k) {
+        const int c = spec.codes[k];
+        if (c == QND_OBS_NORM) out[off++] = sums[0];
+        else if (c == QND_OBS_DIPOLE_R)
+            for (int j = 0; j < dim; ++j) out[off++] = sums[1 + j];
+        else if (c == QND_OBS_DIPOLE_A)
+            for (int j = 0; j < dim; ++j) out[off++] = -sums[1 + QND_MAXD + j] - spec.E[j] * sums[0];
+    }
+}
+
+template <typename R>
+int observables_noalloc(qnd_ctx *c, const void *psi_dev, const ObsSpec &spec, double *out_dev)
+{
+    double *scr = (c->device >= 0 && c->device < QND_MAX_DEVICES) ? g_noalloc_ptr[c->device] : nullptr;
+    if (!scr)
+        return fail(c, QND_EHIP, "device scratch of the allocation-free entry point unavailable on device %d (%s)",
+                    c->device,
+                    c->device >= 0 && c->device < QND_MAX_DEVICES ? hipGetErrorString(g_noalloc_err[c->device])
+                                                                  : "device index out of range");
+    const int grid = flat_grid(c->N, 8, 512, 8192), npart = QND_NOBS;
+    if ((size_t)grid * npart + QND_NOBS > (size_t)QND_NOALLOC_DOUBLES)
+        return fail(c, QND_EINVAL, "%d partial rows exceed the %d rows of the allocation-free scratch", grid,
+                    QND_NOALLOC_ROWS);
+    double *part = scr, *sums = scr + (size_t)QND_NOALLOC_ROWS * QND_NOBS;
+    hipLaunchKernelGGL(k_red<R>, dim3(grid), dim3(256), 0, c->stream, (int)RM_OBS, (const C<R> *)psi_dev,
+                       (C<R> *)nullptr, (const C<R> *)nullptr, 0, 0.0, c->N, c->P, (const double *)nullptr, part,
+                       npart);
+    HIPCK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_sum_partials, dim3((QND_NOBS + 3) / 4), dim3(256), 0, c->stream, (const double *)part,
+                       grid, npart, QND_NOBS, c->P.dNr, sums);
+    HIPCK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_obs_assemble, dim3(1), dim3(64), 0, c->stream, (const double *)sums, spec, c->dim,
+                       out_dev);
+    HIPCK(c, hipGetLastError());
+    return 0;
+}
+
 bool potential_params(const char *name, int *kind, double *b, double *c2)
 {
     *kind = 0;
@@ -2010,6 +2088,7 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
         return fail(nullptr, QND_EHIP, "no HIP device available");
     }
     hipGetDevice(&c->device);
+    noalloc_setup(c->device);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
         // dynamic LDS per workgroup, less room for the kernels' static arrays
@@ -2299,6 +2378,27 @@ int qnd_observables(qnd_ctx *c, int slot, const int *codes, int ncodes, const do
     rc = DISPATCH(c, observables, c, slot, codes, ncodes, alpha, Efield, out);
     prof_flush(c);
     return rc;
+}
+
+int qnd_observables_noalloc(qnd_ctx *c, const void *psi_dev, const int *codes, int ncodes, const double *Efield,
+                            double *out_dev)
+{
+    if (!c) return QND_EINVAL;
+    if (!c->have_pot) return fail(c, QND_ESTATE, "potential not set (qnd_set_potential)");
+    if (!psi_dev || !out_dev || (ncodes > 0 && !codes)) return fail(c, QND_EINVAL, "null state, output or codes");
+    if (ncodes < 0 || ncodes > 8) return fail(c, QND_EINVAL, "between 0 and 8 observables per call");
+    ObsSpec spec;
+    memset(&spec, 0, sizeof spec);
+    spec.ncodes = ncodes;
+    for (int k = 0; k < ncodes; ++k) {
+        if (codes[k] != QND_OBS_NORM && codes[k] != QND_OBS_DIPOLE_R && codes[k] != QND_OBS_DIPOLE_A)
+            return fail(c, QND_EUNSUPPORTED, "observable code %d: the allocation-free entry point records norm, "
+                        "dipole_r and dipole_a only", codes[k]);
+        spec.codes[k] = codes[k];
+    }
+    for (int j = 0; j < c->dim; ++j) spec.E[j] = Efield ? Efield[j] : 0.0;
+    if (ncodes == 0) return 0;
+    return DISPATCH(c, observables_noalloc, c, psi_dev, spec, out_dev);
 }
 
 int qnd_set_field(qnd_ctx *c, int64_t ntime, const double *Et, const double *At, const double *Eobs,
