@@ -89,6 +89,13 @@ struct or_ctx {
 /* ------------------------------------------------------------------ */
 /* Threads                                                              */
 /* ------------------------------------------------------------------ */
+/* Pointwise loops in parallel (1, default) or serial (0).  The reference
+ * has no parallel pointwise loop: its only threads are FFTW's
+ * (src/meshgrid.c:159-161), so a single-rank run is FFT-threaded with
+ * serial pointwise passes. */
+static int g_pw_parallel = 1;
+void or_set_pointwise_parallel(int on) { g_pw_parallel = on != 0; }
+
 void or_set_threads(int nthreads)
 {
 #ifdef _OPENMP
@@ -485,19 +492,19 @@ void or_step_imag(or_ctx *c, double *psi_d)
     for (int ia = 0; ia < c->s; ia++) {
         double a0 = c->as_scaled[2 * ia], a1 = c->as_scaled[2 * ia + 1];
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (g_pw_parallel)
 #endif
         for (int64_t i = 0; i < N; i++)
             c->in[i] = exp(clampexp(-a0 * c->Vpot[i])) * psi[i];
         fwd(c);
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (g_pw_parallel)
 #endif
         for (int64_t i = 0; i < N; i++)
             c->in[i] = exp(clampexp(-(a0 + a1) * c->Kin[i])) * c->out[i];
         bwd(c);
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (g_pw_parallel)
 #endif
         for (int64_t i = 0; i < N; i++)
             psi[i] = exp(clampexp(-a1 * c->Vpot[i])) * c->out[i] * c->fft_scaling;
@@ -512,13 +519,13 @@ void or_step_length(or_ctx *c, double *psi_d, const double *Et)
     for (int ia = 0; ia < c->s; ia++) {
         double a0 = c->as_scaled[2 * ia], a1 = c->as_scaled[2 * ia + 1];
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (g_pw_parallel)
 #endif
         for (int64_t i = 0; i < N; i++)
             c->in[i] = cexp(-I * a0 * c->Kin[i]) * psi[i];
         bwd(c);
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (g_pw_parallel)
 #endif
         for (int64_t i = 0; i < N; i++) {
             double W = 0.;
@@ -528,7 +535,7 @@ void or_step_length(or_ctx *c, double *psi_d, const double *Et)
         }
         fwd(c);
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (g_pw_parallel)
 #endif
         for (int64_t i = 0; i < N; i++)
             psi[i] = cexp(-I * a1 * c->Kin[i]) * c->out[i] * c->fft_scaling;
@@ -543,7 +550,7 @@ void or_step_velocity(or_ctx *c, double *psi_d, const double *At)
     for (int ia = 0; ia < c->s; ia++) {
         double a0 = c->as_scaled[2 * ia], a1 = c->as_scaled[2 * ia + 1];
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (g_pw_parallel)
 #endif
         for (int64_t i = 0; i < N; i++) {
             double arg = clampexp(-a0 * c->Vcap[i]);
@@ -551,7 +558,7 @@ void or_step_velocity(or_ctx *c, double *psi_d, const double *At)
         }
         fwd(c);
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (g_pw_parallel)
 #endif
         for (int64_t i = 0; i < N; i++) {
             double W = 0.;
@@ -563,7 +570,7 @@ void or_step_velocity(or_ctx *c, double *psi_d, const double *At)
         }
         bwd(c);
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (g_pw_parallel)
 #endif
         for (int64_t i = 0; i < N; i++) {
             double arg = clampexp(-a1 * c->Vcap[i]);

@@ -37,6 +37,9 @@ or_ctx *or_create(int dim, const int64_t *nr, const double *dr,
                   const char *integrator, double dt, char *err, int errlen);
 void or_destroy(or_ctx *c);
 void or_set_threads(int nthreads);         /* 0 = OpenMP default */
+/* 1 (default): pointwise loops in parallel; 0: serial, the FFT alone
+ * threaded (the reference's single-rank threading, src/meshgrid.c:159-161). */
+void or_set_pointwise_parallel(int on);
 
 /* Accessors (for tests). */
 int64_t or_npoints(const or_ctx *c);

@@ -15,7 +15,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libqnd.so")
+# QND_LIBRARY selects another build of the library (e.g. a variant built
+# with different compile-time options); default: the in-tree libqnd.so
+LIB_PATH = os.environ.get("QND_LIBRARY") or os.path.join(HERE, "libqnd.so")
 BIN_DIR = os.path.join(HERE, "bin")
 
 C128, C64 = 0, 1

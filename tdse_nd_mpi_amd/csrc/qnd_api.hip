@@ -80,6 +80,7 @@ struct qnd_ctx {
     int tw_override[2] = {0, 0}; // lines per workgroup (contiguous, strided); QND_TW env
     int vpt_override = 0;        // values per thread (4 or 8); QND_VPT env
     int wave_lines = 1;          // wavefront-local exchanges for 64-thread lines; QND_WAVE_LINES=0 disables
+    bool round32 = false;        // QND_ROUND_STATE=1: complex128 passes store complex64-rounded states
     std::vector<SlotState> slots;
     AxisPlan plan[QND_MAXD];
     std::vector<double> Et, At, Eobs, Aobs;
@@ -110,6 +111,10 @@ struct qnd_ctx {
     // boundary), E(t_n) on the device, and the ring of saved record states
     void *d_atab = nullptr;
     int64_t atab_gen = -1;
+    // FP64 potential at every element (four-value passes), rebuilt when the
+    // potential changes
+    double *d_vtab = nullptr;
+    int64_t pot_gen = 0, vtab_gen = -1;
     int atab_prec = -1;
     double *d_Eo = nullptr;
     void *d_side = nullptr;
@@ -438,12 +443,13 @@ void launch_vpt(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, hipEvent_t e0, h
     // line tiles (whose order and XCD placement stay as they are)
     const int extra = (KIND == PK_KIN && (a.flags & PF_FINALIZE)) ? a.B.nb : 0;
     const dim3 grid(g.grid + extra);
+    const unsigned fl = a.flags | (c->round32 ? (unsigned)PF_ROUND32 : 0u);
     if (e0)
         hipExtLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), grid, dim3(g.threads), g.lds, c->stream, e0, e1, 0,
-                              a.data, a.tw, a.S, geo, a.flags, (int)g.grid, a);
+                              a.data, a.tw, a.S, geo, fl, (int)g.grid, a);
     else
         hipLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), grid, dim3(g.threads), g.lds, c->stream, a.data, a.tw,
-                           a.S, geo, a.flags, (int)g.grid, a);
+                           a.S, geo, fl, (int)g.grid, a);
 }
 
 template <typename R, int KIND, int DIR>
@@ -462,9 +468,36 @@ int launch(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, int cls)
     return 0;
 }
 
-// axis-0 pass with r-space operators (IFFT first)
-template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R> &a)
+// The FP64 potential table of the latency-bound grids (four values per
+// thread): read with the state at kernel entry instead of evaluating sqrt and
+// exp in the pass.  Not built above QND_VTAB_MAX_BYTES (those grids run eight
+// values per thread anyway).
+#define QND_VTAB_MAX_BYTES ((size_t)256 << 20)
+const double *vtab_for(qnd_ctx *c, int *rc)
 {
+    *rc = 0;
+    if ((size_t)c->N * sizeof(double) > QND_VTAB_MAX_BYTES || !c->have_pot) return nullptr;
+    if (c->vtab_gen == c->pot_gen && c->d_vtab) return c->d_vtab;
+    if (!c->d_vtab) HIPCK_P(c, hipMalloc(&c->d_vtab, (size_t)c->N * sizeof(double)), rc);
+    {
+        ProfScope ps(c, PC_SMALL);
+        hipLaunchKernelGGL(k_vtab, dim3(flat_grid(c->N, 4, 64, 4096)), dim3(256), 0, c->stream, c->d_vtab, c->N,
+                           c->P);
+        HIPCK_P(c, hipGetLastError(), rc);
+    }
+    c->vtab_gen = c->pot_gen;
+    return c->d_vtab;
+}
+
+// axis-0 pass with r-space operators (IFFT first)
+template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R> &a_in)
+{
+    PassArgs<R> a = a_in;
+    if (g.vpt == 4 && !a.vtab) {
+        int rc;
+        a.vtab = vtab_for(c, &rc);
+        if (rc) return rc;
+    }
     if (c->dim == 1) {
         if (a.flags == (PF_V1 | PF_V2 | PF_SECOND | PF_KPOST) && a.vmode == VM_IMAG && a.kmode == KM_IMAG)
             return launch<R, PK_P1DI, +1>(c, g, a, PC_POT);
@@ -2132,6 +2165,7 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
         }
     }
     if (const char *e = getenv("QND_WAVE_LINES")) c->wave_lines = atoi(e) != 0;
+    if (const char *e = getenv("QND_ROUND_STATE")) c->round32 = precision == QND_C128 && atoi(e) != 0;
     if (const char *e = getenv("QND_VPT")) {  // tuning knob: values per thread
         const int v = atoi(e);
         if (v == 4 || v == 8) c->vpt_override = v;
@@ -2210,6 +2244,7 @@ void qnd_destroy(qnd_ctx *c)
     if (c->d_Eo) hipFree(c->d_Eo);
     if (c->d_side) hipFree(c->d_side);
     if (c->d_kcoef) hipFree(c->d_kcoef);
+    if (c->d_vtab) hipFree(c->d_vtab);
     for (void *p : {c->d_adj_ckpt, c->d_adj_final, c->d_adj_save, (void *)c->d_adj_part})
         if (p) hipFree(p);
     for (int k = 0; k < 16; ++k)
@@ -2238,6 +2273,7 @@ int qnd_set_potential(qnd_ctx *c, const char *name, double softcore, const doubl
     c->P.pot_kind = kind;
     c->P.pot_b = b;
     c->P.pot_c2 = c2;
+    ++c->pot_gen;
     // separable CAP: Vcap = sum_j cap0 (|x_j| - L_j)^2 for |x_j| >= L_j,
     // L_j = n_j dr_j / 2 - width (src/meshgrid.c:132-136)
     const double cap0 = cap ? cap[0] : 0.0, cap1 = cap ? cap[1] : 0.0;

@@ -487,6 +487,28 @@ template <bool SOFT, bool HASB> struct PotPoint<float, SOFT, HASB> {
         }
     }
 };
+// FP64 potential for the complex64 passes (QND_POT_F64): 1/sqrt(u + c2)
+// from an FP32 seed refined by two FP64 Newton steps (relative error ~1e-14,
+// well below the FP32 state's rounding) instead of FP64 sqrt and divide;
+// the exponentials in FP64.
+template <bool SOFT, bool HASB> struct PotPointMix {
+    static __device__ __forceinline__ double eval(const Phys &P, double rr, double &dVdu)
+    {
+        if constexpr (SOFT) {
+            const double w = rr + P.pot_c2;
+            double y = (double)rsqrtf((float)w);
+            y = y * (1.5 - 0.5 * w * y * y);
+            y = y * (1.5 - 0.5 * w * y * y);
+            const double g = HASB ? P.pot_b * exp(-rr) : 0.0;
+            dVdu = g * y + .5 * (1. + g) * (y * y * y);
+            return -(1. + g) * y;
+        } else {
+            const double g = 2.7459 * exp(-.5 * rr / 1.96);
+            dVdu = g * .5 / 1.96;
+            return -g;
+        }
+    }
+};
 template <typename R> __device__ __forceinline__ R potential(const Phys &P, double rr, R &dVdu)
 {
     if (P.pot_kind != 0) return PotPoint<R, false, false>::eval(P, rr, dVdu);
@@ -495,10 +517,23 @@ template <typename R> __device__ __forceinline__ R potential(const Phys &P, doub
 }
 // V and dV/du at the VPT points of a thread, rr[m] = r^2: the family is
 // chosen once, outside the loop, so the points form independent chains.
-template <typename R, int VPT>
+// MIX: FP64 values for a complex64 pass (PotPointMix), else the state's
+// precision (PotPoint).
+template <typename R, int VPT, bool MIX = false>
 __device__ __forceinline__ void potential_pts(const Phys &P, const double (&rr)[VPT], R (&V)[VPT], R (&dVdu)[VPT])
 {
-    if (P.pot_kind != 0) {
+    if constexpr (MIX) {
+        if (P.pot_kind != 0) {
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) V[m] = PotPointMix<false, false>::eval(P, rr[m], dVdu[m]);
+        } else if (P.pot_b != 0.0) {
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) V[m] = PotPointMix<true, true>::eval(P, rr[m], dVdu[m]);
+        } else {
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) V[m] = PotPointMix<true, false>::eval(P, rr[m], dVdu[m]);
+        }
+    } else if (P.pot_kind != 0) {
 #pragma unroll
         for (int m = 0; m < VPT; ++m) V[m] = PotPoint<R, false, false>::eval(P, rr[m], dVdu[m]);
     } else if (P.pot_b != 0.0) {
@@ -625,10 +660,28 @@ __device__ __forceinline__ Fac<float> phase_factor_f32(float arg, float ph)
     const float gx = exp2f((arg > (float)QND_MAX_EXP_ARG_F32 ? (float)QND_MAX_EXP_ARG_F32 : arg) * 1.44269504f);
     return fac_octant_f32(rd, (int)j, (arg == 0.0f) ? 1.0f : gx);
 }
-// Potential factor e^{damp} (cos ph - i sin ph) with the angle formed in the
-// precision of the state.
+// Potential factor e^{damp} (cos ph - i sin ph).  complex128: FP64
+// throughout.  complex64: the damping in FP32; the angle either formed and
+// reduced mod pi/2 in FP64 (QND_POT_ANGLE_FP64, the default: c h (V + x.E)
+// with V in FP32, the product, sum and reduction in FP64, sin and cos from
+// the FP32 polynomials of the octant) or formed and reduced in FP32.
+#ifndef QND_POT_ANGLE_FP64
+#define QND_POT_ANGLE_FP64 1
+#endif
+// QND_POT_F64: the complex64 passes evaluate V and dV/du in FP64 (else FP32)
+#ifndef QND_POT_F64
+#define QND_POT_F64 1
+#endif
 __device__ __forceinline__ Fac<double> pot_factor(double damp, double ph) { return phase_factor<double>(damp, ph); }
 __device__ __forceinline__ Fac<float> pot_factor(float damp, float ph) { return phase_factor_f32(damp, ph); }
+__device__ __forceinline__ Fac<float> pot_factor(float damp, double ph)
+{
+    const double j = rint(ph * 0.63661977236758134307553505349006);          // 2/pi
+    double rd = fma(-j, 1.5707963267948965579989817342721, ph);               // pi/2 hi
+    rd = fma(-j, 6.1232339957367660359e-17, rd);                             // pi/2 lo
+    const float gx = exp2f((damp > (float)QND_MAX_EXP_ARG_F32 ? (float)QND_MAX_EXP_ARG_F32 : damp) * 1.44269504f);
+    return fac_octant_f32((float)rd, (int)j, (damp == 0.0f) ? 1.0f : gx);
+}
 // A tabulated unit factor (cos, -sin) as a Fac: complex64 tables hold the
 // FP64 value as an FP32 hi + lo pair (x, y, z, w) = (hi.re, hi.im, lo.re,
 // lo.im); conj gives the adjoint's conjugated factor.
@@ -729,6 +782,14 @@ __device__ __forceinline__ double wave_sum_column(const double *part, int nwg, i
     return wave_sum(s);
 }
 
+// complex128 value rounded to complex64 and back (PF_ROUND32); identity in
+// complex64
+template <typename R> __device__ __forceinline__ C<R> round_to_c64(C<R> v)
+{
+    if constexpr (std::is_same<R, double>::value) return mk<double>((double)(float)v.x, (double)(float)v.y);
+    else return v;
+}
+
 // ---------------------------------------------------------------------
 // pass arguments
 // ---------------------------------------------------------------------
@@ -788,6 +849,8 @@ enum PassFlags : unsigned {
     PF_ZERO = 1u << 19,      // with PF_SRC: the costate is zero (the pass forms the source term alone)
     PF_POSTKX = 1u << 20,    // axis-0 pass: after the second transform, the conjugated axis-0 factor
                              // from `atab`, inverse, [axis-0 mask], forward (adjoint of PF_PREKX)
+    PF_ROUND32 = 1u << 21,   // complex128 passes: round every stored state value to complex64 (a
+                             // precision experiment: FP64 arithmetic with complex64 storage)
 };
 enum KMode { KM_PHASE = 0, KM_VELOCITY = 1, KM_IMAG = 2 };
 enum VMode { VM_LENGTH = 0, VM_VELOCITY = 1, VM_IMAG = 2 };
@@ -838,6 +901,9 @@ template <typename R> struct PassArgs {
     // per-axis kinetic factors e^{-i c K_j} of one coefficient (the step's
     // first, a_0), Sigma n_j entries at the per-axis offsets (PK_KINS, PF_PREKX)
     const typename TwT<R>::T *atab;
+    // V(r) in FP64 at every element of one instance (row-major), when
+    // tabulated (four values per thread); nullptr: evaluated in the pass
+    const double *vtab;
     // adjoint passes: forward states saved in position space, source weights
     C<R> *save;
     double src[QND_MAXD];
@@ -1038,6 +1104,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                       : (KIND == PK_KINI || KIND == PK_P1DI)                  ? (int)KM_IMAG
                       : KIND == PK_KINV                                       ? (int)KM_VELOCITY
                                                                               : a.kmode;
+    // complex128 only: the stored state rounded to complex64 (experiment switch)
+    const bool r32 = std::is_same<R, double>::value && (fl_arg & PF_ROUND32) != 0;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[16 * QND_NOBS];
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
@@ -1174,6 +1242,17 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             } else {
 #pragma unroll
                 for (int j = 0; j < QND_MAXD; ++j) fv[j] = (j < P.dim) ? (use_a ? a.Ak[j] : a.Ev[j]) : 0.0;
+            }
+        }
+
+        // tabulated potential of this thread's elements (four values per
+        // thread), requested before the transform
+        double vtv[VPT];
+        if constexpr (POTK && VPT == 4) {
+            if (a.vtab) {
+                const int64_t ib = lbase - (L / a.B.lpi) * ((int64_t)a.B.lpi * n);  // instance-relative
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) vtv[m] = a.vtab[ib + (int64_t)(t + m * T) * estride];
             }
         }
 
@@ -1336,6 +1415,9 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             // FP64 for complex128 and FP32 for complex64 (a thread sums its VPT
             // points in obr, which then enter the FP64 workgroup reduction).
             constexpr bool F64 = std::is_same<R, double>::value;
+            // precision of the potential: the state's, or FP64 in complex64
+            // too (QND_POT_F64)
+            using PR = typename std::conditional<F64 || QND_POT_F64, double, R>::type;
             R obr[QND_NOBS];
 #pragma unroll
             for (int q = 0; q < QND_NOBS; ++q) {
@@ -1347,6 +1429,13 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                 else obr[q] += val;
             };
             const R cv1 = (R)a.cv1, cv2 = (R)a.cv2, csr = (R)cs, fva = (R)fv[ax], Wvr = (R)Wv;
+            // potential factor of damping `damp` and angle c * arg: the angle
+            // in FP64 (argd, cd) or, for the FP32-angle build of complex64, in
+            // FP32 (argr, cr)
+            auto pf = [&](R damp, R cr, double cd, R argr, double argd) -> Fac<R> {
+                if constexpr (F64 || QND_POT_ANGLE_FP64) return pot_factor(damp, cs * cd * argd);
+                else return pot_factor(damp, csr * cr * argr);
+            };
             R xor_[QND_MAXD];
 #pragma unroll
             for (int j = 0; j < QND_MAXD; ++j) xor_[j] = (R)xo[j];
@@ -1362,7 +1451,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             for (int m0 = 0; m0 < VPT; m0 += G) {
                 C<R> v[G];
                 double xc[G], rr[G];
-                R xr[G], V[G], dVdu[G], Vc[G];
+                R xr[G], Vc[G];
+                PR V[G], dVdu[G];
 #pragma unroll
                 for (int i = 0; i < G; ++i) {
                     const int m = m0 + i;
@@ -1370,11 +1460,20 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                     xc[i] = coord_x(P, ax, t + m * T);
                     xr[i] = (R)xc[i];
                     rr[i] = xc[i] * xc[i] + rro;
-                    V[i] = (R)0;
-                    dVdu[i] = (R)0;
+                    V[i] = (PR)0;
+                    dVdu[i] = (PR)0;
                     Vc[i] = P.cap_on ? (R)(cap_at(m) + Vco) : (R)0;
                 }
-                if (need_v) potential_pts<R, G>(P, rr, V, dVdu);
+                // V from the table where only V is needed; V and dV/du from
+                // the closed forms where the observables or the adjoint need
+                // the derivative (the same FP64 arithmetic as the table's)
+                const bool need_dv = (fl & (PF_OBS | PF_SRC | PF_GRAD)) != 0;
+                if (VPT == 4 && a.vtab && !need_dv) {
+#pragma unroll
+                    for (int i = 0; i < G; ++i) V[i] = (PR)vtv[m0 + i];
+                } else if (need_v) {
+                    potential_pts<PR, G, !F64 && QND_POT_F64>(P, rr, V, dVdu);
+                }
                 if (fl & (PF_GRAD | PF_SRC)) {
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
@@ -1418,20 +1517,23 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                         if (P.cap_on) {
 #pragma unroll
                             for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i], pot_factor(-cv1 * Vc[i], csr * cv1 * (V[i] + (xr[i] * fva + Wvr))));
+                                v[i] = fac_mul<R>(v[i], pf(-cv1 * Vc[i], cv1, a.cv1, V[i] + (xr[i] * fva + Wvr),
+                                                           (double)V[i] + (xc[i] * fv[ax] + Wv)));
                         } else {
 #pragma unroll
                             for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i], pot_factor((R)0, csr * cv1 * (V[i] + (xr[i] * fva + Wvr))));
+                                v[i] = fac_mul<R>(v[i], pf((R)0, cv1, a.cv1, V[i] + (xr[i] * fva + Wvr),
+                                                           (double)V[i] + (xc[i] * fv[ax] + Wv)));
                         }
                     } else {
                         if (P.cap_on) {
 #pragma unroll
                             for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i], pot_factor(-cv1 * Vc[i], csr * cv1 * V[i]));
+                                v[i] = fac_mul<R>(v[i], pf(-cv1 * Vc[i], cv1, a.cv1, V[i], (double)V[i]));
                         } else {
 #pragma unroll
-                            for (int i = 0; i < G; ++i) v[i] = fac_mul<R>(v[i], pot_factor((R)0, csr * cv1 * V[i]));
+                            for (int i = 0; i < G; ++i)
+                                v[i] = fac_mul<R>(v[i], pf((R)0, cv1, a.cv1, V[i], (double)V[i]));
                         }
                     }
                 }
@@ -1465,10 +1567,11 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
                         }
                     } else if (P.cap_on) {
 #pragma unroll
-                        for (int i = 0; i < G; ++i) v[i] = fac_mul<R>(v[i], pot_factor(-cv2 * Vc[i], csr * cv2 * V[i]));
+                        for (int i = 0; i < G; ++i)
+                            v[i] = fac_mul<R>(v[i], pf(-cv2 * Vc[i], cv2, a.cv2, V[i], (double)V[i]));
                     } else {
 #pragma unroll
-                        for (int i = 0; i < G; ++i) v[i] = fac_mul<R>(v[i], pot_factor((R)0, csr * cv2 * V[i]));
+                        for (int i = 0; i < G; ++i) v[i] = fac_mul<R>(v[i], pf((R)0, cv2, a.cv2, V[i], (double)V[i]));
                     }
                 }
                 if (fl & PF_SAVE) {
@@ -1533,7 +1636,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
             // first kinetic factor
             if (fl & PF_SAVE) {
 #pragma unroll
-                for (int m = 0; m < VPT; ++m) a.save[lbase + (int64_t)(t + m * T) * estride] = x[m];
+                for (int m = 0; m < VPT; ++m)
+                    a.save[lbase + (int64_t)(t + m * T) * estride] = r32 ? round_to_c64<R>(x[m]) : x[m];
             }
             if ((fl & PF_MASK) && P.mask_on) {
 #pragma unroll
@@ -1550,6 +1654,10 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
 
     // ---- store ----
     if (!(POTK && (fl & PF_NOSTORE))) {
+        if (r32) {
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) x[m] = round_to_c64<R>(x[m]);
+        }
 #pragma unroll
         for (int m = 0; m < VPT; ++m) data[lbase + (int64_t)(t + m * T) * estride] = x[m];
     }
@@ -1730,6 +1838,26 @@ __global__ void __launch_bounds__(256) k_ktab(T *tab, const double *__restrict__
             sincos(coef[q] * K, &sn, &cn);
             tab_store(tab + (int64_t)q * N + f, cn, -sn);
         }
+    }
+}
+
+// Potential table: V at every element of one instance, FP64, from the same
+// closed forms (PotPoint<double>) as the passes.
+__global__ void __launch_bounds__(256) k_vtab(double *tab, int64_t N, Phys P)
+{
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < N; f += (int64_t)gridDim.x * blockDim.x) {
+        int idx[QND_MAXD];
+        flat_index(P, f, idx);
+        double rr = 0.0;
+#pragma unroll
+        for (int j = 0; j < QND_MAXD; ++j) {
+            if (j < P.dim) {
+                const double x = coord_x(P, j, idx[j]);
+                rr += x * x;
+            }
+        }
+        double dVdu;
+        tab[f] = potential<double>(P, rr, dVdu);
     }
 }
 

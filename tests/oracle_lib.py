@@ -32,6 +32,7 @@ def lib():
                                 C.c_char_p, C.c_double, C.c_char_p, C.c_int]
         L.or_destroy.argtypes = [C.c_void_p]
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_pointwise_parallel.argtypes = [C.c_int]
         L.or_npoints.restype = C.c_int64
         L.or_npoints.argtypes = [C.c_void_p]
         L.or_stages.argtypes = [C.c_void_p]

@@ -472,7 +472,13 @@ def test_batched_real_steps(gpu, nr, gauge, prec):
         # complex64 agrees to its own rounding, complex128 to FP64 rounding
         tol = 1e-12 if prec == "c128" else TOL64
         assert rel(got[b], s.download(0)) < tol, (b, rel(got[b], s.download(0)))
-        assert np.allclose(obs[b], one, rtol=tol, atol=tol * 1e-2), (b, np.abs(obs[b] - one).max())
+        if prec == "c128":
+            assert np.allclose(obs[b], one, rtol=tol, atol=tol * 1e-2), (b, np.abs(obs[b] - one).max())
+        else:
+            # two FP32 rounding sequences: compared at each record's scale (a
+            # component crossing zero has no relative accuracy of its own)
+            err = np.abs(obs[b] - one) / (np.abs(one).max(axis=0) + 1e-300)
+            assert err.max() < tol, (b, err.max(axis=0))
         s.close()
     if prec == "c128":
         o.set_mask(0.1)
