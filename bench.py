@@ -142,9 +142,20 @@ def sweep_ground_state(q, n):
     return psi, E
 
 
-def sweep_run(q, F, steps, warmup):
+def sweep_run(q, F, steps, warmup, compare_steps=1000):
+    """The 96-instance sweep, timed over `steps` steps from the ground states
+    (steps <= 0: the whole pulse, 6619 steps) in two forms:
+      one_launch: every pass of a step is one kernel launch over all three
+        size classes (qnd_group_real_steps), the steps after the first a
+        replayed HIP graph (also timed without the graph over compare_steps);
+      three_contexts: one batched context per size class (one launch per
+        pass per class), the three on three host threads.
+    `value` is the faster form."""
     import threading
-    ceps, ints, tabs, ntime_full = sweep_fields(F, steps + warmup + 8)
+    import torch
+    _, _, _, ntime_full = sweep_fields(F, 1)
+    nsteps = ntime_full if steps <= 0 else steps
+    ceps, ints, tabs, _ = sweep_fields(F, max(nsteps, compare_steps) + warmup + 8)
     names = ["dipole_r", "dipole_a"]
     classes = []
     for k, n in enumerate(SWEEP_SIZES):
@@ -159,32 +170,66 @@ def sweep_run(q, F, steps, warmup):
             g.upload(b, psi0)
         from tdse_nd_mpi_amd import tuning
         best, table = tuning.autotune_batch(g, 0, SWEEP_PER, "length", 0, nsteps=4)
-        g.real_steps_batch(0, SWEEP_PER, "length", 0, warmup, names)
-        classes.append(dict(n=n, grid=g, E0=E0, best=best, table=table))
-    out = [None] * len(classes)
+        classes.append(dict(n=n, grid=g, psi0=psi0, E0=E0, best=best, table=table,
+                            cached=getattr(g, "tuning_cached", False)))
+    grids = [c["grid"] for c in classes]
 
-    def work(i):
-        out[i] = classes[i]["grid"].real_steps_batch(0, SWEEP_PER, "length", warmup, steps, names)
+    def reset():
+        for c in classes:
+            for b in range(SWEEP_PER):
+                c["grid"].upload(b, c["psi0"])
 
-    import torch
-    torch.cuda.synchronize()
-    th = [threading.Thread(target=work, args=(i,)) for i in range(len(classes))]
-    t0 = time.perf_counter()
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    wall = time.perf_counter() - t0
+    def three_contexts(n):
+        res3 = [None] * len(classes)
+
+        def work(i):
+            res3[i] = classes[i]["grid"].real_steps_batch(0, SWEEP_PER, "length", 0, n, names)
+
+        th = [threading.Thread(target=work, args=(i,)) for i in range(len(classes))]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return time.perf_counter() - t0, res3
+
     pts = sum(SWEEP_PER * c["n"] ** 2 for c in classes)
-    value = pts * steps / wall
-    res = {"metric": "grid-point-steps/s (all 96 instances)", "value": value, "wall_s": wall, "steps": steps,
+    q.group_real_steps(grids, "length", 0, warmup, names, use_graph=True)      # warm-up, then from t = 0
+    reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out1 = q.group_real_steps(grids, "length", 0, nsteps, names, use_graph=True)
+    wall1 = time.perf_counter() - t0
+    reset()
+    t0 = time.perf_counter()
+    q.group_real_steps(grids, "length", 0, compare_steps, names, use_graph=False)
+    wall_ng = time.perf_counter() - t0
+    reset()
+    three_contexts(warmup)
+    reset()
+    wall3, out3 = three_contexts(nsteps)
+    v1, v3 = pts * nsteps / wall1, pts * nsteps / wall3
+    value = max(v1, v3)
+    agree = max(float(np.abs(a - b).max() / (np.abs(b).max() + 1e-300)) for a, b in zip(out1, out3))
+    res = {"metric": "grid-point-steps/s (all 96 instances)", "value": value,
+           "form": "one_launch" if v1 >= v3 else "three_contexts", "steps": nsteps,
            "steps_full_pulse": ntime_full, "points_per_step": pts, "seed": SWEEP_SEED,
            "step_roofline_frac": 14 * 2 * 8 * value / (HBM_PEAK_GBS * 1e9),
+           "one_launch": {"value": v1, "wall_s": wall1,
+                          "launch": "one kernel launch per pass over all 96 instances of the three sizes "
+                                    "(qnd_group_real_steps); steps after the first: one captured HIP graph "
+                                    "replayed per step",
+                          "without_graph": {"steps": compare_steps, "value": pts * compare_steps / wall_ng}},
+           "three_contexts": {"value": v3, "wall_s": wall3,
+                              "launch": "one batched context per size (one launch per pass per size), three "
+                                        "host threads"},
+           "records_max_rel_diff_between_forms": agree,
            "classes": [{"grid": [c["n"], c["n"]], "instances": SWEEP_PER, "E0": c["E0"],
-                        "tuned_vpt_twc_tws": list(c["best"]),
+                        "tuned_vpt_twc_tws": list(c["best"]), "tuning_cached": bool(c["cached"]),
                         "tuning_ms_per_step": {str(list(k)): round(v, 4) for k, v in c["table"]}}
                        for c in classes],
-           "finite": bool(all(np.isfinite(o).all() for o in out))}
+           "finite": bool(all(np.isfinite(o).all() for o in out1 + out3))}
     for c in classes:
         c["grid"].close()
     return res
@@ -607,8 +652,9 @@ def main():
                     help="skip the untimed whole-pulse precision leg (complex64 against complex128)")
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--sweep-steps", type=int, default=2000,
-                    help="steps of the 96-instance CEP/intensity sweep (0 skips it)")
+    ap.add_argument("--sweep-steps", type=int, default=-1,
+                    help="timed steps of the 96-instance CEP/intensity sweep (-1: the whole 6619-step pulse; "
+                         "0 skips it)")
     ap.add_argument("--adjoint-steps", type=int, default=ADJ_STEPS,
                     help="steps of the adjoint-gradient workload (0 skips it)")
     ap.add_argument("--fd-samples", type=int, default=16)
@@ -644,7 +690,7 @@ def main():
         runs = {"eig1d": lambda: eig1d_run(q),
                 "big3d": lambda: big3d_run(q, F, max(args.big3d_steps, 1), max(args.big3d_steps, 1), fixed3d),
                 "adjoint": lambda: adjoint_run(q, args.adjoint_steps, args.fd_samples),
-                "sweep": lambda: sweep_run(q, F, args.sweep_steps, 10)}
+                "sweep": lambda: sweep_run(q, F, args.sweep_steps, 10, compare_steps=200)}
         print(json.dumps({"workload": args.only, "result": runs[args.only]()}))
         return
     if args.traffic_probe:
@@ -785,8 +831,9 @@ def main():
         log(f"adjoint gradient: 256^2, {args.adjoint_steps} steps, FD on {args.fd_samples} samples ...")
         out["adjoint_256"] = adjoint_run(q, args.adjoint_steps, args.fd_samples)
         log(f"  adjoint/forward = {out['adjoint_256']['value']:.2f}")
-    if args.sweep_steps > 0:
-        log(f"CEP/intensity sweep: 96 instances, {args.sweep_steps} steps ...")
+    if args.sweep_steps != 0:
+        log(f"CEP/intensity sweep: 96 instances, {'the whole pulse' if args.sweep_steps < 0 else args.sweep_steps} "
+            f"steps, one launch per pass ...")
         out["sweep_96"] = sweep_run(q, F, args.sweep_steps, 10)
         log(f"  sweep: {out['sweep_96']['value']:.3e} grid-point-steps/s")
     print(json.dumps(out))

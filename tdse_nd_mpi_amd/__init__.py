@@ -31,7 +31,7 @@ EXPORTS = ("qnd_create", "qnd_destroy", "qnd_last_error", "qnd_set_potential", "
            "qnd_real_steps", "qnd_final_observables", "qnd_event_record", "qnd_event_elapsed",
            "qnd_synchronize", "qnd_profile_enable", "qnd_profile_read", "qnd_version",
            "qnd_set_field_batch", "qnd_real_steps_batch", "qnd_set_tuning", "qnd_forward_checkpointed",
-           "qnd_adjoint", "qnd_observables_noalloc")
+           "qnd_adjoint", "qnd_observables_noalloc", "qnd_group_real_steps")
 
 _lib = None
 
@@ -76,6 +76,8 @@ def library():
     L.qnd_forward_checkpointed.argtypes = [vp, C.c_int, C.c_int64, C.c_int64, dp, dp]
     L.qnd_adjoint.argtypes = [vp, C.c_int64, dp, C.c_double, dp, dp, dp]
     L.qnd_real_steps_batch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
+    L.qnd_group_real_steps.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int,
+                                       C.POINTER(dp), C.c_int]
     L.qnd_final_observables.argtypes = [vp, C.c_int, dp, dp, C.c_int, C.c_double, dp, dp]
     L.qnd_event_record.argtypes = [vp, C.c_int]
     L.qnd_event_elapsed.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]
@@ -321,6 +323,28 @@ class Grid:
         ms = np.zeros(5)
         self._ck(library().qnd_profile_read(self.h, cnt.ctypes.data_as(C.POINTER(C.c_int64)), _d(ms), 5))
         return cnt, ms
+
+
+def group_real_steps(grids, gauge, step0, nsteps, names=(), use_graph=True):
+    """qnd_group_real_steps: every grid's batch (its slots 0 .. nb-1 under its
+    set_field_batch tables) stepped together, one kernel launch per pass over
+    all grids; returns one [nb][nsteps][width] record array per grid."""
+    L = library()
+    codes = np.array([OBS[n] for n in names], dtype=np.int32)
+    outs, ptrs = [], []
+    for g in grids:
+        width = sum({5: 1, 3: g.dim, 6: g.dim}.get(int(c), 0) for c in codes)
+        o = np.zeros((g.batch, max(nsteps, 1), max(width, 1)))
+        outs.append((o, width))
+        ptrs.append(_d(o))
+    hs = (C.c_void_p * len(grids))(*[g.h.value if isinstance(g.h, C.c_void_p) else g.h for g in grids])
+    pp = (C.POINTER(C.c_double) * len(grids))(*ptrs)
+    rc = L.qnd_group_real_steps(hs, len(grids), GAUGES[gauge], int(step0), int(nsteps),
+                                codes.ctypes.data_as(C.POINTER(C.c_int)) if len(codes) else None, len(codes),
+                                pp, 1 if use_graph else 0)
+    if rc:
+        raise QNDError(L.qnd_last_error(grids[0].h).decode())
+    return [o[:, :nsteps, :w] for o, w in outs]
 
 
 def imag_step_count(T, dt):

@@ -115,6 +115,9 @@ struct qnd_ctx {
     // potential changes
     double *d_vtab = nullptr;
     int64_t pot_gen = 0, vtab_gen = -1;
+    // descriptors of the group launches this context leads (qnd_group_real_steps)
+    void *d_gdesc = nullptr;
+    size_t gdesc_cap = 0;
     int atab_prec = -1;
     double *d_Eo = nullptr;
     void *d_side = nullptr;
@@ -164,7 +167,7 @@ static void noalloc_setup(int device)
 
 namespace {
 
-const int SMALL_COEF = 0, SMALL_GRAM = 4096, SMALL_SUMS = 12288, SMALL_TOTAL = 16384;
+const int SMALL_COEF = 0, SMALL_GRAM = 4096, SMALL_SUMS = 12288, SMALL_STEP = 16000, SMALL_TOTAL = 16384;
 
 int fail(qnd_ctx *c, int code, const char *fmt, ...)
 {
@@ -1868,6 +1871,339 @@ int adj_backward(qnd_ctx *c, const double *dL_da, double dL_dnorm, double *dL_dE
     return 0;
 }
 
+// ---------------------------------------------------------------------
+// Several grids in one launch per pass (qnd_group_real_steps): the batched
+// length-gauge step of each member context, every pass launched once over
+// all of them (k_pass_group), the step sequence replayable through the
+// device step counter, optionally as a captured graph.
+// ---------------------------------------------------------------------
+struct GroupGeo {
+    std::vector<Geo> g;  // per member
+    int threads = 0;
+    size_t lds = 0;
+};
+
+// Launch geometry of a pass along `axis` for every member with one
+// workgroup size: each member's own geometry, then its lines per workgroup
+// rescaled to the largest member's thread count (halving it until every
+// member admits it).  False when no common size fits.
+bool group_geometry(qnd_ctx *const *cs, int n, int axis, int kind, GroupGeo &G)
+{
+    G.g.assign(n, Geo{});
+    int want = 64, tmax = 1;  // at least a wavefront; at least every member's line
+    for (int k = 0; k < n; ++k) {
+        G.g[k] = geometry(cs[k], axis, cs[k]->bnb, kind);
+        want = std::max(want, G.g[k].threads);
+        tmax = std::max(tmax, G.g[k].T);
+    }
+    want = pow2floor(2 * want - 1);  // the power of two at or above
+    for (int thr = std::min(want, QND_PASS_THREADS); thr >= tmax; thr /= 2) {
+        bool ok = true;
+        std::vector<Geo> out(G.g);
+        for (int k = 0; k < n && ok; ++k) {
+            Geo &g = out[k];
+            qnd_ctx *c = cs[k];
+            if (thr % g.T != 0) { ok = false; break; }
+            const int tw = thr / g.T;
+            const int64_t lpi = c->N / g.n;
+            if ((tw & (tw - 1)) != 0 || lpi % tw != 0 || (int64_t)tw > g.nlines || (g.S > 1 && (g.S % tw) != 0) ||
+                2 * (size_t)tw * (g.n + (tw > 1 ? 1 : 0)) * c->esz > c->lds_max) {
+                ok = false;
+                break;
+            }
+            g.TW = tw;
+            g.log2TW = ilog2(tw);
+            g.threads = g.T * tw;
+            g.grid = (int)(g.nlines / tw);
+            g.ldsStride = g.n + (tw > 1 ? 1 : 0);
+            g.lds = 2 * (size_t)tw * g.ldsStride * c->esz;
+        }
+        if (ok) {
+            G.g = out;
+            G.threads = thr;
+            G.lds = 0;
+            for (auto &g : G.g) G.lds = std::max(G.lds, g.lds);
+            return true;
+        }
+    }
+    return false;
+}
+
+// One recorded launch of a group step: a pass kernel with its descriptor
+// (an index into the call's descriptor array), or the step counter advance.
+struct GLaunch {
+    const void *fn;
+    int grid, threads;
+    size_t lds;
+    int desc;  // -1: k_step_advance
+};
+
+// Record a pass over every member: its descriptor goes to `descs` (uploaded
+// once per call), its launch to `out`.
+template <typename R, int KIND, int DIR>
+void record_group(const GroupGeo &G, const std::vector<PassArgs<R>> &as, std::vector<GroupArgs<R>> &descs,
+                  std::vector<GLaunch> &out)
+{
+    GroupArgs<R> ga;
+    memset(&ga, 0, sizeof ga);
+    const int n = (int)as.size();
+    ga.ncls = n;
+    int w = 0;
+    for (int k = 0; k < n; ++k) {
+        const Geo &g = G.g[k];
+        const int extra = (KIND == PK_KIN && (as[k].flags & PF_FINALIZE)) ? as[k].B.nb : 0;
+        ga.blk0[k] = w;
+        ga.nblk[k] = g.grid;
+        ga.vpt[k] = g.vpt;
+        ga.geo[k] = pass_geo(g.log2n, g.log2T, g.log2TW, as[k].P.pow2);
+        ga.a[k] = as[k];
+        w += (g.grid + extra + 7) / 8 * 8;  // the next class starts on a multiple of 8
+    }
+    ga.blk0[n] = w;
+    descs.push_back(ga);
+    out.push_back(GLaunch{(const void *)k_pass_group<R, KIND, DIR>, w, G.threads, G.lds, (int)descs.size() - 1});
+}
+
+template <typename R>
+int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, const int *codes, int ncodes,
+                     double *const *obs_out, int use_graph)
+{
+    qnd_ctx *c0 = cs[0];
+    const int d = c0->dim, s = c0->s;
+    int rc;
+    int width = 0;
+    for (int k = 0; k < ncodes; ++k) width += obs_width(codes[k], d);
+    const bool want_obs = width > 0;
+    // each member: its batch in one state, in the axis-0 Fourier representation
+    for (int k = 0; k < n; ++k) {
+        qnd_ctx *c = cs[k];
+        const int nb = c->bnb;
+        bool same = true;
+        for (int b = 1; b < nb; ++b) {
+            const SlotState &a0 = c->slots[0], &ab = c->slots[b];
+            if (ab.rep != a0.rep || ab.pend_mask != a0.pend_mask || ab.pend_vend != a0.pend_vend) same = false;
+        }
+        if (!same || c->slots[0].pend_vend)
+            for (int b = 0; b < nb; ++b)
+                if ((rc = ensure_rep<R>(c, b, REP_R))) return rc;
+        if (c->slots[0].rep == REP_R) {
+            if ((rc = plain_pass<R>(c, state_ptr<R>(c, 0), 0, -1, 1.0, nb))) return rc;
+            for (int b = 0; b < nb; ++b) c->slots[b].rep = REP_M;
+        }
+        if (want_obs) {
+            const size_t need = (size_t)nb * nsteps * width;
+            if (need > c->rec_cap) {
+                if (c->d_rec) HIPCK(c, hipFree(c->d_rec));
+                c->d_rec = nullptr;
+                c->rec_cap = 0;
+                HIPCK(c, hipMalloc(&c->d_rec, need * sizeof(double)));
+                c->rec_cap = need;
+            }
+        }
+    }
+    GroupGeo G0, G1;
+    if (!group_geometry(cs, n, 0, PK_POT, G0) || !group_geometry(cs, n, 1, PK_KIN, G1))
+        return fail(c0, QND_EUNSUPPORTED, "no workgroup size fits every member of the group");
+    for (int k = 0; k < n; ++k) {
+        qnd_ctx *c = cs[k];
+        if ((rc = ensure_part(c, (size_t)G0.g[k].grid * QND_NOBS))) return rc;
+        // tables the passes read, built before any capture
+        if (G1.g[k].vpt == 4) {
+            for (int ia = 0; ia <= s; ++ia) {
+                const double ck = ia == 0 ? c->as_scaled[0]
+                                          : c->as_scaled[2 * ia - 1] + (ia < s ? c->as_scaled[2 * ia] : 0.0);
+                (void)ktab_for<R>(c, ck, G1.g[k], &rc);
+                if (rc) return rc;
+            }
+        }
+        if (G0.g[k].vpt == 4) {
+            (void)vtab_for(c, &rc);
+            if (rc) return rc;
+        }
+        HIPCK(c, hipStreamSynchronize(c->stream));  // member work on its own stream is done
+    }
+    int64_t *d_step = reinterpret_cast<int64_t *>(c0->d_small + SMALL_STEP);
+    HIPCK(c0, hipMemcpyAsync(d_step, &step0, sizeof(int64_t), hipMemcpyHostToDevice, c0->stream));
+    HIPCK(c0, hipStreamSynchronize(c0->stream));
+    ObsSpec spec;
+    memset(&spec, 0, sizeof spec);
+    spec.ncodes = ncodes;
+    for (int k = 0; k < ncodes; ++k) spec.codes[k] = codes[k];
+    auto batch_rows = [&](PassArgs<R> &a, qnd_ctx *c) {
+        a.B.nb = c->bnb;
+        a.B.step = d_step;
+        a.B.step_base = step0;
+    };
+    // one step: boundary pass (owed mask, records), K(a_0) with the records'
+    // finalize, then per stage the potential pass and the next kinetic pass
+    // pend[k]: member k owes its mask (each member's own flag: the group
+    // kernel reads flags per member).  record_step lists one step's launches:
+    // boundary pass (owed mask, records), K(a_0) with the records' finalize,
+    // then per stage the potential pass and the next kinetic pass, then the
+    // step counter.
+    std::vector<GroupArgs<R>> descs;
+    auto record_step = [&](const std::vector<char> &pend, std::vector<GLaunch> &out) -> int {
+        std::vector<PassArgs<R>> as(n);
+        bool boundary = want_obs;
+        for (int k = 0; k < n; ++k) boundary |= pend[k] != 0;
+        if (boundary) {
+            for (int k = 0; k < n; ++k) {
+                qnd_ctx *c = cs[k];
+                PassArgs<R> &a = as[k];
+                a = base_args<R>(c, state_ptr<R>(c, 0), G0.g[k]);
+                batch_rows(a, c);
+                a.scale1 = 1.0 / (double)c->nr[0];
+                a.flags = PF_SECOND | (pend[k] ? PF_MASK : 0u);
+                if (want_obs) {
+                    a.flags |= PF_OBS;
+                    a.partials = c->d_part;
+                    a.npart = QND_NOBS;
+                }
+                if (c->round32) a.flags |= PF_ROUND32;
+            }
+            record_group<R, PK_POT, +1>(G0, as, descs, out);
+        }
+        auto kin = [&](int ia, bool fin) -> int {
+            for (int k = 0; k < n; ++k) {
+                qnd_ctx *c = cs[k];
+                const double ck = ia == 0 ? c->as_scaled[0]
+                                          : c->as_scaled[2 * ia - 1] + (ia < s ? c->as_scaled[2 * ia] : 0.0);
+                PassArgs<R> &a = as[k];
+                a = base_args<R>(c, state_ptr<R>(c, 0), G1.g[k]);
+                batch_rows(a, c);
+                a.kmode = KM_PHASE;
+                a.kin_axes = 3u;
+                a.ck_mid = ck;
+                a.scale2 = 1.0 / (double)c->nr[1];
+                int rr;
+                a.ktab = ktab_for<R>(c, ck, G1.g[k], &rr);
+                if (rr) return rr;
+                if (a.ktab) a.flags |= PF_KTAB;
+                if (fin) {
+                    a.flags |= PF_FINALIZE;
+                    a.fin_part = c->d_part;
+                    a.fin_nwg = G0.g[k].grid / c->bnb;
+                    a.fin_rec = c->d_rec;
+                    a.fin_spec = spec;
+                    a.B.finE = c->d_bEo;
+                    a.B.finE_step = d;
+                    a.B.finE_stride = c->bntime * d;
+                    a.B.rec_step = width;
+                    a.B.rec_stride = nsteps * width;
+                }
+                if (c->round32) a.flags |= PF_ROUND32;
+            }
+            if (fin) record_group<R, PK_KIN, -1>(G1, as, descs, out);
+            else record_group<R, PK_KINP, -1>(G1, as, descs, out);
+            return 0;
+        };
+        int r;
+        if ((r = kin(0, want_obs))) return r;
+        for (int ia = 0; ia < s; ++ia) {
+            for (int k = 0; k < n; ++k) {
+                qnd_ctx *c = cs[k];
+                PassArgs<R> &a = as[k];
+                a = base_args<R>(c, state_ptr<R>(c, 0), G0.g[k]);
+                batch_rows(a, c);
+                a.scale1 = 1.0 / (double)c->nr[0];
+                a.flags = PF_V1 | PF_SECOND;
+                a.vmode = VM_LENGTH;
+                a.cv1 = c->as_scaled[2 * ia] + c->as_scaled[2 * ia + 1];
+                a.B.fld = c->d_bEt + (int64_t)ia * d;
+                a.B.fld_step = (int64_t)s * d;
+                a.B.fld_stride = c->bntime * s * d;
+                if (G0.g[k].vpt == 4) {
+                    int rr;
+                    a.vtab = vtab_for(c, &rr);
+                    if (rr) return rr;
+                }
+                if (c->round32) a.flags |= PF_ROUND32;
+            }
+            record_group<R, PK_POTL, +1>(G0, as, descs, out);
+            if ((r = kin(ia + 1, false))) return r;
+        }
+        out.push_back(GLaunch{nullptr, 1, 64, 0, -1});
+        return 0;
+    };
+    const GroupArgs<R> *d_desc = nullptr;
+    auto run_step = [&](const std::vector<GLaunch> &L) -> int {
+        for (const GLaunch &l : L) {
+            if (l.desc < 0) {
+                hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, c0->stream, d_step);
+            } else {
+                const GroupArgs<R> *ptr = d_desc + l.desc;
+                void *args[] = {(void *)&ptr};
+                HIPCK(c0, hipLaunchKernel(l.fn, dim3(l.grid), dim3(l.threads), args, l.lds, c0->stream));
+            }
+            HIPCK(c0, hipGetLastError());
+        }
+        return 0;
+    };
+    std::vector<char> pend0(n), steady(n);  // owed masks now, and after every step
+    for (int k = 0; k < n; ++k) {
+        pend0[k] = cs[k]->slots[0].pend_mask ? 1 : 0;
+        steady[k] = cs[k]->P.mask_on ? 1 : 0;
+    }
+    std::vector<GLaunch> first, steadyL;
+    if ((rc = record_step(pend0, first))) return rc;  // the first step: what each member owes now
+    if ((rc = record_step(steady, steadyL))) return rc;
+    {   // the call's descriptors, uploaded once (they serve every step)
+        const size_t bytes = descs.size() * sizeof(GroupArgs<R>);
+        if (bytes > c0->gdesc_cap) {
+            if (c0->d_gdesc) HIPCK(c0, hipFree(c0->d_gdesc));
+            c0->d_gdesc = nullptr;
+            c0->gdesc_cap = 0;
+            HIPCK(c0, hipMalloc(&c0->d_gdesc, bytes));
+            c0->gdesc_cap = bytes;
+        }
+        HIPCK(c0, hipMemcpy(c0->d_gdesc, descs.data(), bytes, hipMemcpyHostToDevice));
+        d_desc = reinterpret_cast<const GroupArgs<R> *>(c0->d_gdesc);
+    }
+    int64_t done = 0;
+    if (nsteps > 0) {
+        if ((rc = run_step(first))) return rc;
+        done = 1;
+    }
+    bool prof = false;
+    for (int k = 0; k < n; ++k) prof |= cs[k]->prof;
+    if (use_graph && nsteps - done > 0 && !prof) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPCK(c0, hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
+        const int erc = run_step(steadyL);
+        const hipError_t ec = hipStreamEndCapture(c0->stream, &graph);
+        if (erc) {
+            if (graph) hipGraphDestroy(graph);
+            return erc;
+        }
+        if (ec != hipSuccess) return fail(c0, QND_EHIP, "graph capture failed: %s", hipGetErrorString(ec));
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            hipGraphDestroy(graph);
+            return fail(c0, QND_EHIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        hipError_t el = hipSuccess;
+        for (; done < nsteps && el == hipSuccess; ++done) el = hipGraphLaunch(exec, c0->stream);
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        if (el != hipSuccess) return fail(c0, QND_EHIP, "graph launch failed: %s", hipGetErrorString(el));
+    }
+    for (; done < nsteps; ++done)
+        if ((rc = run_step(steadyL))) return rc;
+    HIPCK(c0, hipStreamSynchronize(c0->stream));
+    for (int k = 0; k < n; ++k) {
+        qnd_ctx *c = cs[k];
+        for (int b = 0; b < c->bnb; ++b) {
+            c->slots[b].rep = REP_M;
+            c->slots[b].pend_mask = nsteps > 0 ? (c->P.mask_on != 0) : c->slots[b].pend_mask;
+        }
+        if (want_obs && obs_out && obs_out[k])
+            HIPCK(c, hipMemcpy(obs_out[k], c->d_rec, (size_t)c->bnb * nsteps * width * sizeof(double),
+                               hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
 // Write_final_observables numerics (src/io_functions.c:159-199)
 template <typename R>
 int final_observables(qnd_ctx *c, int slot, const double *R_mask, const double *pr_max, int Ne, double de,
@@ -2245,6 +2581,7 @@ void qnd_destroy(qnd_ctx *c)
     if (c->d_side) hipFree(c->d_side);
     if (c->d_kcoef) hipFree(c->d_kcoef);
     if (c->d_vtab) hipFree(c->d_vtab);
+    if (c->d_gdesc) hipFree(c->d_gdesc);
     for (void *p : {c->d_adj_ckpt, c->d_adj_final, c->d_adj_save, (void *)c->d_adj_part})
         if (p) hipFree(p);
     for (int k = 0; k < 16; ++k)
@@ -2561,6 +2898,41 @@ int qnd_real_steps_batch(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0
     return c->prec == QND_C64
                ? real_steps_impl<float>(c, slot0, nb, gauge, step0, nsteps, codes, ncodes, obs_out)
                : real_steps_impl<double>(c, slot0, nb, gauge, step0, nsteps, codes, ncodes, obs_out);
+}
+
+int qnd_group_real_steps(qnd_ctx *const *ctxs, int nctx, int gauge, int64_t step0, int64_t nsteps, const int *codes,
+                         int ncodes, double *const *obs_out, int use_graph)
+{
+    if (!ctxs || nctx < 1 || !ctxs[0]) return fail(nullptr, QND_EINVAL, "empty group");
+    qnd_ctx *c0 = ctxs[0];
+    if (nctx > QND_GROUP_MAX) return fail(c0, QND_EINVAL, "at most %d contexts in a group", QND_GROUP_MAX);
+    if (gauge != QND_GAUGE_LENGTH) return fail(c0, QND_EUNSUPPORTED, "groups run the length gauge");
+    if (nsteps < 0) return fail(c0, QND_EINVAL, "nsteps < 0");
+    if (ncodes > 8 || (ncodes > 0 && !codes)) return fail(c0, QND_EINVAL, "at most 8 observable codes");
+    if (!fused_codes(codes, ncodes)) return fail(c0, QND_EUNSUPPORTED, "groups record dipole_r, dipole_a and norm only");
+    for (int k = 0; k < nctx; ++k) {
+        qnd_ctx *c = ctxs[k];
+        if (!c) return fail(c0, QND_EINVAL, "null context in the group");
+        int rc = check_ready(c);
+        if (rc) return fail(c0, rc, "member %d: %s", k, c->err.c_str());
+        for (int j = 0; j < k; ++j)
+            if (ctxs[j] == c) return fail(c0, QND_EINVAL, "context %d appears twice in the group", k);
+        if (c->prec != c0->prec || c->dim != 2)
+            return fail(c0, QND_EUNSUPPORTED, "group members must be 2-D grids of one precision");
+        if (c->device != c0->device) return fail(c0, QND_EINVAL, "group members must share one device");
+        if (c->s != c0->s || memcmp(c->as_scaled, c0->as_scaled, sizeof c->as_scaled) != 0)
+            return fail(c0, QND_EINVAL, "group members must share the integrator and DT");
+        if (c->bnb < 1 || c->bfield_s != c->s)
+            return fail(c0, QND_ESTATE, "member %d has no batch field tables for this integrator "
+                        "(qnd_set_field_batch)", k);
+        if (step0 < 0 || step0 + nsteps > c->bntime)
+            return fail(c0, QND_EINVAL, "steps [%lld, %lld) outside member %d's field table of %lld steps",
+                        (long long)step0, (long long)(step0 + nsteps), k, (long long)c->bntime);
+        if (ncodes > 0 && (!obs_out || !obs_out[k])) return fail(c0, QND_EINVAL, "observable output of member %d", k);
+    }
+    int rc = DISPATCH(c0, group_real_steps, ctxs, nctx, step0, nsteps, codes, ncodes, obs_out, use_graph);
+    for (int k = 0; k < nctx; ++k) prof_flush(ctxs[k]);
+    return rc;
 }
 
 int qnd_final_observables(qnd_ctx *c, int slot, const double *R_mask, const double *pr_max, int Ne, double de,

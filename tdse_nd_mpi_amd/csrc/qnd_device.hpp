@@ -878,6 +878,12 @@ struct Batch {
     int64_t rec_stride;      // doubles between the records of two instances
     const double *srcr;      // PF_SRC: per-instance source weights [dim] (nullptr: PassArgs::src)
     int64_t srcr_stride;
+    // Step counter on the device (nullptr: none): the field rows, E(t_n) and
+    // the record then advance with *step, so that one launch sequence serves
+    // every step (a replayed graph): row = fld + *step * fld_step, E(t_n) =
+    // finE + *step * finE_step, record = rec + (*step - step_base) * rec_step.
+    const int64_t *step;
+    int64_t fld_step, finE_step, rec_step, step_base;
 };
 
 template <typename R> struct PassArgs {
@@ -1041,8 +1047,10 @@ __device__ __forceinline__ void obs_finalize_instance(const double *part, int nw
                                                       const ObsSpec &spec, const Batch &B, int b, double *rec,
                                                       double *red)
 {
-    const double *E = B.finE ? B.finE + b * B.finE_stride : spec.E;
-    obs_finalize(part + (int64_t)b * nwg * QND_NOBS, nwg, dim, dNr, spec, E, rec + b * B.rec_stride, red);
+    const int64_t sd = B.step ? *B.step : 0;
+    const double *E = B.finE ? B.finE + sd * B.finE_step + b * B.finE_stride : spec.E;
+    obs_finalize(part + (int64_t)b * nwg * QND_NOBS, nwg, dim, dNr, spec, E,
+                 rec + (sd - B.step_base) * B.rec_step + b * B.rec_stride, red);
 }
 
 
@@ -1060,13 +1068,13 @@ template <int BYTES> __device__ __forceinline__ void kernarg_touch()
     const __attribute__((address_space(4))) unsigned *k =
         (const __attribute__((address_space(4))) unsigned *)__builtin_amdgcn_kernarg_segment_ptr();
     constexpr int NL = (BYTES + 63) / 64;
-    static_assert(NL <= 12, "argument block larger than the touched lines");
-    unsigned v[12];
+    static_assert(NL <= 13, "argument block larger than the touched lines");
+    unsigned v[13];
 #pragma unroll
-    for (int i = 0; i < 12; ++i) v[i] = k[16 * (i < NL ? i : 0)];
+    for (int i = 0; i < 13; ++i) v[i] = k[16 * (i < NL ? i : 0)];
     // one consumer of all values: every load is issued before the one wait
     asm volatile("" ::"s"(v[0]), "s"(v[1]), "s"(v[2]), "s"(v[3]), "s"(v[4]), "s"(v[5]), "s"(v[6]), "s"(v[7]),
-                 "s"(v[8]), "s"(v[9]), "s"(v[10]), "s"(v[11]));
+                 "s"(v[8]), "s"(v[9]), "s"(v[10]), "s"(v[11]), "s"(v[12]));
 }
 
 // The pass kernel's leading scalar arguments are what the addresses of its
@@ -1083,10 +1091,13 @@ __host__ __device__ inline unsigned pass_geo(int log2n, int log2T, int log2TW, i
     return (unsigned)log2n | ((unsigned)log2T << 4) | ((unsigned)log2TW << 8) | ((unsigned)(pow2 != 0) << 12);
 }
 
-template <typename R, int KIND, int DIR1, int VPT>
-__global__ void __launch_bounds__(QND_PASS_THREADS)
-    k_pass(C<R> *data, const typename TwT<R>::T *tw, int64_t S, unsigned geo, unsigned fl_arg, int nblk,
-           const PassArgs<R> a)
+// The body of a line pass: workgroup `bid` of the pass's nblk line tiles
+// (and, for a finalizing kinetic pass, of the finalize workgroups after
+// them).  TOUCH: touch the argument block's lines at entry (the kernel's
+// PassArgs is its only argument block).
+template <typename R, int KIND, int DIR1, int VPT, bool TOUCH>
+__device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *tw, int64_t S, unsigned geo,
+                                          unsigned fl_arg, int nblk, const PassArgs<R> &a, int bid)
 {
     const unsigned fl = KIND == PK_POTL   ? (unsigned)(PF_V1 | PF_SECOND)
                         : (KIND == PK_POTI || KIND == PK_POTV) ? (unsigned)(PF_V1 | PF_V2 | PF_SECOND)
@@ -1118,8 +1129,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     // workgroups after the nblk line tiles finalize the previous pass's
     // observable record (one per instance) and do no line work, so the
     // record does not delay a tile
-    if (KIND == PK_KIN && (int)blockIdx.x >= nblk) {
-        const int b = (int)blockIdx.x - nblk;
+    if (KIND == PK_KIN && bid >= nblk) {
+        const int b = bid - nblk;
         if ((fl & PF_FINALIZE) && b < a.B.nb)
             obs_finalize_instance(a.fin_part, a.fin_nwg, a.P.dim, a.P.dNr, a.fin_spec, a.B, b, a.fin_rec, red);
         return;
@@ -1144,7 +1155,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     // gives each XCD a contiguous run of tiles: neighbouring strided tiles,
     // which share cache lines, then share one L2 instead of fetching the same
     // lines into several.
-    const int tb = (nblk % 8 == 0) ? (int)(blockIdx.x % 8) * (nblk / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+    const int tb = (nblk % 8 == 0) ? (bid % 8) * (nblk / 8) + bid / 8 : bid;
     const int64_t L = (int64_t)tb * TW + l;  // this thread's line
     int64_t lbase, estride;                          // element i at lbase + i * estride
     if (contig) { lbase = L * n; estride = 1; }
@@ -1160,7 +1171,7 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
 #pragma unroll
     for (int m = 0; m < VPT; ++m) x[m] = data[lbase + (int64_t)(t + m * T) * estride];
     // the argument block, now that the first loads are out
-    kernarg_touch<sizeof(PassArgs<R>) + 40>();
+    if constexpr (TOUCH) kernarg_touch<sizeof(PassArgs<R>) + 40>();
     const int ldsS = a.ldsStride;
     C<R> *bufA = lds + l * ldsS;
     C<R> *bufB = lds + (TW + l) * ldsS;
@@ -1236,7 +1247,8 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
         {
             const bool use_a = KINK || kmode == KM_VELOCITY;
             if (a.B.fld) {
-                const double *src = a.B.fld + (L / a.B.lpi) * a.B.fld_stride;
+                const int64_t sd = a.B.step ? *a.B.step : 0;
+                const double *src = a.B.fld + sd * a.B.fld_step + (L / a.B.lpi) * a.B.fld_stride;
 #pragma unroll
                 for (int j = 0; j < QND_MAXD; ++j) fv[j] = (j < P.dim) ? src[j] : 0.0;
             } else {
@@ -1686,12 +1698,102 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
     }
 }
 
+template <typename R, int KIND, int DIR1, int VPT>
+__global__ void __launch_bounds__(QND_PASS_THREADS)
+    k_pass(C<R> *data, const typename TwT<R>::T *tw, int64_t S, unsigned geo, unsigned fl_arg, int nblk,
+           const PassArgs<R> a)
+{
+    pass_body<R, KIND, DIR1, VPT, true>(data, tw, S, geo, fl_arg, nblk, a, (int)blockIdx.x);
+}
+
+// One launch of a pass over several grids ("classes": e.g. the instances of
+// a sweep on 128^2, 256^2 and 384^2 grids).  Class c owns workgroups
+// [blk0[c], blk0[c+1]) (blk0 multiples of 8, so each class keeps the
+// XCD-aware tile order of its own pass): nblk[c] line tiles, then, for a
+// finalizing kinetic pass, one finalize workgroup per instance, then idle
+// padding.  Every class has the same workgroup size; its values per thread
+// select the body.
+#define QND_GROUP_MAX 4
+template <typename R> struct GroupArgs {
+    int ncls;
+    int blk0[QND_GROUP_MAX + 1];
+    int nblk[QND_GROUP_MAX];
+    int vpt[QND_GROUP_MAX];
+    unsigned geo[QND_GROUP_MAX];
+    PassArgs<R> a[QND_GROUP_MAX];
+};
+
+// The descriptor lives in device memory (it is the same for every step of a
+// call: the step counter moves the rows): indexing a kernel-argument array
+// by the runtime class made the compiler copy the whole block to scratch.
+template <typename R, int KIND, int DIR1>
+__global__ void __launch_bounds__(QND_PASS_THREADS) k_pass_group(const GroupArgs<R> *__restrict__ gp)
+{
+    const GroupArgs<R> &g = *gp;
+    const int w = (int)blockIdx.x;
+    int c = 0;
+#pragma unroll
+    for (int k = 1; k < QND_GROUP_MAX; ++k)
+        if (k < g.ncls && w >= g.blk0[k]) c = k;
+    const int bid = w - g.blk0[c];
+    const PassArgs<R> &a = g.a[c];
+    const int extra = (KIND == PK_KIN && (a.flags & PF_FINALIZE)) ? a.B.nb : 0;
+    const int nblk = g.nblk[c];
+    if (bid >= nblk + extra) return;  // padding up to the next class
+    const int vpt = g.vpt[c];
+    const unsigned geo = g.geo[c];
+    if (vpt == 4) pass_body<R, KIND, DIR1, 4, false>(a.data, a.tw, a.S, geo, a.flags, nblk, a, bid);
+    else if (vpt == 12) pass_body<R, KIND, DIR1, 12, false>(a.data, a.tw, a.S, geo, a.flags, nblk, a, bid);
+    else pass_body<R, KIND, DIR1, 8, false>(a.data, a.tw, a.S, geo, a.flags, nblk, a, bid);
+}
+
+// ---------------------------------------------------------------------
+// Every line-pass kernel the library launches.  They are compiled in
+// several kernel translation units (qnd_kernels_*.hip, one share each, in
+// parallel) and declared extern in the host translation unit
+// (qnd_api.hip), which then compiles no pass kernel itself.
+// QND_PASS_SET_k(X, R) lists share k as X(R, KIND, DIR); the values per
+// thread 4, 8 and 12 of each are one entry.
+// ---------------------------------------------------------------------
+#define QND_PASS_SET_0(X, R) X(R, PK_POTL, +1) X(R, PK_KINP, -1) X(R, PK_POT, +1) X(R, PK_PLAIN, -1) \
+                             X(R, PK_PLAIN, +1)
+#define QND_PASS_SET_1(X, R) X(R, PK_KIN, -1) X(R, PK_KINS, -1) X(R, PK_POTX, +1) X(R, PK_POTO, +1) \
+                             X(R, PK_POTV, +1) X(R, PK_KINV, -1)
+#define QND_PASS_SET_2(X, R) X(R, PK_POTI, +1) X(R, PK_KINI, -1) X(R, PK_P1DI, +1) X(R, PK_POT1D, +1) \
+                             X(R, PK_KINR, -1) X(R, PK_POTS, +1) X(R, PK_POTG, +1)
+#define QND_GROUP_SET(X, R) X(R, PK_POT, +1) X(R, PK_POTL, +1) X(R, PK_KIN, -1) X(R, PK_KINP, -1)
+
+#define QND_PASS_ARGS(R) C<R> *, const typename TwT<R>::T *, int64_t, unsigned, unsigned, int, const PassArgs<R>
+#define QND_PASS_EXTERN(R, K, D)                                                                     \
+    extern template __global__ void k_pass<R, K, D, 4>(QND_PASS_ARGS(R));                           \
+    extern template __global__ void k_pass<R, K, D, 8>(QND_PASS_ARGS(R));                           \
+    extern template __global__ void k_pass<R, K, D, 12>(QND_PASS_ARGS(R));
+#define QND_PASS_INSTANTIATE(R, K, D)                                                                \
+    template __global__ void k_pass<R, K, D, 4>(QND_PASS_ARGS(R));                                  \
+    template __global__ void k_pass<R, K, D, 8>(QND_PASS_ARGS(R));                                  \
+    template __global__ void k_pass<R, K, D, 12>(QND_PASS_ARGS(R));
+#define QND_GROUP_EXTERN(R, K, D) extern template __global__ void k_pass_group<R, K, D>(const GroupArgs<R> *);
+#define QND_GROUP_INSTANTIATE(R, K, D) template __global__ void k_pass_group<R, K, D>(const GroupArgs<R> *);
+
+#ifdef QND_HOST_TU
+QND_PASS_SET_0(QND_PASS_EXTERN, float) QND_PASS_SET_0(QND_PASS_EXTERN, double)
+QND_PASS_SET_1(QND_PASS_EXTERN, float) QND_PASS_SET_1(QND_PASS_EXTERN, double)
+QND_PASS_SET_2(QND_PASS_EXTERN, float) QND_PASS_SET_2(QND_PASS_EXTERN, double)
+QND_GROUP_SET(QND_GROUP_EXTERN, float) QND_GROUP_SET(QND_GROUP_EXTERN, double)
+#endif
+
+// advances the device step counter of a replayed step sequence
+static __global__ void k_step_advance(int64_t *step)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
 // ---------------------------------------------------------------------
 // small kernels
 // ---------------------------------------------------------------------
 
 // out[q] = scale * sum_w part[w*npart + q], one wavefront per q
-__global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__ part, int nwg, int npart,
+static __global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__ part, int nwg, int npart,
                                                       int nq, double scale, double *out)
 {
     const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
@@ -1703,7 +1805,7 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double *__restrict__
 
 // out[p][q] = sum_w part[p][w][q] for npass blocks of [nwg][npart] partials,
 // one wavefront per (p, q), fixed order
-__global__ void __launch_bounds__(256) k_sum_blocks(const double *__restrict__ part, int npass, int nwg, int npart,
+static __global__ void __launch_bounds__(256) k_sum_blocks(const double *__restrict__ part, int npass, int nwg, int npart,
                                                     double *out)
 {
     const int wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
@@ -1715,7 +1817,7 @@ __global__ void __launch_bounds__(256) k_sum_blocks(const double *__restrict__ p
 }
 
 // one block per instance
-__global__ void __launch_bounds__(1024) k_obs_finalize(const double *__restrict__ part, int nwg, int dim,
+static __global__ void __launch_bounds__(1024) k_obs_finalize(const double *__restrict__ part, int nwg, int dim,
                                                        double dNr, ObsSpec spec, Batch B, double *rec)
 {
     __shared__ double red[16 * QND_NOBS];
@@ -1729,7 +1831,7 @@ __global__ void __launch_bounds__(1024) k_obs_finalize(const double *__restrict_
 // coef = [c_0 .. c_{N-1} (re,im), invnorm, flag, norm, step of the first collapse]
 // `step` is the caller's index of the step whose dots these are: the first
 // collapse (norm below 1e-12) records it beside the sticky flag.
-__global__ void __launch_bounds__(1024) k_gs_coeffs(const double *__restrict__ part, int nwg, int npart,
+static __global__ void __launch_bounds__(1024) k_gs_coeffs(const double *__restrict__ part, int nwg, int npart,
                                                     int nprev, double scale, const double *__restrict__ gram,
                                                     double *coef, double step)
 {
@@ -1843,7 +1945,7 @@ __global__ void __launch_bounds__(256) k_ktab(T *tab, const double *__restrict__
 
 // Potential table: V at every element of one instance, FP64, from the same
 // closed forms (PotPoint<double>) as the passes.
-__global__ void __launch_bounds__(256) k_vtab(double *tab, int64_t N, Phys P)
+static __global__ void __launch_bounds__(256) k_vtab(double *tab, int64_t N, Phys P)
 {
     for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < N; f += (int64_t)gridDim.x * blockDim.x) {
         int idx[QND_MAXD];
