@@ -468,6 +468,29 @@ def measured_traffic(kernel="void qnd::k_pass<float, 6, 1, 4>"):
             "source": os.path.relpath(TRAFFIC_PROFILE, ROOT) + " (default launch geometry)"}
 
 
+def latency_probes():
+    """Grid barrier (one cooperative launch, device-scope release/acquire
+    atomics, bounded waits) against kernel boundary (back-to-back empty
+    launches), microseconds each, for the workgroup shapes of the 512^2
+    passes (tdse_nd_mpi_amd/libqnd_probe.so)."""
+    import ctypes as C
+    path = os.path.join(ROOT, "tdse_nd_mpi_amd", "libqnd_probe.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    for f in (L.qnd_probe_barrier, L.qnd_probe_boundary):
+        f.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    out = []
+    for blocks, threads in ((256, 256), (512, 128), (1024, 64)):
+        b, k = C.c_double(), C.c_double()
+        rb = L.qnd_probe_barrier(blocks, threads, 2000, C.byref(b))
+        L.qnd_probe_boundary(blocks, threads, 2000, C.byref(k))
+        out.append({"workgroups": blocks, "threads": threads,
+                    "grid_barrier_us": b.value if rb == 0 else None, "barrier_status": rb,
+                    "kernel_boundary_us": k.value})
+    return out
+
+
 def copy_bandwidth(reps=10):
     """Measured device-copy bandwidth (GB/s, read + write) of a 1 GiB FP32
     copy, timed with events on torch's stream, where the copies run."""
@@ -786,6 +809,11 @@ def main():
                        "copy_GBs": (cbw := copy_bandwidth()), "frac_of_copy": ach / cbw,
                        "launches_per_step": {names[i]: float(cnt[i]) / args.profile_steps for i in range(5)},
                        "class_ms_per_step": {names[i]: float(ms[i]) / args.profile_steps for i in range(5)}}
+    out["latency_probes"] = latency_probes()
+    if out["latency_probes"]:
+        log("  grid barrier vs kernel boundary (us): " + ", ".join(
+            f"{p['workgroups']}x{p['threads']}: {p['grid_barrier_us']} vs {p['kernel_boundary_us']:.2f}"
+            for p in out["latency_probes"]))
     if not args.skip_cpu:
         log("cpu baseline (FP64 oracle on the host cores, best-case and faithful threading) ...")
         var, threads, ref_obs = cpu_baseline(psi0, Et, At, Eo, Ao, args.cpu_budget)
