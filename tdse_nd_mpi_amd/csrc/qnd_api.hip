@@ -115,6 +115,13 @@ struct qnd_ctx {
     // potential changes
     double *d_vtab = nullptr;
     int64_t pot_gen = 0, vtab_gen = -1;
+    // per-axis kinetic factors of every kinetic coefficient of a step (phase:
+    // s + 1 coefficients; imaginary time: s), Sigma n_j entries each
+    void *d_kax = nullptr;
+    std::vector<std::pair<int, double>> kax_coef;
+    int64_t kax_gen = -1;
+    int kax_prec = -1;
+    bool use_kax = true;                  // QND_KAX=0: kinetic factors formed per element instead
     // descriptors of the group launches this context leads (qnd_group_real_steps)
     void *d_gdesc = nullptr;
     size_t gdesc_cap = 0;
@@ -512,6 +519,9 @@ template <typename R> int launch_pot(qnd_ctx *c, const Geo &g, const PassArgs<R>
         return launch<R, PK_POTV, +1>(c, g, a, PC_POT);
     if ((a.flags & ~PF_MASK) == (PF_V1 | PF_SECOND | PF_PREKX) && a.vmode == VM_LENGTH)
         return launch<R, PK_POTX, +1>(c, g, a, PC_BND);
+    if (a.vmode == VM_IMAG && (a.flags & ~(PF_V1 | PF_V2 | PF_SECOND | PF_GS_UPDATE | PF_GS_DOTS)) == 0)
+        return launch<R, PK_POTJ, +1>(c, g, a, PC_POT);
+    if ((a.flags & ~PF_MASK) == (PF_OBS | PF_SECOND)) return launch<R, PK_POTB, +1>(c, g, a, PC_POT);
     return launch<R, PK_POT, +1>(c, g, a, PC_POT);
 }
 
@@ -624,6 +634,81 @@ template <typename R> const typename TwT<R>::T *atab_for(qnd_ctx *c, int *rc)
     return (const T *)c->d_atab;
 }
 
+// Per-axis kinetic factors of every kinetic coefficient c of a step: the
+// phase factors e^{-i c p_j(i)^2 / 2} of the real-time step's s + 1
+// coefficients (a_0, a_{2i+1} + a_{2i+2}, a_{2s-1}) and the damping factors
+// e^{-c p_j(i)^2 / 2} of the imaginary-time step's s coefficients
+// (a_{2i} + a_{2i+1}), Sigma n_j entries per coefficient at the per-axis
+// offsets.  Formed in FP64 on the host (complex64: FP32 hi + lo; a damping
+// factor in the first component, its exponent clamped as in the pass).  The
+// kinetic factor of a pass is then one table entry per element and one per
+// folded axis of its line, instead of an angle reduction and sin/cos per
+// element.  Rebuilt when the set-up or the precision changes; nullptr when c
+// is not one of the step's coefficients (or QND_KAX=0).
+template <typename R>
+const typename TwT<R>::T *kax_for(qnd_ctx *c, int kmode, double ck, int *rc)
+{
+    using T = typename TwT<R>::T;
+    *rc = 0;
+    if (!c->use_kax || !c->have_int || (kmode != KM_PHASE && kmode != KM_IMAG)) return nullptr;
+    const int tot = c->P.off[c->dim - 1] + c->P.n[c->dim - 1];
+    if (c->kax_gen != c->setup_gen || c->kax_prec != c->prec || !c->d_kax) {
+        const int s = c->s;
+        const double *as = c->as_scaled;
+        std::vector<std::pair<int, double>> coef{{KM_PHASE, as[0]}};
+        for (int i = 0; i < s - 1; ++i) coef.push_back({KM_PHASE, as[2 * i + 1] + as[2 * i + 2]});
+        coef.push_back({KM_PHASE, as[2 * s - 1] + 0.0});
+        for (int i = 0; i < s; ++i) coef.push_back({KM_IMAG, as[2 * i] + as[2 * i + 1]});
+        const double clamp = Prec<R>::clamp;
+        std::vector<T> h(coef.size() * (size_t)tot);
+        for (size_t q = 0; q < coef.size(); ++q)
+            for (int j = 0; j < c->dim; ++j)
+                for (int i = 0; i < c->P.n[j]; ++i) {
+                    const double p = (i < (c->P.n[j] >> 1)) ? i * c->P.dp[j] : (i - c->P.n[j]) * c->P.dp[j];
+                    const double K = .5 * p * p;
+                    double cn, sn;
+                    if (coef[q].first == KM_PHASE) {
+                        const double ph = coef[q].second * K;
+                        cn = cos(ph);
+                        sn = -sin(ph);
+                    } else {
+                        const double arg = -coef[q].second * K;
+                        cn = exp(arg > clamp ? clamp : arg);
+                        sn = 0.0;
+                    }
+                    T &e = h[q * tot + c->P.off[j] + i];
+                    if constexpr (std::is_same<T, double2>::value) {
+                        e.x = cn;
+                        e.y = sn;
+                    } else {
+                        e.x = (float)cn;
+                        e.y = (float)sn;
+                        e.z = (float)(cn - (double)e.x);
+                        e.w = (float)(sn - (double)e.y);
+                    }
+                }
+        if (c->d_kax) HIPCK_P(c, hipFree(c->d_kax), rc);
+        c->d_kax = nullptr;
+        HIPCK_P(c, hipMalloc(&c->d_kax, h.size() * sizeof(T)), rc);
+        HIPCK_P(c, hipMemcpy(c->d_kax, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), rc);
+        c->kax_gen = c->setup_gen;
+        c->kax_prec = c->prec;
+        c->kax_coef = coef;
+    }
+    for (size_t q = 0; q < c->kax_coef.size(); ++q)
+        if (c->kax_coef[q].first == kmode && c->kax_coef[q].second == ck) return (const T *)c->d_kax + q * tot;
+    return nullptr;
+}
+
+// Where the per-axis tables pay (measured): 2-D kinetic passes with eight or
+// twelve values per thread (the 384^2 sweep class 3.5 % faster, 256^2
+// unchanged).  Not with four values per thread, where one wavefront per SIMD
+// would wait on the table read between the transforms, and not in 3-D, where
+// the 512^3 kinetic passes became 7 % slower with 25 % fewer VALU
+// instructions: they are bound by the state's memory latency, and the table
+// read adds one more wait.
+static bool kax_pays(const qnd_ctx *c, const Geo &g) { return c->dim == 2 && g.vpt != 4; }
+
 // A pending observable record (partials of the previous axis-0 pass) is
 // finalized by workgroups appended to the first kinetic pass when fin is given.
 struct Finalize {
@@ -690,6 +775,11 @@ int kinetic_block(qnd_ctx *c, C<R> *data, int kmode, double ck, const double *A,
             a.ktab = ktab_for<R>(c, ck, g, &rc);
             if (rc) return rc;
             if (a.ktab) a.flags |= PF_KTAB;
+        }
+        if ((kmode == KM_PHASE || kmode == KM_IMAG) && !a.ktab && kax_pays(c, g)) {
+            a.ktab = kax_for<R>(c, kmode, ck, &rc);
+            if (rc) return rc;
+            if (a.ktab) a.flags |= PF_KAX;
         }
         if (A)
             for (int j = 0; j < d; ++j) a.Ak[j] = A[j];
@@ -2079,6 +2169,11 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
                 a.ktab = ktab_for<R>(c, ck, G1.g[k], &rr);
                 if (rr) return rr;
                 if (a.ktab) a.flags |= PF_KTAB;
+                else if (kax_pays(c, G1.g[k])) {
+                    a.ktab = kax_for<R>(c, KM_PHASE, ck, &rr);
+                    if (rr) return rr;
+                    if (a.ktab) a.flags |= PF_KAX;
+                }
                 if (fin) {
                     a.flags |= PF_FINALIZE;
                     a.fin_part = c->d_part;
@@ -2502,6 +2597,7 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     }
     if (const char *e = getenv("QND_WAVE_LINES")) c->wave_lines = atoi(e) != 0;
     if (const char *e = getenv("QND_ROUND_STATE")) c->round32 = precision == QND_C128 && atoi(e) != 0;
+    if (const char *e = getenv("QND_KAX")) c->use_kax = atoi(e) != 0;
     if (const char *e = getenv("QND_VPT")) {  // tuning knob: values per thread
         const int v = atoi(e);
         if (v == 4 || v == 8) c->vpt_override = v;
@@ -2577,6 +2673,7 @@ void qnd_destroy(qnd_ctx *c)
     if (c->d_bEo) hipFree(c->d_bEo);
     if (c->d_ktab) hipFree(c->d_ktab);
     if (c->d_atab) hipFree(c->d_atab);
+    if (c->d_kax) hipFree(c->d_kax);
     if (c->d_Eo) hipFree(c->d_Eo);
     if (c->d_side) hipFree(c->d_side);
     if (c->d_kcoef) hipFree(c->d_kcoef);

@@ -819,9 +819,14 @@ template <typename R> __device__ __forceinline__ C<R> round_to_c64(C<R> v)
 // (PF_V1 | PF_V2 | PF_SECOND) and kinetic pass.
 // PK_P1DI is the 1-D imaginary-time stage pass (PF_V1 | PF_V2 | PF_SECOND |
 // PF_KPOST, both modes imaginary).
+// PK_POTJ is the imaginary-time potential pass with its flags from the launch,
+// among PF_V1, PF_V2, PF_SECOND and the Gram-Schmidt update and dots (the
+// first and last potential pass of a step, dim >= 2); PK_POTB the observable
+// boundary pass (PF_OBS | PF_SECOND, [PF_MASK]: owed mask and records, no
+// potential factor).
 enum PassKind { PK_PLAIN = 0, PK_KIN = 1, PK_POT = 2, PK_POT1D = 3, PK_KINS = 4, PK_KINR = 5, PK_POTL = 6,
                 PK_KINP = 7, PK_POTI = 8, PK_KINI = 9, PK_POTX = 10, PK_POTO = 11, PK_POTS = 12, PK_POTG = 13,
-                PK_POTV = 14, PK_KINV = 15, PK_P1DI = 16 };
+                PK_POTV = 14, PK_KINV = 15, PK_P1DI = 16, PK_POTJ = 17, PK_POTB = 18 };
 enum PassFlags : unsigned {
     PF_GS_UPDATE = 1u << 0,  // prologue: v = (v - sum_j c_j prev_j) * invnorm
     PF_KPRE = 1u << 1,       // (dim = 1) kinetic factor before the first transform
@@ -839,6 +844,9 @@ enum PassFlags : unsigned {
     PF_GRAD = 1u << 12,      // partials of x_j Im(conj(u) phi) and dV/da Im(conj(u) phi), phi from `save`
     PF_SRC = 1u << 13,       // u += sum_j src_j (w_j - E_j) y + src_norm y (y from `save`), w = -grad V
     PF_KTAB = 1u << 14,      // kinetic factor read from `ktab` (tabulated e^{-i c K}, four values per thread)
+    PF_KAX = 1u << 15,       // kinetic factor from per-axis tables in `ktab` (phase: e^{-i c K_j}; imaginary
+                             // time: e^{-c K_j} in the first component), Sigma n_j entries at the per-axis
+                             // offsets: this axis's entry per element times the folded axes' entries of the line
     // merged step boundary (2-D length gauge): K(a_last) | record | Mask | K(a_0);
     // its kinetic pass is the kernel kind PK_KINS
     PF_PREKX = 1u << 16,     // axis-0 pass: after the first transform, [axis-0 mask], forward, axis-0
@@ -902,7 +910,8 @@ template <typename R> struct PassArgs {
     double cv1, cv2;
     double Ev[QND_MAXD];
     // kinetic factors e^{-i c K} of this pass's coefficient at every element of
-    // one instance (row-major), when tabulated (PF_KTAB)
+    // one instance (row-major), when tabulated (PF_KTAB); or the per-axis
+    // factors of this pass's coefficient (PF_KAX)
     const typename TwT<R>::T *ktab;
     // per-axis kinetic factors e^{-i c K_j} of one coefficient (the step's
     // first, a_0), Sigma n_j entries at the per-axis offsets (PK_KINS, PF_PREKX)
@@ -1058,6 +1067,10 @@ __device__ __forceinline__ void obs_finalize_instance(const double *part, int nw
 // the line-pass kernel
 // ---------------------------------------------------------------------
 #define QND_PASS_THREADS 512   // workgroup cap: leaves 256 VGPRs per lane
+// Gram-Schmidt in the passes: previous states whose values are in flight at
+// once (one memory latency per group), and states per LDS chunk of the dots
+#define QND_GS_G(R) (sizeof(R) == 4 ? 2 : 1)
+#define QND_GS_CHUNK 8
 // Touch every 64-byte line of the kernel's argument block at entry, so that
 // their cache misses overlap.  The compiler reads arguments lazily, near
 // their first use and again later; each first read of a line would
@@ -1106,9 +1119,11 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
                         : KIND == PK_POTO ? (unsigned)(PF_OBS | PF_NOSTORE) | (fl_arg & PF_MASK)
                         : KIND == PK_POTS ? (unsigned)(PF_V1 | PF_SAVE | PF_SECOND)
                         : KIND == PK_POTG ? (unsigned)(PF_V1 | PF_CONJ | PF_GRAD | PF_SECOND)
+                        : KIND == PK_POTJ ? fl_arg & (unsigned)(PF_V1 | PF_V2 | PF_SECOND | PF_GS_UPDATE | PF_GS_DOTS)
+                        : KIND == PK_POTB ? (unsigned)(PF_OBS | PF_SECOND) | (fl_arg & PF_MASK)
                                           : fl_arg;
     const int vmode = (KIND == PK_POTL || KIND == PK_POTX || KIND == PK_POTS || KIND == PK_POTG) ? (int)VM_LENGTH
-                      : (KIND == PK_POTI || KIND == PK_P1DI)                                   ? (int)VM_IMAG
+                      : (KIND == PK_POTI || KIND == PK_P1DI || KIND == PK_POTJ)                ? (int)VM_IMAG
                       : KIND == PK_POTV                                                        ? (int)VM_VELOCITY
                                                                                                : a.vmode;
     const int kmode = (KIND == PK_KINP || KIND == PK_KINS || KIND == PK_KINR) ? (int)KM_PHASE
@@ -1122,7 +1137,7 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
     C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
     constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL || KIND == PK_POTI ||
                            KIND == PK_POTX || KIND == PK_POTO || KIND == PK_POTS || KIND == PK_POTG ||
-                           KIND == PK_POTV || KIND == PK_P1DI);
+                           KIND == PK_POTV || KIND == PK_P1DI || KIND == PK_POTJ || KIND == PK_POTB);
     constexpr bool KINK = (KIND == PK_KIN || KIND == PK_KINS || KIND == PK_KINR || KIND == PK_KINP ||
                            KIND == PK_KINI || KIND == PK_KINV);
 
@@ -1182,16 +1197,30 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
         double vr[VPT], vi[VPT];
 #pragma unroll
         for (int m = 0; m < VPT; ++m) { vr[m] = (double)x[m].x; vi[m] = (double)x[m].y; }
-        for (int j = 0; j < a.nprev; ++j) {
-            const double cr = a.gs_coef[2 * j], ci = a.gs_coef[2 * j + 1];
-            const C<R> *pj = a.prev + (int64_t)j * a.slot_stride;
-            C<R> p[VPT];
+        // the previous states' values are requested QND_GS_G states at a
+        // time, so that one memory latency serves several states (the
+        // subtraction order stays j = 0, 1, ...)
+        constexpr int G = QND_GS_G(R);
+        for (int j0 = 0; j0 < a.nprev; j0 += G) {
+            C<R> p[G][VPT];
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) p[m] = pj[lbase + (int64_t)(t + m * T) * estride];
+            for (int g = 0; g < G; ++g) {
+                if (j0 + g < a.nprev) {
+                    const C<R> *pj = a.prev + (int64_t)(j0 + g) * a.slot_stride;
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) {
-                vr[m] -= cr * (double)p[m].x - ci * (double)p[m].y;
-                vi[m] -= cr * (double)p[m].y + ci * (double)p[m].x;
+                    for (int m = 0; m < VPT; ++m) p[g][m] = pj[lbase + (int64_t)(t + m * T) * estride];
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (j0 + g < a.nprev) {
+                    const double cr = a.gs_coef[2 * (j0 + g)], ci = a.gs_coef[2 * (j0 + g) + 1];
+#pragma unroll
+                    for (int m = 0; m < VPT; ++m) {
+                        vr[m] -= cr * (double)p[g][m].x - ci * (double)p[g][m].y;
+                        vi[m] -= cr * (double)p[g][m].y + ci * (double)p[g][m].x;
+                    }
+                }
             }
         }
 #pragma unroll
@@ -1386,6 +1415,31 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
             if (KINK && VPT == 4 && (fl & PF_KTAB)) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(ktv[m], cs < 0.0));
+            } else if ((fl & PF_KAX) && kmode == KM_IMAG) {
+                // e^{-c K} = prod_j e^{-c K_j}: this axis's factor per element,
+                // times the folded axes' factors of the line
+                const typename TwT<R>::T *kx = a.ktab;
+                R fo = (R)1;
+#pragma unroll
+                for (int j = 0; j < QND_MAXD; ++j)
+                    if (j < P.dim && j != ax && ((a.kin_axes >> j) & 1u)) fo *= (R)kx[P.off[j] + idx[j]].x;
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], (R)kx[P.off[ax] + t + m * T].x * fo);
+            } else if ((fl & PF_KAX) && kmode == KM_PHASE) {
+                // e^{-i c K} = prod_j e^{-i c K_j}, each factor a unit table
+                // entry applied by the compensated product (one rounding of
+                // the state per factor)
+                const typename TwT<R>::T *kx = a.ktab;
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(kx[P.off[ax] + t + m * T], cs < 0.0));
+#pragma unroll
+                for (int j = 0; j < QND_MAXD; ++j) {
+                    if (j < P.dim && j != ax && ((a.kin_axes >> j) & 1u)) {
+                        const Fac<R> fo = tab_fac<R>(kx[P.off[j] + idx[j]], cs < 0.0);
+#pragma unroll
+                        for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], fo);
+                    }
+                }
             } else if (kmode == KM_IMAG) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) {
@@ -1674,26 +1728,62 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
         for (int m = 0; m < VPT; ++m) data[lbase + (int64_t)(t + m * T) * estride] = x[m];
     }
 
-    // ---- Gram-Schmidt dots on the stored representation ----
+    // ---- Gram-Schmidt dots on the stored representation: <prev_j|v> for
+    // j < nprev, then <v|v>, in chunks of QND_GS_CHUNK.  The previous states'
+    // values are requested QND_GS_G states at a time; each dot is summed over
+    // the wavefront and parked in LDS, and one barrier per chunk precedes the
+    // sums over the wavefronts (in wavefront order: deterministic). ----
     if (POTK && (fl & PF_GS_DOTS)) {
-        for (int j = 0; j <= a.nprev; ++j) {
-            double sr = 0.0, si = 0.0;
-            if (j < a.nprev) {
-                const C<R> *pj = a.prev + (int64_t)j * a.slot_stride;
-                C<R> p[VPT];
+        __shared__ double gsred[2 * QND_GS_CHUNK * 16];
+        constexpr int G = QND_GS_G(R);
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int nw = (blockDim.x + 63) >> 6;
+        const int nq = a.nprev + 1;
+        for (int q0 = 0; q0 < nq; q0 += QND_GS_CHUNK) {
+            const int q1 = nq < q0 + QND_GS_CHUNK ? nq : q0 + QND_GS_CHUNK;
+            for (int j0 = q0; j0 < q1; j0 += G) {
+                C<R> p[G][VPT];
 #pragma unroll
-                for (int m = 0; m < VPT; ++m) p[m] = pj[lbase + (int64_t)(t + m * T) * estride];
+                for (int g = 0; g < G; ++g) {
+                    if (j0 + g < q1 && j0 + g < a.nprev) {
+                        const C<R> *pj = a.prev + (int64_t)(j0 + g) * a.slot_stride;
 #pragma unroll
-                for (int m = 0; m < VPT; ++m) {
-                    sr += (double)p[m].x * (double)x[m].x + (double)p[m].y * (double)x[m].y;
-                    si += (double)p[m].x * (double)x[m].y - (double)p[m].y * (double)x[m].x;
+                        for (int m = 0; m < VPT; ++m) p[g][m] = pj[lbase + (int64_t)(t + m * T) * estride];
+                    }
                 }
-            } else {
 #pragma unroll
-                for (int m = 0; m < VPT; ++m) sr += (double)x[m].x * (double)x[m].x + (double)x[m].y * (double)x[m].y;
+                for (int g = 0; g < G; ++g) {
+                    const int j = j0 + g;
+                    if (j < q1) {
+                        double sr = 0.0, si = 0.0;
+                        if (j < a.nprev) {
+#pragma unroll
+                            for (int m = 0; m < VPT; ++m) {
+                                sr += (double)p[g][m].x * (double)x[m].x + (double)p[g][m].y * (double)x[m].y;
+                                si += (double)p[g][m].x * (double)x[m].y - (double)p[g][m].y * (double)x[m].x;
+                            }
+                        } else {
+#pragma unroll
+                            for (int m = 0; m < VPT; ++m)
+                                sr += (double)x[m].x * (double)x[m].x + (double)x[m].y * (double)x[m].y;
+                        }
+                        sr = wave_sum(sr);
+                        si = wave_sum(si);
+                        if (lane == 0) {
+                            gsred[(2 * (j - q0)) * 16 + wv] = sr;
+                            gsred[(2 * (j - q0) + 1) * 16 + wv] = si;
+                        }
+                    }
+                }
             }
-            double s2v[2] = {sr, si};
-            block_sum_vec<2>(s2v, red, a.partials + (int64_t)tb * a.npart + 2 * j);
+            lds_barrier();
+            const int ncol = 2 * (q1 - q0);  // may exceed a small workgroup's threads
+            for (int col = (int)threadIdx.x; col < ncol; col += (int)blockDim.x) {
+                double sum = 0.0;
+                for (int i = 0; i < nw; ++i) sum += gsred[col * 16 + i];
+                a.partials[(int64_t)tb * a.npart + 2 * q0 + col] = sum;
+            }
+            lds_barrier();
         }
     }
 }
@@ -1758,9 +1848,9 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass_group(const GroupArgs
 #define QND_PASS_SET_0(X, R) X(R, PK_POTL, +1) X(R, PK_KINP, -1) X(R, PK_POT, +1) X(R, PK_PLAIN, -1) \
                              X(R, PK_PLAIN, +1)
 #define QND_PASS_SET_1(X, R) X(R, PK_KIN, -1) X(R, PK_KINS, -1) X(R, PK_POTX, +1) X(R, PK_POTO, +1) \
-                             X(R, PK_POTV, +1) X(R, PK_KINV, -1)
+                             X(R, PK_POTV, +1) X(R, PK_KINV, -1) X(R, PK_POTB, +1)
 #define QND_PASS_SET_2(X, R) X(R, PK_POTI, +1) X(R, PK_KINI, -1) X(R, PK_P1DI, +1) X(R, PK_POT1D, +1) \
-                             X(R, PK_KINR, -1) X(R, PK_POTS, +1) X(R, PK_POTG, +1)
+                             X(R, PK_KINR, -1) X(R, PK_POTS, +1) X(R, PK_POTG, +1) X(R, PK_POTJ, +1)
 #define QND_GROUP_SET(X, R) X(R, PK_POT, +1) X(R, PK_POTL, +1) X(R, PK_KIN, -1) X(R, PK_KINP, -1)
 
 #define QND_PASS_ARGS(R) C<R> *, const typename TwT<R>::T *, int64_t, unsigned, unsigned, int, const PassArgs<R>
