@@ -117,17 +117,17 @@ SWEEP_DR = 0.25
 SWEEP_SEED = 20261203
 
 
-def sweep_fields(F, steps):
+def sweep_fields(F):
+    """The sweep's per-instance laser modes (random CEP and intensity); their
+    field tables are formed on the device (qnd_set_field_modes)."""
     rng = np.random.default_rng(SWEEP_SEED)
     nb = SWEEP_PER * len(SWEEP_SIZES)
     ceps = rng.uniform(0.0, 2 * np.pi, nb)
     ints = 10.0 ** rng.uniform(np.log10(5e13), np.log10(5e14), nb)
-    tabs = []
-    for i in range(nb):
-        m = F.Mode(intensity=ints[i], wavelength=800.0, ellipticity=0.0, cep=ceps[i], envelope="cos2", cycles=3)
-        tabs.append(F.tables([m], "BM4", DT, steps, 2))
+    modes = [[F.Mode(intensity=ints[i], wavelength=800.0, ellipticity=0.0, cep=ceps[i], envelope="cos2",
+                     cycles=3)] for i in range(nb)]
     ntime_full = int(F.Mode(cycles=3).tau / DT)
-    return ceps, ints, tabs, ntime_full
+    return ceps, ints, modes, ntime_full
 
 
 def sweep_ground_state(q, n):
@@ -153,9 +153,9 @@ def sweep_run(q, F, steps, warmup, compare_steps=1000):
     `value` is the faster form."""
     import threading
     import torch
-    _, _, _, ntime_full = sweep_fields(F, 1)
+    ceps, ints, modes, ntime_full = sweep_fields(F)
     nsteps = ntime_full if steps <= 0 else steps
-    ceps, ints, tabs, _ = sweep_fields(F, max(nsteps, compare_steps) + warmup + 8)
+    ntime = max(nsteps, compare_steps) + warmup + 8
     names = ["dipole_r", "dipole_a"]
     classes = []
     for k, n in enumerate(SWEEP_SIZES):
@@ -164,8 +164,7 @@ def sweep_run(q, F, steps, warmup, compare_steps=1000):
         g.set_potential("H_2D", softcore=1.0)
         g.set_integrator("BM4", DT)
         sl = slice(k * SWEEP_PER, (k + 1) * SWEEP_PER)
-        g.set_field_batch(np.stack([t[0] for t in tabs[sl]]), np.stack([t[1] for t in tabs[sl]]),
-                          np.stack([t[2] for t in tabs[sl]]))
+        g.set_field_modes(modes[sl], ntime)
         for b in range(SWEEP_PER):
             g.upload(b, psi0)
         from tdse_nd_mpi_amd import tuning

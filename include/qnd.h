@@ -191,6 +191,46 @@ int qnd_set_tuning(qnd_ctx *ctx, int vpt, int tw_contig, int tw_strided);
 int qnd_set_field_batch(qnd_ctx *ctx, int nb, int64_t ntime, const double *Et,
                         const double *At, const double *Eobs);
 
+/* Field synthesis on the device.  Field_vectors (src/parameters.c:899-949)
+ * at the stage times of Initialize_field_vectors (src/integrators.c:74-129)
+ * and at t_n = n DT (Q2), for nb instances of nmodes modes each: modes is a
+ * host array [nb][nmodes]; instance b's tables are those qnd_set_field_batch
+ * would receive from Field_vectors with its modes, formed by a kernel
+ * directly in the batch tables (no host tables, no copy).  A mode is the
+ * reference's laser mode (parameters.c:737-738: F0 = 5.33803e-9 sqrt(I),
+ * omega = 45.5637639947958 / lambda; A = F0 f (cos, xi sin)(omega t + phi) /
+ * omega on (x, y), E = -dA/dt with the envelope's analytic derivative), plus
+ * `angle`, an extension that rotates the mode's (x, y) vector in the xy plane
+ * (0 reproduces the reference).  Uses the context's integrator and DT.  With
+ * nb == 1 the single-instance tables of qnd_set_field (including A(t_n)) are
+ * set too.  Errors: QND_ESTATE without an integrator; QND_EINVAL for nb
+ * outside [1, n_slots], nmodes < 1, an unknown envelope, a wavelength <= 0,
+ * an intensity < 0 or a duration <= 0. */
+#define QND_ENV_CONSTANT 0
+#define QND_ENV_TRAPZ 1
+#define QND_ENV_COS2 2
+#define QND_ENV_COS4 3
+typedef struct qnd_mode {
+    double intensity;   /* W/cm^2 */
+    double wavelength;  /* nm */
+    double ellipticity; /* amplitude ratio of the y component */
+    double cep;         /* rad */
+    double delay;       /* a.u. of time */
+    double duration;    /* envelope length tau, a.u. of time */
+    double angle;       /* rad, rotation of the (x, y) vector (extension) */
+    int envelope;       /* QND_ENV_* */
+    int pad_;
+} qnd_mode;
+int qnd_set_field_modes(qnd_ctx *ctx, int nb, int64_t ntime, int nmodes,
+                        const qnd_mode *modes);
+
+/* Copy the batch field tables to the host (any pointer may be NULL):
+ * Et, At [nb][ntime][s][dim], Eobs [nb][ntime][dim] with nb and ntime those
+ * of the last qnd_set_field_batch / qnd_set_field_modes call (no reference
+ * counterpart: the executables write the single-instance tables to
+ * electric_field.real / vector_potential.real, src/integrators.c:103-127). */
+int qnd_get_field_batch(qnd_ctx *ctx, double *Et, double *At, double *Eobs);
+
 /* qnd_real_steps for the nb slots slot0 .. slot0+nb-1 at once: every pass of
  * a step is one launch over all instances (the batch is an extra,
  * untransformed leading axis), each instance reading its own field rows.

@@ -31,7 +31,16 @@ EXPORTS = ("qnd_create", "qnd_destroy", "qnd_last_error", "qnd_set_potential", "
            "qnd_real_steps", "qnd_final_observables", "qnd_event_record", "qnd_event_elapsed",
            "qnd_synchronize", "qnd_profile_enable", "qnd_profile_read", "qnd_version",
            "qnd_set_field_batch", "qnd_real_steps_batch", "qnd_set_tuning", "qnd_forward_checkpointed",
-           "qnd_adjoint", "qnd_observables_noalloc", "qnd_group_real_steps")
+           "qnd_adjoint", "qnd_observables_noalloc", "qnd_group_real_steps", "qnd_set_field_modes",
+           "qnd_get_field_batch")
+ENVELOPES = {"constant": 0, "trapz": 1, "cos2": 2, "cos4": 3}
+
+
+class ModeStruct(C.Structure):
+    """struct qnd_mode of include/qnd.h (one laser mode of Field_vectors)."""
+    _fields_ = [("intensity", C.c_double), ("wavelength", C.c_double), ("ellipticity", C.c_double),
+                ("cep", C.c_double), ("delay", C.c_double), ("duration", C.c_double), ("angle", C.c_double),
+                ("envelope", C.c_int), ("pad_", C.c_int)]
 
 _lib = None
 
@@ -72,6 +81,8 @@ def library():
     L.qnd_set_field.argtypes = [vp, C.c_int64, dp, dp, dp, dp]
     L.qnd_real_steps.argtypes = [vp, C.c_int, C.c_int, C.c_int64, C.c_int64, ip, C.c_int, dp]
     L.qnd_set_field_batch.argtypes = [vp, C.c_int, C.c_int64, dp, dp, dp]
+    L.qnd_set_field_modes.argtypes = [vp, C.c_int, C.c_int64, C.c_int, C.POINTER(ModeStruct)]
+    L.qnd_get_field_batch.argtypes = [vp, dp, dp, dp]
     L.qnd_set_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.qnd_forward_checkpointed.argtypes = [vp, C.c_int, C.c_int64, C.c_int64, dp, dp]
     L.qnd_adjoint.argtypes = [vp, C.c_int64, dp, C.c_double, dp, dp, dp]
@@ -280,6 +291,40 @@ class Grid:
         self._check_shape("Eobs", Eo, (nb, nt, self.dim))
         self._ck(library().qnd_set_field_batch(self.h, nb, nt, _d(Et), _d(At), _d(Eo)))
         self.batch = nb
+        self.field_ntime = nt
+
+    def set_field_modes(self, modes, ntime):
+        """Field tables formed on the device (qnd_set_field_modes): `modes` is
+        one list of field.Mode per instance (nb = len(modes), the same number
+        of modes each); nb == 1 also sets the single-instance tables."""
+        if not self.s:
+            raise QNDError("integrator not set (set_integrator) before the field")
+        nb = len(modes)
+        nm = len(modes[0]) if nb else 0
+        if any(len(m) != nm for m in modes):
+            raise QNDError("every instance needs the same number of modes")
+        arr = (ModeStruct * max(nb * nm, 1))()
+        for b, ms in enumerate(modes):
+            for k, m in enumerate(ms):
+                if m.env not in ENVELOPES:
+                    raise QNDError(f"Unknown envelope: {m.env}")
+                arr[b * nm + k] = ModeStruct(m.intensity, m.wavelength, m.xi, m.phi, m.delay, m.tau, m.angle,
+                                             ENVELOPES[m.env], 0)
+        self._ck(library().qnd_set_field_modes(self.h, nb, int(ntime), nm, arr))
+        self.batch = nb
+        self.field_ntime = int(ntime)
+        if nb == 1:
+            self.ntime = int(ntime)
+
+    def field_batch(self):
+        """The batch field tables on the device: Et, At [nb][ntime][s][dim],
+        Eobs [nb][ntime][dim]."""
+        nb, nt = self.batch, self.field_ntime
+        Et = np.zeros((nb, nt, self.s, self.dim))
+        At = np.zeros((nb, nt, self.s, self.dim))
+        Eo = np.zeros((nb, nt, self.dim))
+        self._ck(library().qnd_get_field_batch(self.h, _d(Et), _d(At), _d(Eo)))
+        return Et, At, Eo
 
     def real_steps_batch(self, slot0, nb, gauge, step0, nsteps, names=()):
         """real_steps for slots slot0..slot0+nb-1 in one launch per pass, each

@@ -2952,12 +2952,105 @@ int qnd_set_tuning(qnd_ctx *c, int vpt, int tw_contig, int tw_strided)
     return 0;
 }
 
-int qnd_set_field_batch(qnd_ctx *c, int nb, int64_t ntime, const double *Et, const double *At,
-                        const double *Eobs)
+// ---------------------------------------------------------------------
+// Field synthesis on the device: Field_vectors (src/parameters.c:899-949) at
+// the stage times of Initialize_field_vectors (src/integrators.c:74-129) and
+// at t_n, for a batch of instances with their own modes.
+// ---------------------------------------------------------------------
+struct FieldMode {
+    double w, F0, xi, phi, delay, tau, ca, sa;  // omega, amplitude, ellipticity, CEP, delay, duration, rotation
+    int env;
+};
+// stage-time increments of one step: t_0 = t_n + a0, t_{j+1} = t_j + inc[j]
+struct StageTimes {
+    double a0;
+    double inc[20];
+};
+
+// Envelope f and its derivative df at ts (time since the mode's delay).
+__device__ __forceinline__ void envelope_at(const FieldMode &m, double ts, double &f, double &df)
 {
-    if (!c || nb < 1 || ntime < 0) return QND_EINVAL;
-    if (!c->have_int) return fail(c, QND_ESTATE, "integrator not set (qnd_set_integrator)");
-    if (nb > c->n_slots) return fail(c, QND_EINVAL, "batch of %d exceeds the %d slots", nb, c->n_slots);
+    const double pi = 3.141592653589793;
+    if (m.env == QND_ENV_CONSTANT) {
+        f = 1.0;
+        df = 0.0;
+    } else if (m.env == QND_ENV_TRAPZ) {
+        // two-cycle sin^4 ramps either side of a flat top
+        const double tr = 4.0 * pi / m.w;
+        const bool up = ts >= 0.0 && ts <= tr, top = ts > tr && ts < m.tau - tr, dn = ts >= m.tau - tr && ts <= m.tau;
+        const double su = sin(0.5 * pi * ts / tr), sd = sin(0.5 * pi * (ts - m.tau) / tr);
+        f = (up ? su * su * su * su : 0.0) + (top ? 1.0 : 0.0) + (dn ? sd * sd * sd * sd : 0.0);
+        df = pi / tr * ((up ? su * su * sin(pi * ts / tr) : 0.0) + (dn ? sd * sd * sin(pi * (ts - m.tau) / tr) : 0.0));
+    } else {
+        const bool in = ts >= 0.0 && ts <= m.tau;
+        const double sn = sin(pi * ts / m.tau);
+        if (m.env == QND_ENV_COS2) {
+            f = in ? sn * sn : 0.0;
+            df = in ? pi * sin(2.0 * pi * ts / m.tau) / m.tau : 0.0;
+        } else {  // QND_ENV_COS4
+            f = in ? sn * sn * sn * sn : 0.0;
+            df = in ? 4.0 * pi * cos(pi * ts / m.tau) * sn * sn * sn / m.tau : 0.0;
+        }
+    }
+}
+
+// Sum of the modes' A(t), E(t) (x, y; the other components zero).
+__device__ __forceinline__ void field_at(const FieldMode *modes, int nmodes, double t, double *A, double *E)
+{
+    double ax = 0.0, ay = 0.0, ex = 0.0, ey = 0.0;
+    for (int k = 0; k < nmodes; ++k) {
+        const FieldMode &m = modes[k];
+        const double ts = t - m.delay;
+        double f, df;
+        envelope_at(m, ts, f, df);
+        double sa, ca;
+        sincos(m.w * ts + m.phi, &sa, &ca);
+        // A = F0 f (cos, xi sin)(w t + phi) / w, E = -dA/dt
+        const double mex = -m.F0 * (df * ca - f * m.w * sa) / m.w;
+        const double mey = -m.F0 * m.xi * (df * sa + f * m.w * ca) / m.w;
+        const double max_ = m.F0 * f * ca / m.w, may = m.F0 * m.xi * f * sa / m.w;
+        ex += m.ca * mex - m.sa * mey;
+        ey += m.sa * mex + m.ca * mey;
+        ax += m.ca * max_ - m.sa * may;
+        ay += m.sa * max_ + m.ca * may;
+    }
+    A[0] = ax;
+    A[1] = ay;
+    E[0] = ex;
+    E[1] = ey;
+}
+
+// One thread per (instance, step): the s stage rows and the t_n row.
+static __global__ void k_field(const FieldMode *modes, int nmodes, int nb, int64_t ntime, int s, int dim, double dt,
+                               StageTimes st, double *Et, double *At, double *Eo, double *Ao)
+{
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (int64_t)nb * ntime) return;
+    const int b = (int)(id / ntime);
+    const int64_t i = id - (int64_t)b * ntime;
+    const FieldMode *mb = modes + (size_t)b * nmodes;
+    const double tn = (double)i * dt;
+    double A[2], E[2];
+    double t = tn + st.a0;
+    for (int j = 0; j < s; ++j) {
+        field_at(mb, nmodes, t, A, E);
+        const int64_t row = (id * s + j) * dim;
+        for (int k = 0; k < dim; ++k) {
+            Et[row + k] = k < 2 ? E[k] : 0.0;
+            At[row + k] = k < 2 ? A[k] : 0.0;
+        }
+        if (j < s - 1) t += st.inc[j];
+    }
+    field_at(mb, nmodes, tn, A, E);
+    for (int k = 0; k < dim; ++k) {
+        Eo[id * dim + k] = k < 2 ? E[k] : 0.0;
+        if (Ao) Ao[id * dim + k] = k < 2 ? A[k] : 0.0;
+    }
+}
+
+// (Re)allocate the batch tables for nb instances of ntime steps, zeroed.
+static int alloc_field_batch(qnd_ctx *c, int nb, int64_t ntime)
+{
     const size_t ns = (size_t)nb * ntime * c->s * c->dim, no = (size_t)nb * ntime * c->dim;
     if (c->d_bEt) HIPCK(c, hipFree(c->d_bEt));
     if (c->d_bAt) HIPCK(c, hipFree(c->d_bAt));
@@ -2969,12 +3062,118 @@ int qnd_set_field_batch(qnd_ctx *c, int nb, int64_t ntime, const double *Et, con
     HIPCK(c, hipMemset(c->d_bEt, 0, (ns ? ns : 1) * sizeof(double)));
     HIPCK(c, hipMemset(c->d_bAt, 0, (ns ? ns : 1) * sizeof(double)));
     HIPCK(c, hipMemset(c->d_bEo, 0, (no ? no : 1) * sizeof(double)));
-    if (Et && ns) HIPCK(c, hipMemcpy(c->d_bEt, Et, ns * sizeof(double), hipMemcpyHostToDevice));
-    if (At && ns) HIPCK(c, hipMemcpy(c->d_bAt, At, ns * sizeof(double), hipMemcpyHostToDevice));
-    if (Eobs && no) HIPCK(c, hipMemcpy(c->d_bEo, Eobs, no * sizeof(double), hipMemcpyHostToDevice));
     c->bnb = nb;
     c->bntime = ntime;
     c->bfield_s = c->s;
+    return 0;
+}
+
+int qnd_set_field_modes(qnd_ctx *c, int nb, int64_t ntime, int nmodes, const qnd_mode *modes)
+{
+    if (!c) return QND_EINVAL;
+    if (!c->have_int) return fail(c, QND_ESTATE, "integrator not set (qnd_set_integrator)");
+    if (nb < 1 || nb > c->n_slots) return fail(c, QND_EINVAL, "batch of %d outside [1, %d] slots", nb, c->n_slots);
+    if (ntime < 0) return fail(c, QND_EINVAL, "ntime < 0");
+    if (nmodes < 1 || !modes) return fail(c, QND_EINVAL, "at least one mode per instance");
+    std::vector<FieldMode> h((size_t)nb * nmodes);
+    for (size_t k = 0; k < h.size(); ++k) {
+        const qnd_mode &m = modes[k];
+        if (m.envelope < QND_ENV_CONSTANT || m.envelope > QND_ENV_COS4)
+            return fail(c, QND_EINVAL, "mode %zu: unknown envelope %d", k, m.envelope);
+        if (!(m.wavelength > 0.0)) return fail(c, QND_EINVAL, "mode %zu: wavelength must be > 0", k);
+        if (!(m.intensity >= 0.0)) return fail(c, QND_EINVAL, "mode %zu: intensity must be >= 0", k);
+        if (m.envelope != QND_ENV_CONSTANT && !(m.duration > 0.0))
+            return fail(c, QND_EINVAL, "mode %zu: duration must be > 0", k);
+        FieldMode &f = h[k];
+        f.w = 45.5637639947958 / m.wavelength;  // src/parameters.c:809
+        f.F0 = 5.33803e-9 * sqrt(m.intensity);  // src/parameters.c:808
+        f.xi = m.ellipticity;
+        f.phi = m.cep;
+        f.delay = m.delay;
+        f.tau = m.duration;
+        f.ca = cos(m.angle);
+        f.sa = sin(m.angle);
+        if (m.angle == 0.0) {  // exactly the reference's (x, y) components
+            f.ca = 1.0;
+            f.sa = 0.0;
+        }
+        f.env = m.envelope;
+    }
+    StageTimes st{};
+    st.a0 = c->as_scaled[0];
+    for (int j = 0; j + 1 < c->s; ++j) st.inc[j] = c->as_scaled[2 * j + 1] + c->as_scaled[2 * j + 2];
+    int rc;
+    if ((rc = alloc_field_batch(c, nb, ntime))) return rc;
+    const size_t no = (size_t)nb * ntime * c->dim;
+    FieldMode *d_modes = nullptr;
+    double *d_Ao = nullptr;
+    auto cleanup = [&]() {
+        if (d_modes) hipFree(d_modes);
+        if (d_Ao) hipFree(d_Ao);
+    };
+    if (hipMalloc(&d_modes, h.size() * sizeof(FieldMode)) != hipSuccess ||
+        (nb == 1 && hipMalloc(&d_Ao, (no ? no : 1) * sizeof(double)) != hipSuccess)) {
+        cleanup();
+        return fail(c, QND_EHIP, "out of device memory for the field modes");
+    }
+    if (hipMemcpy(d_modes, h.data(), h.size() * sizeof(FieldMode), hipMemcpyHostToDevice) != hipSuccess) {
+        cleanup();
+        return fail(c, QND_EHIP, "copy of the field modes failed");
+    }
+    const int64_t nthreads = (int64_t)nb * ntime;
+    if (nthreads > 0) {
+        hipLaunchKernelGGL(k_field, dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, c->stream, d_modes, nmodes,
+                           nb, ntime, c->s, c->dim, c->dt, st, c->d_bEt, c->d_bAt, c->d_bEo, d_Ao);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(c, QND_EHIP, "field synthesis kernel: %s", hipGetErrorString(e));
+    }
+    rc = 0;
+    if (nb == 1) {
+        // the single-instance tables of qnd_set_field as well
+        const size_t ns = (size_t)ntime * c->s * c->dim;
+        std::vector<double> Et(ns), At(ns), Eo(no), Ao(no);
+        if ((ns && (hipMemcpy(Et.data(), c->d_bEt, ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                    hipMemcpy(At.data(), c->d_bAt, ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)) ||
+            (no && (hipMemcpy(Eo.data(), c->d_bEo, no * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                    hipMemcpy(Ao.data(), d_Ao, no * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))) {
+            cleanup();
+            return fail(c, QND_EHIP, "copy of the field tables failed");
+        }
+        cleanup();
+        return qnd_set_field(c, ntime, Et.data(), At.data(), Eo.data(), Ao.data());
+    }
+    cleanup();
+    return rc;
+}
+
+int qnd_get_field_batch(qnd_ctx *c, double *Et, double *At, double *Eobs)
+{
+    if (!c) return QND_EINVAL;
+    if (!c->d_bEt) return fail(c, QND_ESTATE, "no batch field tables (qnd_set_field_batch / qnd_set_field_modes)");
+    const size_t ns = (size_t)c->bnb * c->bntime * c->bfield_s * c->dim, no = (size_t)c->bnb * c->bntime * c->dim;
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    if (Et && ns) HIPCK(c, hipMemcpy(Et, c->d_bEt, ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (At && ns) HIPCK(c, hipMemcpy(At, c->d_bAt, ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (Eobs && no) HIPCK(c, hipMemcpy(Eobs, c->d_bEo, no * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int qnd_set_field_batch(qnd_ctx *c, int nb, int64_t ntime, const double *Et, const double *At,
+                        const double *Eobs)
+{
+    if (!c || nb < 1 || ntime < 0) return QND_EINVAL;
+    if (!c->have_int) return fail(c, QND_ESTATE, "integrator not set (qnd_set_integrator)");
+    if (nb > c->n_slots) return fail(c, QND_EINVAL, "batch of %d exceeds the %d slots", nb, c->n_slots);
+    const size_t ns = (size_t)nb * ntime * c->s * c->dim, no = (size_t)nb * ntime * c->dim;
+    int rc;
+    if ((rc = alloc_field_batch(c, nb, ntime))) return rc;
+    if (Et && ns) HIPCK(c, hipMemcpy(c->d_bEt, Et, ns * sizeof(double), hipMemcpyHostToDevice));
+    if (At && ns) HIPCK(c, hipMemcpy(c->d_bAt, At, ns * sizeof(double), hipMemcpyHostToDevice));
+    if (Eobs && no) HIPCK(c, hipMemcpy(c->d_bEo, Eobs, no * sizeof(double), hipMemcpyHostToDevice));
     return 0;
 }
 

@@ -32,6 +32,7 @@ class Mode:
 
     def __init__(self, intensity=1e14, wavelength=800.0, ellipticity=0.0, cep=0.0, delay=0.0,
                  envelope="cos2", duration=None, cycles=None, angle=0.0):
+        self.intensity, self.wavelength = intensity, wavelength
         self.w = AU_TIME_NM / wavelength
         self.F0 = AU_FIELD * np.sqrt(intensity)
         self.xi = ellipticity

@@ -43,6 +43,24 @@ def test_library_exports_every_declared_symbol():
     assert L.qnd_version().startswith(b"libqnd")
 
 
+def test_mode_struct_matches_header(tmp_path):
+    """The ctypes mirror of struct qnd_mode has the C layout (size and
+    offsets, from a C program compiled against include/qnd.h)."""
+    import ctypes
+    src = tmp_path / "m.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "qnd.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(qnd_mode), offsetof(qnd_mode, duration),'
+                   ' offsetof(qnd_mode, angle), offsetof(qnd_mode, envelope)); return 0;}\n')
+    exe = tmp_path / "m"
+    r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    M = q.ModeStruct
+    assert got == [ctypes.sizeof(M), M.duration.offset, M.angle.offset, M.envelope.offset]
+    assert q.ENVELOPES == {"constant": 0, "trapz": 1, "cos2": 2, "cos4": 3}
+
+
 @pytest.mark.skipif(have_gpu(), reason="checks the no-GPU failure path")
 def test_create_fails_loudly_without_gpu():
     with pytest.raises(q.QNDError, match="no HIP device"):
