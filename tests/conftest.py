@@ -4,6 +4,11 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# Under several pytest workers (-n) each oracle runs single-threaded: the
+# workers' spinning OpenMP teams would otherwise contend for the same cores
+# and a suite of seconds takes tens of minutes.
+if "PYTEST_XDIST_WORKER" in os.environ:
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
