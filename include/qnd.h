@@ -240,16 +240,20 @@ int qnd_real_steps_batch(qnd_ctx *ctx, int slot0, int nb, int gauge,
                          int64_t step0, int64_t nsteps, const int *codes,
                          int ncodes, double *obs_out);
 
-/* Several grids in one launch per pass (no reference counterpart: the
- * reference runs one grid per process; this serves sweeps over uneven grid
- * sizes).  Each member context ctxs[k] (2-D, one precision, one integrator
- * and DT, at most 4 members) propagates its batch of slots 0 .. nb_k - 1
- * under its qnd_set_field_batch tables, as qnd_real_steps_batch would, but
- * every pass of a step is one kernel launch over all members' instances.
- * Length gauge; recordable codes dipole_r, dipole_a, norm; obs_out[k]:
- * [nb_k][nsteps][width] host array of member k.  The steps after the first
- * advance a device step counter, so one step's launch sequence serves them
- * all; use_graph != 0 captures it once as a HIP graph and replays it. */
+/* Several grids stepped together (no reference counterpart: the reference
+ * runs one grid per process; this serves sweeps over uneven grid sizes).
+ * Each member context ctxs[k] (2-D, one precision, one integrator and DT, at
+ * most 4 members, each with its own stream) propagates its batch of slots
+ * 0 .. nb_k - 1 under its qnd_set_field_batch tables, as
+ * qnd_real_steps_batch would.  Length gauge; recordable codes dipole_r,
+ * dipole_a, norm; obs_out[k]: [nb_k][nsteps][width] host array of member k.
+ * The steps after the first advance a device step counter, so one step's
+ * launch sequence serves them all.  use_graph bit 0: that sequence is
+ * captured as a HIP graph and replayed; bit 1 (the branch form): each pass
+ * is each member's own kernel on its own stream, forked from and joined to
+ * the first member's stream every step (one graph launch then runs up to 8
+ * steps of all members); without bit 1 every pass of a step is one kernel
+ * launch over all members' instances. */
 int qnd_group_real_steps(qnd_ctx *const *ctxs, int nctx, int gauge,
                          int64_t step0, int64_t nsteps, const int *codes,
                          int ncodes, double *const *obs_out, int use_graph);

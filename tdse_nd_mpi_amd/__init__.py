@@ -370,10 +370,12 @@ class Grid:
         return cnt, ms
 
 
-def group_real_steps(grids, gauge, step0, nsteps, names=(), use_graph=True):
+def group_real_steps(grids, gauge, step0, nsteps, names=(), use_graph=True, branches=False):
     """qnd_group_real_steps: every grid's batch (its slots 0 .. nb-1 under its
-    set_field_batch tables) stepped together, one kernel launch per pass over
-    all grids; returns one [nb][nsteps][width] record array per grid."""
+    set_field_batch tables) stepped together: one kernel launch per pass over
+    all grids, or (branches) each grid's own kernels on its own stream, forked
+    and joined every step; use_graph replays captured HIP graphs.  Returns one
+    [nb][nsteps][width] record array per grid."""
     L = library()
     codes = np.array([OBS[n] for n in names], dtype=np.int32)
     outs, ptrs = [], []
@@ -386,7 +388,7 @@ def group_real_steps(grids, gauge, step0, nsteps, names=(), use_graph=True):
     pp = (C.POINTER(C.c_double) * len(grids))(*ptrs)
     rc = L.qnd_group_real_steps(hs, len(grids), GAUGES[gauge], int(step0), int(nsteps),
                                 codes.ctypes.data_as(C.POINTER(C.c_int)) if len(codes) else None, len(codes),
-                                pp, 1 if use_graph else 0)
+                                pp, (1 if use_graph else 0) | (2 if branches else 0))
     if rc:
         raise QNDError(L.qnd_last_error(grids[0].h).decode())
     return [o[:, :nsteps, :w] for o, w in outs]

@@ -1250,9 +1250,14 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
         return fail(c, QND_EUNSUPPORTED, "batched runs record dipole_r, dipole_a and norm only");
     if (ncodes > 8) return fail(c, QND_EINVAL, "at most 8 observables per step");
     if (want_obs && fused) {
+        // records of at least 4096 steps, so that the short calls after a
+        // first one allocate nothing
         size_t need = (size_t)nb * nsteps * width;
         if (need > c->rec_cap) {
+            need = std::max(need, (size_t)nb * 4096 * width);
             if (c->d_rec) HIPCK(c, hipFree(c->d_rec));
+            c->d_rec = nullptr;
+            c->rec_cap = 0;
             HIPCK(c, hipMalloc(&c->d_rec, need * sizeof(double)));
             c->rec_cap = need;
         }
@@ -1306,11 +1311,13 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
         if (rc) return rc;
         if (side_slots > 0) {
             if (c->side_cap < (size_t)side_slots * sbytes) {
+                // the ring's full size at once (later, longer calls reuse it)
+                const size_t cap = (size_t)std::min<int64_t>(side_slots_max, (int64_t)(QND_SIDE_MAX_BYTES / sbytes));
                 if (c->d_side) HIPCK(c, hipFree(c->d_side));
                 c->d_side = nullptr;
                 c->side_cap = 0;
-                HIPCK(c, hipMalloc(&c->d_side, (size_t)side_slots * sbytes));
-                c->side_cap = (size_t)side_slots * sbytes;
+                HIPCK(c, hipMalloc(&c->d_side, cap * sbytes));
+                c->side_cap = cap * sbytes;
             }
             gside = geometry(c, 0, side_slots, -1, false);
             if ((rc = ensure_part(c, (size_t)std::max(g0.grid, gside.grid) * QND_NOBS))) return rc;
@@ -2028,15 +2035,44 @@ struct GLaunch {
     int desc;  // -1: k_step_advance
 };
 
+// One member's pass in the branch form of a group step: the member's own
+// pass kernel (its values per thread and geometry) with its arguments.
+template <typename R> struct BLaunch {
+    const void *fn;
+    int grid, threads, nblk;
+    size_t lds;
+    unsigned geo, fl;
+    PassArgs<R> a;
+};
+template <typename R> using BLists = std::vector<std::vector<BLaunch<R>>>;
+
 // Record a pass over every member: its descriptor goes to `descs` (uploaded
-// once per call), its launch to `out`.
+// once per call), its launch to `out`; or, in the branch form (br given),
+// each member's own launch to its list (*br)[k].
 template <typename R, int KIND, int DIR>
 void record_group(const GroupGeo &G, const std::vector<PassArgs<R>> &as, std::vector<GroupArgs<R>> &descs,
-                  std::vector<GLaunch> &out)
+                  std::vector<GLaunch> &out, BLists<R> *br = nullptr)
 {
+    const int n = (int)as.size();
+    if (br) {
+        for (int k = 0; k < n; ++k) {
+            const Geo &g = G.g[k];
+            BLaunch<R> b;
+            b.fn = pass_fn<R, KIND, DIR>(g.vpt);
+            const int extra = (KIND == PK_KIN && (as[k].flags & PF_FINALIZE)) ? as[k].B.nb : 0;
+            b.grid = g.grid + extra;
+            b.threads = g.threads;
+            b.nblk = g.grid;
+            b.lds = g.lds;
+            b.geo = pass_geo(g.log2n, g.log2T, g.log2TW, as[k].P.pow2);
+            b.fl = as[k].flags;
+            b.a = as[k];
+            (*br)[k].push_back(b);
+        }
+        return;
+    }
     GroupArgs<R> ga;
     memset(&ga, 0, sizeof ga);
-    const int n = (int)as.size();
     ga.ncls = n;
     int w = 0;
     for (int k = 0; k < n; ++k) {
@@ -2052,6 +2088,74 @@ void record_group(const GroupGeo &G, const std::vector<PassArgs<R>> &as, std::ve
     ga.blk0[n] = w;
     descs.push_back(ga);
     out.push_back(GLaunch{(const void *)k_pass_group<R, KIND, DIR>, w, G.threads, G.lds, (int)descs.size() - 1});
+}
+
+// A context's device step counter (a replayed step sequence reads its field
+// rows and records at the counter's step).
+int64_t *member_step(qnd_ctx *c) { return reinterpret_cast<int64_t *>(c->d_small + SMALL_STEP); }
+
+// After a group call: every member's slots in the axis-0 Fourier
+// representation owing their mask, and the records copied out.
+int group_finish(qnd_ctx *const *cs, int n, int64_t nsteps, bool want_obs, int width, double *const *obs_out)
+{
+    for (int k = 0; k < n; ++k) {
+        qnd_ctx *c = cs[k];
+        for (int b = 0; b < c->bnb; ++b) {
+            c->slots[b].rep = REP_M;
+            c->slots[b].pend_mask = nsteps > 0 ? (c->P.mask_on != 0) : c->slots[b].pend_mask;
+        }
+        if (want_obs && obs_out && obs_out[k])
+            HIPCK(c, hipMemcpy(obs_out[k], c->d_rec, (size_t)c->bnb * nsteps * width * sizeof(double),
+                               hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+// Steps of a group in the branch form, in chunks: step(first, k) enqueues k
+// steps (the first of them as recorded for the call's first step when
+// `first`).  The first chunk is that one step, then chunks of the steady
+// step, eagerly or as captured HIP graphs of up to QND_GRAPH_STEPS steps each
+// (one graph launch runs that many steps of every member).
+#define QND_GRAPH_STEPS 32
+template <class F> int branch_steps(qnd_ctx *c0, int64_t nsteps, bool use_graph, F &&step)
+{
+    int rc;
+    int64_t done = 0;
+    if (nsteps > 0) {
+        if ((rc = step(true, 1))) return rc;
+        done = 1;
+    }
+    if (use_graph && !c0->prof) {
+        for (int64_t k = std::min<int64_t>(QND_GRAPH_STEPS, nsteps - done); k > 0;
+             k = std::min<int64_t>(k, nsteps - done)) {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            HIPCK(c0, hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
+            const int erc = step(false, k);
+            const hipError_t ec = hipStreamEndCapture(c0->stream, &graph);
+            if (erc) {
+                if (graph) hipGraphDestroy(graph);
+                return erc;
+            }
+            if (ec != hipSuccess) return fail(c0, QND_EHIP, "graph capture failed: %s", hipGetErrorString(ec));
+            hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (ei != hipSuccess) {
+                hipGraphDestroy(graph);
+                return fail(c0, QND_EHIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+            }
+            hipError_t el = hipSuccess;
+            for (; nsteps - done >= k && el == hipSuccess; done += k) el = hipGraphLaunch(exec, c0->stream);
+            hipGraphExecDestroy(exec);
+            hipGraphDestroy(graph);
+            if (el != hipSuccess) return fail(c0, QND_EHIP, "graph launch failed: %s", hipGetErrorString(el));
+        }
+    }
+    while (done < nsteps) {
+        const int64_t k = std::min<int64_t>(QND_GRAPH_STEPS, nsteps - done);
+        if ((rc = step(false, k))) return rc;
+        done += k;
+    }
+    return 0;
 }
 
 template <typename R>
@@ -2091,9 +2195,20 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
             }
         }
     }
+    // the branch form (use_graph bit 1): each member's own kernels and
+    // geometry on its own stream, the members' passes of a step concurrent
+    const bool branch = (use_graph & 2) != 0;
     GroupGeo G0, G1;
-    if (!group_geometry(cs, n, 0, PK_POT, G0) || !group_geometry(cs, n, 1, PK_KIN, G1))
+    if (branch) {
+        G0.g.resize(n);
+        G1.g.resize(n);
+        for (int k = 0; k < n; ++k) {
+            G0.g[k] = geometry(cs[k], 0, cs[k]->bnb, PK_POT);
+            G1.g[k] = geometry(cs[k], 1, cs[k]->bnb, PK_KIN);
+        }
+    } else if (!group_geometry(cs, n, 0, PK_POT, G0) || !group_geometry(cs, n, 1, PK_KIN, G1)) {
         return fail(c0, QND_EUNSUPPORTED, "no workgroup size fits every member of the group");
+    }
     for (int k = 0; k < n; ++k) {
         qnd_ctx *c = cs[k];
         if ((rc = ensure_part(c, (size_t)G0.g[k].grid * QND_NOBS))) return rc;
@@ -2112,16 +2227,20 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
         }
         HIPCK(c, hipStreamSynchronize(c->stream));  // member work on its own stream is done
     }
-    int64_t *d_step = reinterpret_cast<int64_t *>(c0->d_small + SMALL_STEP);
-    HIPCK(c0, hipMemcpyAsync(d_step, &step0, sizeof(int64_t), hipMemcpyHostToDevice, c0->stream));
-    HIPCK(c0, hipStreamSynchronize(c0->stream));
+    // the device step counter (the first member's; in the branch form each
+    // member advances its own)
+    int64_t *d_step = member_step(c0);
+    for (int k = 0; k < (branch ? n : 1); ++k) {
+        HIPCK(c0, hipMemcpyAsync(member_step(cs[k]), &step0, sizeof(int64_t), hipMemcpyHostToDevice, cs[k]->stream));
+        HIPCK(c0, hipStreamSynchronize(cs[k]->stream));
+    }
     ObsSpec spec;
     memset(&spec, 0, sizeof spec);
     spec.ncodes = ncodes;
     for (int k = 0; k < ncodes; ++k) spec.codes[k] = codes[k];
     auto batch_rows = [&](PassArgs<R> &a, qnd_ctx *c) {
         a.B.nb = c->bnb;
-        a.B.step = d_step;
+        a.B.step = branch ? member_step(c) : d_step;
         a.B.step_base = step0;
     };
     // one step: boundary pass (owed mask, records), K(a_0) with the records'
@@ -2132,7 +2251,7 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
     // then per stage the potential pass and the next kinetic pass, then the
     // step counter.
     std::vector<GroupArgs<R>> descs;
-    auto record_step = [&](const std::vector<char> &pend, std::vector<GLaunch> &out) -> int {
+    auto record_step = [&](const std::vector<char> &pend, std::vector<GLaunch> &out, BLists<R> *br) -> int {
         std::vector<PassArgs<R>> as(n);
         bool boundary = want_obs;
         for (int k = 0; k < n; ++k) boundary |= pend[k] != 0;
@@ -2151,7 +2270,7 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
                 }
                 if (c->round32) a.flags |= PF_ROUND32;
             }
-            record_group<R, PK_POT, +1>(G0, as, descs, out);
+            record_group<R, PK_POT, +1>(G0, as, descs, out, br);
         }
         auto kin = [&](int ia, bool fin) -> int {
             for (int k = 0; k < n; ++k) {
@@ -2188,8 +2307,8 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
                 }
                 if (c->round32) a.flags |= PF_ROUND32;
             }
-            if (fin) record_group<R, PK_KIN, -1>(G1, as, descs, out);
-            else record_group<R, PK_KINP, -1>(G1, as, descs, out);
+            if (fin) record_group<R, PK_KIN, -1>(G1, as, descs, out, br);
+            else record_group<R, PK_KINP, -1>(G1, as, descs, out, br);
             return 0;
         };
         int r;
@@ -2214,10 +2333,40 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
                 }
                 if (c->round32) a.flags |= PF_ROUND32;
             }
-            record_group<R, PK_POTL, +1>(G0, as, descs, out);
+            record_group<R, PK_POTL, +1>(G0, as, descs, out, br);
             if ((r = kin(ia + 1, false))) return r;
         }
         out.push_back(GLaunch{nullptr, 1, 64, 0, -1});
+        return 0;
+    };
+    // the branch form: k steps of every member on its own stream, forked
+    // from the first member's stream and joined back once; each member's
+    // steps advance its own step counter, so the members proceed at their
+    // own pace between the fork and the join
+    hipEvent_t ev_fork = nullptr, ev_join[QND_GROUP_MAX] = {};
+    BLists<R> bfirst(n), bsteady(n);  // the first and the steady step of each member
+    auto run_branch = [&](bool first, int64_t k) -> int {
+        HIPCK(c0, hipEventRecord(ev_fork, c0->stream));
+        for (int m = 1; m < n; ++m) HIPCK(c0, hipStreamWaitEvent(cs[m]->stream, ev_fork, 0));
+        for (int64_t i = 0; i < k; ++i) {
+            const BLists<R> &Lb = (first && i == 0) ? bfirst : bsteady;
+            size_t len = 0;
+            for (int m = 0; m < n; ++m) len = std::max(len, Lb[m].size());
+            for (size_t j = 0; j < len; ++j)  // the members' passes interleaved
+                for (int m = 0; m < n; ++m) {
+                    if (j >= Lb[m].size()) continue;
+                    BLaunch<R> b = Lb[m][j];
+                    void *args[] = {&b.a.data, &b.a.tw, &b.a.S, &b.geo, &b.fl, &b.nblk, &b.a};
+                    HIPCK(c0, hipLaunchKernel(b.fn, dim3(b.grid), dim3(b.threads), args, b.lds, cs[m]->stream));
+                }
+            for (int m = 0; m < n; ++m)
+                hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, cs[m]->stream, member_step(cs[m]));
+            HIPCK(c0, hipGetLastError());
+        }
+        for (int m = 1; m < n; ++m) {
+            HIPCK(c0, hipEventRecord(ev_join[m], cs[m]->stream));
+            HIPCK(c0, hipStreamWaitEvent(c0->stream, ev_join[m], 0));
+        }
         return 0;
     };
     const GroupArgs<R> *d_desc = nullptr;
@@ -2240,8 +2389,22 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
         steady[k] = cs[k]->P.mask_on ? 1 : 0;
     }
     std::vector<GLaunch> first, steadyL;
-    if ((rc = record_step(pend0, first))) return rc;  // the first step: what each member owes now
-    if ((rc = record_step(steady, steadyL))) return rc;
+    // the first step: what each member owes now
+    if ((rc = record_step(pend0, first, branch ? &bfirst : nullptr))) return rc;
+    if ((rc = record_step(steady, steadyL, branch ? &bsteady : nullptr))) return rc;
+    if (branch) {
+        for (int k = 1; k < n; ++k)
+            if (cs[k]->stream == c0->stream) return fail(c0, QND_EINVAL, "group members share a stream");
+        HIPCK(c0, hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        for (int k = 1; k < n; ++k) HIPCK(c0, hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming));
+        rc = branch_steps(c0, nsteps, (use_graph & 1) != 0,
+                          [&](bool first, int64_t k) { return run_branch(first, k); });
+        (void)hipEventDestroy(ev_fork);
+        for (int k = 1; k < n; ++k) (void)hipEventDestroy(ev_join[k]);
+        if (rc) return rc;
+        HIPCK(c0, hipStreamSynchronize(c0->stream));
+        return group_finish(cs, n, nsteps, want_obs, width, obs_out);
+    }
     {   // the call's descriptors, uploaded once (they serve every step)
         const size_t bytes = descs.size() * sizeof(GroupArgs<R>);
         if (bytes > c0->gdesc_cap) {
@@ -2261,7 +2424,7 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
     }
     bool prof = false;
     for (int k = 0; k < n; ++k) prof |= cs[k]->prof;
-    if (use_graph && nsteps - done > 0 && !prof) {
+    if ((use_graph & 1) && nsteps - done > 0 && !prof) {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIPCK(c0, hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal));
@@ -2286,17 +2449,7 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
     for (; done < nsteps; ++done)
         if ((rc = run_step(steadyL))) return rc;
     HIPCK(c0, hipStreamSynchronize(c0->stream));
-    for (int k = 0; k < n; ++k) {
-        qnd_ctx *c = cs[k];
-        for (int b = 0; b < c->bnb; ++b) {
-            c->slots[b].rep = REP_M;
-            c->slots[b].pend_mask = nsteps > 0 ? (c->P.mask_on != 0) : c->slots[b].pend_mask;
-        }
-        if (want_obs && obs_out && obs_out[k])
-            HIPCK(c, hipMemcpy(obs_out[k], c->d_rec, (size_t)c->bnb * nsteps * width * sizeof(double),
-                               hipMemcpyDeviceToHost));
-    }
-    return 0;
+    return group_finish(cs, n, nsteps, want_obs, width, obs_out);
 }
 
 // Write_final_observables numerics (src/io_functions.c:159-199)

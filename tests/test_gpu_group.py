@@ -2,10 +2,12 @@
 grid's own batched run (qnd_real_steps_batch) and, for one instance, the FP64
 oracle.  The group's steps after the first run from one launch sequence (a
 device step counter advances the field rows and records); with use_graph that
-sequence is a captured HIP graph, replayed once per step.  Each call's
-records (every replay) and final states are checked against the eager
-results, with the program's own eager work and synchronizations between the
-calls."""
+sequence is a captured HIP graph, replayed once per step.  In the branch
+form each grid runs its own kernels on its own stream, forked and joined every
+step, and one captured graph holds up to 32 steps (a call of 70 steps runs
+one step eagerly, replays a 32-step graph twice, then runs a 5-step one).  Each call's records (every replay)
+and final states are checked against the eager results, with the program's
+own eager work and synchronizations between the calls."""
 import numpy as np
 import pytest
 
@@ -18,7 +20,7 @@ pytestmark = pytest.mark.gpu
 # (grid, instances): a power-of-two grid with four values per thread, one with
 # eight, and a 3 * 2^k grid (twelve values per thread, radix-3 stage)
 MEMBERS = [((32, 32), 3), ((48, 48), 2), ((128, 64), 2)]
-DR, DT, NT = 0.4, 0.05, 24
+DR, DT, NT = 0.4, 0.05, 140
 NAMES = ["dipole_r", "dipole_a", "norm"]
 
 
@@ -50,8 +52,9 @@ def rel(a, b):
 
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_group_matches_member_batches(gpu, prec, use_graph):
-    print(f"case group prec={prec} graph={use_graph}", flush=True)
+@pytest.mark.parametrize("branches", [False, True])
+def test_group_matches_member_batches(gpu, prec, use_graph, branches):
+    print(f"case group prec={prec} graph={use_graph} branches={branches}", flush=True)
     inputs = [member_inputs(k, nr, nb) for k, (nr, nb) in enumerate(MEMBERS)]
     # each member alone, its batch in one launch per pass
     ref_obs, ref_psi = [], []
@@ -63,10 +66,10 @@ def test_group_matches_member_batches(gpu, prec, use_graph):
     # the group: two calls, eager work in between
     grids = [make_grid(nr, nb, prec, psi, tabs) for (nr, nb), (o, psi, tabs) in zip(MEMBERS, inputs)]
     half = NT // 2
-    o1 = q.group_real_steps(grids, "length", 0, half, NAMES, use_graph=use_graph)
+    o1 = q.group_real_steps(grids, "length", 0, half, NAMES, use_graph=use_graph, branches=branches)
     _ = grids[0].observables(1, ["norm"])          # eager work on a member between the calls
     grids[1].synchronize()
-    o2 = q.group_real_steps(grids, "length", half, NT - half, NAMES, use_graph=use_graph)
+    o2 = q.group_real_steps(grids, "length", half, NT - half, NAMES, use_graph=use_graph, branches=branches)
     # complex64: the eager observable call sends member 0 through one more
     # inverse / forward transform pair than its reference run, another FP32
     # rounding sequence
