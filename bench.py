@@ -483,26 +483,37 @@ def measured_traffic(kernel="void qnd::k_pass<float, 6, 1, 4>"):
             "source": os.path.relpath(TRAFFIC_PROFILE, ROOT) + " (default launch geometry)"}
 
 
+BARRIER_FORMS = {0: "one counter, acquire polls", 1: "one counter, relaxed polls + one acquire fence",
+                 2: "per-XCD counters then a top counter, relaxed polls + one acquire fence",
+                 3: "per-workgroup arrival flags polled by one workgroup, then a release flag"}
+
+
 def latency_probes():
-    """Grid barrier (one cooperative launch, device-scope release/acquire
-    atomics, bounded waits) against kernel boundary (back-to-back empty
-    launches), microseconds each, for the workgroup shapes of the 512^2
-    passes (tdse_nd_mpi_amd/libqnd_probe.so)."""
+    """Grid barriers (one cooperative launch, device-scope release/acquire
+    atomics, bounded waits; three forms, BARRIER_FORMS) against the kernel
+    boundary (back-to-back empty launches), microseconds each, for the
+    workgroup shapes of the 512^2 passes (tdse_nd_mpi_amd/libqnd_probe.so)."""
     import ctypes as C
     path = os.path.join(ROOT, "tdse_nd_mpi_amd", "libqnd_probe.so")
     if not os.path.exists(path):
         return None
     L = C.CDLL(path)
-    for f in (L.qnd_probe_barrier, L.qnd_probe_boundary):
-        f.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.qnd_probe_barrier.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.qnd_probe_boundary.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
     out = []
     for blocks, threads in ((256, 256), (512, 128), (1024, 64)):
-        b, k = C.c_double(), C.c_double()
-        rb = L.qnd_probe_barrier(blocks, threads, 2000, C.byref(b))
+        k = C.c_double()
         L.qnd_probe_boundary(blocks, threads, 2000, C.byref(k))
-        out.append({"workgroups": blocks, "threads": threads,
-                    "grid_barrier_us": b.value if rb == 0 else None, "barrier_status": rb,
-                    "kernel_boundary_us": k.value})
+        rec = {"workgroups": blocks, "threads": threads, "kernel_boundary_us": k.value, "grid_barrier_us": {}}
+        for form in BARRIER_FORMS:
+            b = C.c_double()
+            rb = L.qnd_probe_barrier(blocks, threads, 2000, form, C.byref(b))
+            rec["grid_barrier_us"][str(form)] = b.value if rb == 0 else None
+            if rb:
+                rec.setdefault("barrier_status", {})[str(form)] = rb
+        rec["best_grid_barrier_us"] = min((v for v in rec["grid_barrier_us"].values() if v is not None),
+                                          default=None)
+        out.append(rec)
     return out
 
 
@@ -826,8 +837,9 @@ def main():
                        "class_ms_per_step": {names[i]: float(ms[i]) / args.profile_steps for i in range(5)}}
     out["latency_probes"] = latency_probes()
     if out["latency_probes"]:
-        log("  grid barrier vs kernel boundary (us): " + ", ".join(
-            f"{p['workgroups']}x{p['threads']}: {p['grid_barrier_us']} vs {p['kernel_boundary_us']:.2f}"
+        out["latency_probes_forms"] = BARRIER_FORMS
+        log("  grid barrier (best form) vs kernel boundary (us): " + ", ".join(
+            f"{p['workgroups']}x{p['threads']}: {p['best_grid_barrier_us']} vs {p['kernel_boundary_us']:.2f}"
             for p in out["latency_probes"]))
     if not args.skip_cpu:
         log("cpu baseline (FP64 oracle on the host cores, best-case and faithful threading) ...")

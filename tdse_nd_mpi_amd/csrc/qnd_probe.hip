@@ -5,10 +5,25 @@
 // persistent kernel would replace each kernel boundary by a grid-wide
 // barrier.  These probes time both on the device:
 //   qnd_probe_barrier  : `iters` grid barriers in one cooperative launch of
-//                        `blocks` workgroups (a counter advanced with
-//                        device-scope release atomics and polled with
-//                        device-scope acquire loads, the cache maintenance a
-//                        real persistent pass needs);
+//                        `blocks` workgroups, in one of three forms:
+//                        0  one counter, advanced with device-scope release
+//                           atomics and polled with device-scope acquire
+//                           loads (each poll invalidates the L2);
+//                        1  one counter, polled with relaxed device-scope
+//                           loads and one acquire fence once it is reached;
+//                        2  two levels: a counter per group of workgroups
+//                           (blockIdx mod 8, the XCD the dispatcher deals the
+//                           workgroup to), whose last arrival advances the
+//                           top counter; relaxed polls of the top counter,
+//                           then one acquire fence.
+//                        3  no shared counter: each workgroup stores its own
+//                           arrival flag (one 128-byte line each), the first
+//                           workgroup polls all flags with relaxed loads (one
+//                           flag per thread) and then stores a release flag,
+//                           which the others poll; one acquire fence.
+//                        Every form makes the data written before the
+//                        barrier visible after it (release before arriving,
+//                        acquire after leaving), as a persistent pass needs;
 //   qnd_probe_boundary : `iters` back-to-back launches of an empty kernel of
 //                        `blocks` workgroups on one stream.
 // Every wait is bounded: a workgroup that polls more than `spin_max` times
@@ -18,21 +33,82 @@
 
 namespace {
 
-__global__ void k_probe_barrier(unsigned *count, unsigned *err, int iters, unsigned spin_max)
+// counters: [0] top (or the only) counter, [64 + 64 g] group g's counter
+// (separate 256-byte lines), err: the error flag
+// flags[32 * w]: workgroup w's arrival (form 3); flags[32 * gridDim.x]: the
+// release flag
+__device__ void barrier_flags(unsigned *flags, unsigned *err, unsigned k, unsigned spin_max)
 {
+    __shared__ int pending;
+    unsigned *rel = flags + 32 * gridDim.x;
+    if (threadIdx.x == 0) __hip_atomic_store(flags + 32 * blockIdx.x, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (blockIdx.x == 0) {
+        unsigned spins = 0;
+        for (;;) {
+            int mine = 0;
+            for (unsigned w = threadIdx.x; w < gridDim.x; w += blockDim.x)
+                mine |= __hip_atomic_load(flags + 32 * w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k;
+            if (threadIdx.x == 0) pending = 0;
+            __syncthreads();
+            if (mine) pending = 1;
+            __syncthreads();
+            const int p = pending;
+            __syncthreads();
+            if (!p) break;
+            if (++spins > spin_max) {
+                if (threadIdx.x == 0) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (threadIdx.x == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __hip_atomic_store(rel, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else if (threadIdx.x == 0) {
+        unsigned spins = 0;
+        while (__hip_atomic_load(rel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k) {
+            if (++spins > spin_max) {
+                __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+}
+
+__global__ void k_probe_barrier(unsigned *cnt, unsigned *err, int iters, unsigned spin_max, int form)
+{
+    const unsigned ng = 8, g = blockIdx.x % ng;
+    const unsigned members = gridDim.x / ng + (g < gridDim.x % ng ? 1u : 0u);
     for (int k = 0; k < iters; ++k) {
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const unsigned target = (unsigned)(k + 1) * gridDim.x;
-            __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (form == 3) {
+            barrier_flags(cnt, err, (unsigned)(k + 1), spin_max);
+        } else if (threadIdx.x == 0) {
+            unsigned *top = cnt;
+            unsigned target;
+            if (form == 2) {
+                unsigned *gc = cnt + 64 + 64 * g;
+                const unsigned old = __hip_atomic_fetch_add(gc, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                if (old + 1u == (unsigned)(k + 1) * members)
+                    __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                target = (unsigned)(k + 1) * (gridDim.x < ng ? gridDim.x : ng);
+            } else {
+                __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                target = (unsigned)(k + 1) * gridDim.x;
+            }
             unsigned spins = 0;
-            while (__hip_atomic_load(count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            const int order = form == 0 ? __ATOMIC_ACQUIRE : __ATOMIC_RELAXED;
+            while (__hip_atomic_load(top, order, __HIP_MEMORY_SCOPE_AGENT) < target) {
                 if (++spins > spin_max) {
                     __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            if (form != 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return;
@@ -48,14 +124,15 @@ __global__ void k_probe_empty(unsigned *sink)
 
 extern "C" {
 
-// Microseconds per grid barrier (*us), or a negative value on error: -1 the
-// launch was refused (e.g. the workgroups cannot all be resident), -2 a
-// bounded wait expired.
-int qnd_probe_barrier(int blocks, int threads, int iters, double *us)
+// Microseconds per grid barrier of form `form` (*us), or a negative value on
+// error: -1 the launch was refused (e.g. the workgroups cannot all be
+// resident), -2 a bounded wait expired.
+#define QND_PROBE_WORDS (32 * 1025 + 64 * 8 + 1)  // form 3's flags for up to 1024 workgroups, or form 2's
+int qnd_probe_barrier(int blocks, int threads, int iters, int form, double *us)
 {
     *us = -1.0;
     unsigned *d = nullptr;
-    if (hipMalloc(&d, 2 * sizeof(unsigned)) != hipSuccess) return -1;
+    if (hipMalloc(&d, QND_PROBE_WORDS * sizeof(unsigned)) != hipSuccess) return -1;
     hipStream_t st;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
         hipFree(d);
@@ -66,11 +143,11 @@ int qnd_probe_barrier(int blocks, int threads, int iters, double *us)
     hipEventCreate(&e1);
     int rc = 0;
     for (int rep = 0; rep < 2 && rc == 0; ++rep) {  // the first launch warms up
-        hipMemsetAsync(d, 0, 2 * sizeof(unsigned), st);
-        unsigned *count = d, *err = d + 1;
+        hipMemsetAsync(d, 0, QND_PROBE_WORDS * sizeof(unsigned), st);
+        unsigned *count = d, *err = d + QND_PROBE_WORDS - 1;
         unsigned spin_max = 1u << 20;
-        int it = iters;
-        void *args[] = {&count, &err, &it, &spin_max};
+        int it = iters, fm = form;
+        void *args[] = {&count, &err, &it, &spin_max, &fm};
         hipEventRecord(e0, st);
         if (hipLaunchCooperativeKernel((const void *)k_probe_barrier, dim3(blocks), dim3(threads), args, 0, st) !=
             hipSuccess) {
@@ -80,9 +157,9 @@ int qnd_probe_barrier(int blocks, int threads, int iters, double *us)
         }
         hipEventRecord(e1, st);
         hipStreamSynchronize(st);
-        unsigned h[2] = {0, 0};
-        hipMemcpy(h, d, sizeof h, hipMemcpyDeviceToHost);
-        if (h[1] != 0u) rc = -2;
+        unsigned h = 0;
+        hipMemcpy(&h, d + QND_PROBE_WORDS - 1, sizeof h, hipMemcpyDeviceToHost);
+        if (h != 0u) rc = -2;
         float ms = 0.f;
         hipEventElapsedTime(&ms, e0, e1);
         *us = 1e3 * ms / iters;
