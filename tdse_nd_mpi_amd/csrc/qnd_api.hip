@@ -1068,6 +1068,11 @@ template <typename R> int gram(qnd_ctx *c, int nprev)
 // ---------------------------------------------------------------------
 // imaginary time (src/integrators.c:135-163 + src/QuantumND_imag.c:265-311)
 // ---------------------------------------------------------------------
+// Blocks of the first-level sums of the Gram-Schmidt dot partials: none for
+// up to 4096 rows (the coefficient kernel sums them directly), else one per
+// 128 rows, at most 256.
+int gs_row_blocks(int nrows) { return nrows <= 4096 ? 0 : std::min(256, (nrows + 127) / 128); }
+
 template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int gs)
 {
     const int d = c->dim, s = c->s;
@@ -1082,7 +1087,10 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
     HIPCK(c, hipMemsetAsync(coef, 0, (2 * nprev + 4) * sizeof(double), c->stream));
     const Geo g0 = geometry(c, 0);
     const int npart = 2 * (nprev + 1);
-    if (gs && (rc = ensure_part(c, (size_t)g0.grid * npart))) return rc;
+    // Gram-Schmidt dot partials: one row per workgroup of the last pass, and
+    // (large grids) one row per block of their first-level sums behind them
+    const int gs_blocks = gs_row_blocks(g0.grid);
+    if (gs && (rc = ensure_part(c, (size_t)(g0.grid + gs_blocks) * npart))) return rc;
     const double inv_n0 = 1.0 / (double)c->nr[0];
     bool pending = false;
     // Gram-Schmidt collapse (src/QuantumND_imag.c:297-304): the coefficient
@@ -1141,8 +1149,19 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
         if ((rc = launch_pot<R>(c, g0, a))) return rc;
         if (gs) {
             ProfScope ps(c, PC_SMALL);
-            hipLaunchKernelGGL(k_gs_coeffs, dim3(1), dim3(reduce_threads(g0.grid)), 0, c->stream, c->d_part,
-                               g0.grid, npart,
+            const double *rows = c->d_part;
+            int nrows = g0.grid;
+            if (gs_blocks > 0) {
+                // first-level sums of the rows in gs_blocks fixed slices (many
+                // CUs), so that the one-block coefficient kernel sums a few
+                // hundred rows instead of tens of thousands
+                double *sums = c->d_part + (size_t)g0.grid * npart;
+                hipLaunchKernelGGL(k_row_blocks, dim3(gs_blocks), dim3(256), 0, c->stream, c->d_part, g0.grid, npart,
+                                   sums);
+                rows = sums;
+                nrows = gs_blocks;
+            }
+            hipLaunchKernelGGL(k_gs_coeffs, dim3(1), dim3(reduce_threads(nrows)), 0, c->stream, rows, nrows, npart,
                                nprev, c->P.dNr * inv_n0, c->d_small + SMALL_GRAM, coef, (double)it);
             HIPCK(c, hipGetLastError());
             pending = true;

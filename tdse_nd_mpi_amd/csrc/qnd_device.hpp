@@ -1921,6 +1921,26 @@ static __global__ void __launch_bounds__(1024) k_obs_finalize(const double *__re
 // coef = [c_0 .. c_{N-1} (re,im), invnorm, flag, norm, step of the first collapse]
 // `step` is the caller's index of the step whose dots these are: the first
 // collapse (norm below 1e-12) records it beside the sticky flag.
+// First-level sums of a [nrows][npart] partials array: block b sums the
+// rows of its fixed slice [b * per, (b + 1) * per), per = ceil(nrows /
+// gridDim.x), into out[b][0 .. npart).  The slices and the order inside one
+// depend on nrows and the grid only, so the sums are deterministic.
+static __global__ void __launch_bounds__(256) k_row_blocks(const double *__restrict__ part, int nrows, int npart,
+                                                    double *out)
+{
+    __shared__ double red[16 * 8];
+    const int per = (nrows + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int r0 = (int)blockIdx.x * per, r1 = min(nrows, r0 + per);
+    const int n = r1 > r0 ? r1 - r0 : 0;
+    for (int q0 = 0; q0 < npart; q0 += 8) {
+        double tot[8];
+        block_column_sums<8>(part + (int64_t)r0 * npart, n, npart, q0, tot, red);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 8 && q0 + q < npart; ++q) out[(int64_t)blockIdx.x * npart + q0 + q] = tot[q];
+        lds_barrier();
+    }
+}
+
 static __global__ void __launch_bounds__(1024) k_gs_coeffs(const double *__restrict__ part, int nwg, int npart,
                                                     int nprev, double scale, const double *__restrict__ gram,
                                                     double *coef, double step)
