@@ -517,6 +517,27 @@ def latency_probes():
     return out
 
 
+def tile_probes():
+    """The memory side of a 512^3 complex64 line pass alone (an in-place
+    scaled copy read and written as tiles of TW lines of 512 points, the
+    passes' lane mapping and XCD tile order): microseconds per pass and GB/s
+    of its 2 GiB, per axis and tile width (tdse_nd_mpi_amd/libqnd_probe.so)."""
+    import ctypes as C
+    path = os.path.join(ROOT, "tdse_nd_mpi_amd", "libqnd_probe.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    L.qnd_probe_tiles.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    out = []
+    for axis, tw, vpt in ((2, 4, 8), (1, 8, 8), (0, 4, 8), (0, 8, 8), (0, 16, 8), (0, 16, 16), (0, 32, 16)):
+        us = C.c_double()
+        rc = L.qnd_probe_tiles(axis, tw, vpt, 10, C.byref(us))
+        out.append({"axis": axis, "lines_per_tile": tw, "values_per_thread": vpt,
+                    "us": us.value if rc == 0 else None,
+                    "GBs": 2 * 8 * 512 ** 3 / (us.value * 1e3) if rc == 0 else None})
+    return out
+
+
 def copy_bandwidth(reps=10):
     """Measured device-copy bandwidth (GB/s, read + write) of a 1 GiB FP32
     copy, timed with events on torch's stream, where the copies run."""
@@ -881,6 +902,7 @@ def main():
     if args.big3d_steps > 0:
         log(f"3-D 512^3: {args.big3d_steps} imaginary and real steps ...")
         out["big3d_512"] = big3d_run(q, F, args.big3d_steps, args.big3d_steps, fixed3d)
+        out["big3d_512"]["tile_probes"] = tile_probes()
         log(f"  real {out['big3d_512']['real_BM4_ms_per_step']:.2f} ms/step")
     if args.adjoint_steps > 0:
         log(f"adjoint gradient: 256^2, {args.adjoint_steps} steps, FD on {args.fd_samples} samples ...")

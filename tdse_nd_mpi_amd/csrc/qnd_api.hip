@@ -1338,7 +1338,10 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
                 HIPCK(c, hipMalloc(&c->d_side, cap * sbytes));
                 c->side_cap = cap * sbytes;
             }
-            gside = geometry(c, 0, side_slots, -1, false);
+            // the record pass's geometry for a full ring (its partial flushes
+            // take the same tiles over fewer instances), so that calls of
+            // different lengths reuse one launch shape
+            gside = geometry(c, 0, (int)(c->side_cap / sbytes), -1, false);
             if ((rc = ensure_part(c, (size_t)std::max(g0.grid, gside.grid) * QND_NOBS))) return rc;
         }
     }
@@ -1349,7 +1352,9 @@ int real_steps_impl(qnd_ctx *c, int slot0, int nb, int gauge, int64_t step0, int
     // ring (a batch of side_n instances), then one finalize block per record
     auto flush_side = [&]() -> int {
         if (side_n == 0) return 0;
-        const Geo gs = side_n == side_slots ? gside : geometry(c, 0, side_n, -1, false);
+        Geo gs = gside;
+        gs.nlines = (c->N / gs.n) * side_n;
+        gs.grid = (int)(gs.nlines / gs.TW);
         PassArgs<R> a = base_args<R>(c, side, gs);
         a.scale1 = inv_n0;
         a.flags = PF_OBS | PF_MASK | PF_NOSTORE;

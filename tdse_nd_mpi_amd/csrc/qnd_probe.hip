@@ -25,7 +25,13 @@
 //                        barrier visible after it (release before arriving,
 //                        acquire after leaving), as a persistent pass needs;
 //   qnd_probe_boundary : `iters` back-to-back launches of an empty kernel of
-//                        `blocks` workgroups on one stream.
+//                        `blocks` workgroups on one stream;
+//   qnd_probe_tiles    : the memory side of a 512^3 complex64 line pass alone:
+//                        an in-place scaled copy of the state read and
+//                        written as tiles of TW lines of 512 points along one
+//                        axis, with the passes' lane mapping and XCD tile
+//                        order (what the tile shape costs without the
+//                        transforms).
 // Every wait is bounded: a workgroup that polls more than `spin_max` times
 // sets the error flag and leaves, so the kernel always ends.
 #include <hip/hip_runtime.h>
@@ -115,6 +121,35 @@ __global__ void k_probe_barrier(unsigned *cnt, unsigned *err, int iters, unsigne
     }
 }
 
+// Tile pass of TW lines (VPT values per thread, T = 512 / VPT threads per
+// line): line L starts at ((L >> log2S) << (9 + log2S)) + (L & (S - 1)) and
+// its element i is i * S further (S = 1: contiguous lines at L * 512).  Every
+// index is inside the 512^3 grid for L < 512^2.
+template <int TW, int VPT>
+__global__ void __launch_bounds__(1024) k_probe_tile(float2 *data, int64_t S, int log2S, float scale)
+{
+    constexpr int n = 512, T = n / VPT;
+    constexpr int log2TW = TW == 1 ? 0 : TW == 2 ? 1 : TW == 4 ? 2 : TW == 8 ? 3 : TW == 16 ? 4 : 5;
+    constexpr int log2T = T == 32 ? 5 : T == 64 ? 6 : 7;
+    const int tid = threadIdx.x, bid = blockIdx.x, nblk = gridDim.x;
+    const int tb = (nblk % 8 == 0) ? (bid % 8) * (nblk / 8) + bid / 8 : bid;
+    int l, t;
+    if (S == 1) { l = tid >> log2T; t = tid & (T - 1); }
+    else { l = tid & (TW - 1); t = tid >> log2TW; }
+    const int64_t L = (int64_t)tb * TW + l;
+    const int64_t base = S == 1 ? L * n : ((L >> log2S) << (9 + log2S)) + (L & (S - 1));
+    float2 x[VPT];
+#pragma unroll
+    for (int m = 0; m < VPT; ++m) x[m] = data[base + (int64_t)(t + m * T) * S];
+#pragma unroll
+    for (int m = 0; m < VPT; ++m) {
+        x[m].x *= scale;
+        x[m].y *= scale;
+    }
+#pragma unroll
+    for (int m = 0; m < VPT; ++m) data[base + (int64_t)(t + m * T) * S] = x[m];
+}
+
 __global__ void k_probe_empty(unsigned *sink)
 {
     if (sink && threadIdx.x == 0 && blockIdx.x == 0x7fffffff) *sink = 1u;  // never taken
@@ -193,6 +228,60 @@ int qnd_probe_boundary(int blocks, int threads, int iters, double *us)
     hipEventDestroy(e1);
     hipStreamDestroy(st);
     return 0;
+}
+
+// Microseconds per tile pass over a 512^3 complex64 grid along `axis` (0, 1
+// or 2) with `tw` lines per tile (1 to 32) and `vpt` values per thread (8 or
+// 16; tw * 512 / vpt <= 1024 threads), averaged over `iters` passes; -1 on a
+// refused shape or a HIP error.
+int qnd_probe_tiles(int axis, int tw, int vpt, int iters, double *us)
+{
+    *us = -1.0;
+    const int64_t n = 512, N = n * n * n;
+    const int64_t S = axis == 0 ? n * n : axis == 1 ? n : 1;
+    const int log2S = axis == 0 ? 18 : axis == 1 ? 9 : 0;
+    const int threads = tw * (int)(n / vpt);
+    if (threads > 1024 || (vpt != 8 && vpt != 16)) return -1;
+    const int blocks = (int)(N / n / tw);
+    const void *f = nullptr;
+#define QND_TILE(TWc, VPTc) \
+    if (tw == TWc && vpt == VPTc) f = (const void *)k_probe_tile<TWc, VPTc>;
+    QND_TILE(1, 8) QND_TILE(2, 8) QND_TILE(4, 8) QND_TILE(8, 8) QND_TILE(16, 8)
+    QND_TILE(4, 16) QND_TILE(8, 16) QND_TILE(16, 16) QND_TILE(32, 16)
+#undef QND_TILE
+    if (!f) return -1;
+    float2 *d = nullptr;
+    if (hipMalloc(&d, N * sizeof(float2)) != hipSuccess) return -1;
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        hipFree(d);
+        return -1;
+    }
+    hipMemsetAsync(d, 0, N * sizeof(float2), st);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    int64_t Sa = S;
+    int l2 = log2S;
+    float one = 1.0f;
+    void *args[] = {&d, &Sa, &l2, &one};
+    int rc = 0;
+    for (int k = 0; k < 2 && rc == 0; ++k)  // warm-up
+        if (hipLaunchKernel(f, dim3(blocks), dim3(threads), args, 0, st) != hipSuccess) rc = -1;
+    hipEventRecord(e0, st);
+    for (int k = 0; k < iters && rc == 0; ++k)
+        if (hipLaunchKernel(f, dim3(blocks), dim3(threads), args, 0, st) != hipSuccess) rc = -1;
+    hipEventRecord(e1, st);
+    if (hipStreamSynchronize(st) != hipSuccess) rc = -1;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (rc == 0) *us = 1e3 * ms / iters;
+    (void)hipGetLastError();
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(st);
+    hipFree(d);
+    return rc;
 }
 
 }  // extern "C"
