@@ -122,6 +122,7 @@ struct qnd_ctx {
     int64_t kax_gen = -1;
     int kax_prec = -1;
     bool use_kax = true;                  // QND_KAX=0: kinetic factors formed per element instead
+    int gs_split = -1;                    // QND_GS_SPLIT=0/1: Gram-Schmidt fused / split (-1: by size)
     // descriptors of the group launches this context leads (qnd_group_real_steps)
     void *d_gdesc = nullptr;
     size_t gdesc_cap = 0;
@@ -1073,6 +1074,36 @@ template <typename R> int gram(qnd_ctx *c, int nprev)
 // 128 rows, at most 256.
 int gs_row_blocks(int nrows) { return nrows <= 4096 ? 0 : std::min(256, (nrows + 127) / 128); }
 
+// Gram-Schmidt of large grids as two contiguous elementwise kernels
+// (k_gs_dots after the step, k_gs_update before the next), launched for NP
+// previous states (at most QND_GRAM_MAX) on GS_SPLIT_BLOCKS workgroups.
+#define GS_SPLIT_BLOCKS 2048
+#define QND_GS_SPLIT_MIN ((int64_t)1 << 24)  // grid points from which the split form is used
+template <typename R, int NP> void launch_gs_split(qnd_ctx *c, bool dots, C<R> *psi, double *coef)
+{
+    const int grid = (int)std::min<int64_t>(GS_SPLIT_BLOCKS, (c->N + 255) / 256);
+    if (dots)
+        hipLaunchKernelGGL((k_gs_dots<R, NP>), dim3(grid), dim3(256), 0, c->stream, (const C<R> *)psi,
+                           (const C<R> *)state_ptr<R>(c, 0), c->N, c->N, c->d_part);
+    else
+        hipLaunchKernelGGL((k_gs_update<R, NP>), dim3(grid), dim3(256), 0, c->stream, psi,
+                           (const C<R> *)state_ptr<R>(c, 0), c->N, c->N, (const double *)coef);
+}
+template <typename R> void gs_split(qnd_ctx *c, int nprev, bool dots, C<R> *psi, double *coef)
+{
+    switch (nprev) {
+    case 0: launch_gs_split<R, 0>(c, dots, psi, coef); break;
+    case 1: launch_gs_split<R, 1>(c, dots, psi, coef); break;
+    case 2: launch_gs_split<R, 2>(c, dots, psi, coef); break;
+    case 3: launch_gs_split<R, 3>(c, dots, psi, coef); break;
+    case 4: launch_gs_split<R, 4>(c, dots, psi, coef); break;
+    case 5: launch_gs_split<R, 5>(c, dots, psi, coef); break;
+    case 6: launch_gs_split<R, 6>(c, dots, psi, coef); break;
+    case 7: launch_gs_split<R, 7>(c, dots, psi, coef); break;
+    default: launch_gs_split<R, 8>(c, dots, psi, coef); break;
+    }
+}
+
 template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int gs)
 {
     const int d = c->dim, s = c->s;
@@ -1087,10 +1118,16 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
     HIPCK(c, hipMemsetAsync(coef, 0, (2 * nprev + 4) * sizeof(double), c->stream));
     const Geo g0 = geometry(c, 0);
     const int npart = 2 * (nprev + 1);
-    // Gram-Schmidt dot partials: one row per workgroup of the last pass, and
-    // (large grids) one row per block of their first-level sums behind them
-    const int gs_blocks = gs_row_blocks(g0.grid);
-    if (gs && (rc = ensure_part(c, (size_t)(g0.grid + gs_blocks) * npart))) return rc;
+    // Large grids (or QND_GS_SPLIT=1) take the split form: the dots and the
+    // update as contiguous kernels of their own instead of fused into the
+    // first and last pass of a step
+    const bool split = gs && nprev <= QND_GRAM_MAX &&
+                       (c->gs_split > 0 || (c->gs_split < 0 && c->N >= QND_GS_SPLIT_MIN));
+    const int dot_rows = split ? (int)std::min<int64_t>(GS_SPLIT_BLOCKS, (c->N + 255) / 256) : g0.grid;
+    // Gram-Schmidt dot partials: one row per workgroup that wrote them, and
+    // (many rows) one row per block of their first-level sums behind them
+    const int gs_blocks = gs_row_blocks(dot_rows);
+    if (gs && (rc = ensure_part(c, (size_t)(dot_rows + gs_blocks) * npart))) return rc;
     const double inv_n0 = 1.0 / (double)c->nr[0];
     bool pending = false;
     // Gram-Schmidt collapse (src/QuantumND_imag.c:297-304): the coefficient
@@ -1118,10 +1155,16 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
                 a.cv1 = c->as_scaled[2 * ia - 1];
             }
             if (ia == 0 && pending) {
-                a.flags |= PF_GS_UPDATE;
-                a.prev = state_ptr<R>(c, 0);
-                a.nprev = nprev;
-                a.gs_coef = coef;
+                if (split) {
+                    ProfScope ps(c, PC_SMALL);
+                    gs_split<R>(c, nprev, false, psi, coef);
+                    HIPCK(c, hipGetLastError());
+                } else {
+                    a.flags |= PF_GS_UPDATE;
+                    a.prev = state_ptr<R>(c, 0);
+                    a.nprev = nprev;
+                    a.gs_coef = coef;
+                }
                 pending = false;
             }
             const double ck = c->as_scaled[2 * ia] + c->as_scaled[2 * ia + 1];
@@ -1139,7 +1182,7 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
         a.flags = PF_V1 | PF_SECOND;
         a.vmode = VM_IMAG;
         a.cv1 = c->as_scaled[2 * s - 1];
-        if (gs) {
+        if (gs && !split) {
             a.flags |= PF_GS_DOTS;
             a.prev = state_ptr<R>(c, 0);
             a.nprev = nprev;
@@ -1149,14 +1192,15 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
         if ((rc = launch_pot<R>(c, g0, a))) return rc;
         if (gs) {
             ProfScope ps(c, PC_SMALL);
+            if (split) gs_split<R>(c, nprev, true, psi, coef);
             const double *rows = c->d_part;
-            int nrows = g0.grid;
+            int nrows = dot_rows;
             if (gs_blocks > 0) {
                 // first-level sums of the rows in gs_blocks fixed slices (many
                 // CUs), so that the one-block coefficient kernel sums a few
                 // hundred rows instead of tens of thousands
-                double *sums = c->d_part + (size_t)g0.grid * npart;
-                hipLaunchKernelGGL(k_row_blocks, dim3(gs_blocks), dim3(256), 0, c->stream, c->d_part, g0.grid, npart,
+                double *sums = c->d_part + (size_t)dot_rows * npart;
+                hipLaunchKernelGGL(k_row_blocks, dim3(gs_blocks), dim3(256), 0, c->stream, c->d_part, dot_rows, npart,
                                    sums);
                 rows = sums;
                 nrows = gs_blocks;
@@ -1183,10 +1227,13 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
     }
     if (pending) {
         ProfScope ps(c, PC_SMALL);
-        hipLaunchKernelGGL(k_elem<R>, dim3(flat_grid(c->N, 4, 1024, 16384)), dim3(256),
-                           0, c->stream, (int)EM_GS_UPDATE, psi,
-                           (const C<R> *)psi, 0, 0.0, c->N, c->P, (const C<R> *)state_ptr<R>(c, 0),
-                           c->N, nprev, (const double *)coef);
+        if (nprev <= QND_GRAM_MAX)
+            gs_split<R>(c, nprev, false, psi, coef);
+        else
+            hipLaunchKernelGGL(k_elem<R>, dim3(flat_grid(c->N, 4, 1024, 16384)), dim3(256),
+                               0, c->stream, (int)EM_GS_UPDATE, psi,
+                               (const C<R> *)psi, 0, 0.0, c->N, c->P, (const C<R> *)state_ptr<R>(c, 0),
+                               c->N, nprev, (const double *)coef);
         HIPCK(c, hipGetLastError());
     }
     if (gs) {
@@ -2775,6 +2822,7 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     if (const char *e = getenv("QND_WAVE_LINES")) c->wave_lines = atoi(e) != 0;
     if (const char *e = getenv("QND_ROUND_STATE")) c->round32 = precision == QND_C128 && atoi(e) != 0;
     if (const char *e = getenv("QND_KAX")) c->use_kax = atoi(e) != 0;
+    if (const char *e = getenv("QND_GS_SPLIT")) c->gs_split = atoi(e) != 0 ? 1 : 0;
     if (const char *e = getenv("QND_VPT")) {  // tuning knob: values per thread
         const int v = atoi(e);
         if (v == 4 || v == 8) c->vpt_override = v;

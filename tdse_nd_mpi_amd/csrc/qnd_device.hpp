@@ -2181,6 +2181,70 @@ __global__ void __launch_bounds__(256) k_gram(const C<R> *__restrict__ base, int
         for (int q = 0; q < 2 * NPAIR; ++q) partials[(int64_t)blockIdx.x * (2 * NPAIR) + q] = acc[q];
 }
 
+// Gram-Schmidt of a state v against NP previous states (slots base + j *
+// stride) as two elementwise kernels over contiguous memory, for large grids
+// (the passes' fused update and dots read the previous states with the
+// pass's own tile shape, strided along axis 0).
+// k_gs_dots: block b writes row b of `part`: the FP64 sums of Re and Im of
+// conj(prev_j) v for j < NP, then |v|^2 and 0 -- the row layout of the fused
+// dots, for k_gs_coeffs.
+template <typename R, int NP>
+__global__ void __launch_bounds__(256) k_gs_dots(const C<R> *__restrict__ v, const C<R> *__restrict__ base,
+                                                 int64_t stride, int64_t N, double *part)
+{
+    constexpr int K = 2 * (NP + 1);
+    __shared__ double red[16 * K];
+    double acc[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) acc[q] = 0.0;
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < N;
+         f += (int64_t)gridDim.x * blockDim.x) {
+        const C<R> x = v[f];
+        C<R> pj[NP > 0 ? NP : 1];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) pj[j] = base[j * stride + f];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            acc[2 * j] += (double)pj[j].x * (double)x.x + (double)pj[j].y * (double)x.y;
+            acc[2 * j + 1] += (double)pj[j].x * (double)x.y - (double)pj[j].y * (double)x.x;
+        }
+        acc[2 * NP] += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+    }
+    block_reduce_lds(acc, K, red);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < K; ++q) part[(int64_t)blockIdx.x * K + q] = acc[q];
+}
+// k_gs_update: v = (v - sum_j c_j prev_j) * invnorm from the coefficients of
+// k_gs_coeffs (coef[2j], coef[2j+1], coef[2 NP]), in the order and FP64
+// arithmetic of the fused update.
+template <typename R, int NP>
+__global__ void __launch_bounds__(256) k_gs_update(C<R> *__restrict__ v, const C<R> *__restrict__ base,
+                                                   int64_t stride, int64_t N, const double *__restrict__ coef)
+{
+    double cr[NP > 0 ? NP : 1], ci[NP > 0 ? NP : 1];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        cr[j] = coef[2 * j];
+        ci[j] = coef[2 * j + 1];
+    }
+    const double inv = coef[2 * NP];
+    for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < N;
+         f += (int64_t)gridDim.x * blockDim.x) {
+        const C<R> x = v[f];
+        C<R> pj[NP > 0 ? NP : 1];
+#pragma unroll
+        for (int j = 0; j < NP; ++j) pj[j] = base[j * stride + f];
+        double vr = x.x, vi = x.y;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            vr -= cr[j] * (double)pj[j].x - ci[j] * (double)pj[j].y;
+            vi -= cr[j] * (double)pj[j].y + ci[j] * (double)pj[j].x;
+        }
+        v[f] = mk<R>((R)(vr * inv), (R)(vi * inv));
+    }
+}
+
 enum RedMode { RM_DOT = 0, RM_DOTX = 1, RM_PMOM = 2, RM_OBS = 3, RM_HCOMB = 4 };
 
 // Per-workgroup partial sums (FP64).  Layout per workgroup: npart doubles.

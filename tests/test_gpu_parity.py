@@ -90,6 +90,45 @@ def test_imag_steps_gram_schmidt(gpu, nr, dr, pot, integ):
     g.close()
 
 
+@pytest.mark.parametrize("prec,nr,pot", [("c128", (48, 24), "H_2D"), ("c128", (16, 16, 16), "H_3D"),
+                                         ("c64", (32, 16, 32), "H_3D")])
+def test_gram_schmidt_split_form(gpu, monkeypatch, prec, nr, pot):
+    """The Gram-Schmidt of large grids (QND_GS_SPLIT=1 forces it here): the
+    dots and the update as contiguous kernels of their own (k_gs_dots,
+    k_gs_update) instead of fused into the first and last pass of a step.
+    Against the oracle (complex128) and against the fused form."""
+    print(f"case imag GS split prec={prec} nr={nr}", flush=True)
+    dr, dt, nst, nsteps = 0.5, 0.05, 4, 10
+    o = O.Oracle(nr, dr, pot, "BM4", dt)
+    states = smooth_state(o, 17, nst)
+    out = {}
+    for split in ("1", "0"):
+        monkeypatch.setenv("QND_GS_SPLIT", split)
+        g = q.Grid(nr, dr, prec, n_slots=nst)
+        g.set_potential(pot)
+        g.set_integrator("BM4", dt)
+        for j in range(nst):
+            g.upload(j, states[j])
+            g.imag_steps(j, nsteps // 2, gram_schmidt=True)
+            g.imag_steps(j, nsteps - nsteps // 2, gram_schmidt=True)
+        out[split] = [g.download(j) for j in range(nst)]
+        g.close()
+    tol = TOL128 if prec == "c128" else TOL64
+    for j in range(nst):
+        assert rel(out["1"][j], out["0"][j]) < tol, (j, rel(out["1"][j], out["0"][j]))
+    if prec == "c128":
+        ref = states.copy()
+        for j in range(nst):
+            for _ in range(nsteps):
+                ref[j] = o.step_imag(ref[j])
+                rc, ref = o.gram_schmidt(ref, j)
+                assert rc == 0
+            assert rel(out["1"][j], ref[j]) < TOL128, (j, rel(out["1"][j], ref[j]))
+    # orthonormal to the FP32 noise floor in complex64
+    M = np.array([[np.vdot(a, b) * dr ** len(nr) for b in out["1"]] for a in out["1"]])
+    assert np.abs(M - np.eye(nst)).max() < (1e-10 if prec == "c128" else 1e-5), np.abs(M - np.eye(nst)).max()
+
+
 def test_gram_schmidt_ten_states(gpu):
     """Ten states: the Gram matrix of up to eight previous states comes from one
     pass over them, of nine from one reduction per pair; both against the oracle."""
