@@ -2466,12 +2466,20 @@ int group_real_steps(qnd_ctx *const *cs, int n, int64_t step0, int64_t nsteps, c
     if (branch) {
         for (int k = 1; k < n; ++k)
             if (cs[k]->stream == c0->stream) return fail(c0, QND_EINVAL, "group members share a stream");
+        struct Events {  // the fork and join events, destroyed on every way out
+            hipEvent_t *fork, *join;
+            int n;
+            ~Events()
+            {
+                if (*fork) (void)hipEventDestroy(*fork);
+                for (int k = 1; k < n; ++k)
+                    if (join[k]) (void)hipEventDestroy(join[k]);
+            }
+        } ev_guard{&ev_fork, ev_join, n};
         HIPCK(c0, hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         for (int k = 1; k < n; ++k) HIPCK(c0, hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming));
         rc = branch_steps(c0, nsteps, (use_graph & 1) != 0,
                           [&](bool first, int64_t k) { return run_branch(first, k); });
-        (void)hipEventDestroy(ev_fork);
-        for (int k = 1; k < n; ++k) (void)hipEventDestroy(ev_join[k]);
         if (rc) return rc;
         HIPCK(c0, hipStreamSynchronize(c0->stream));
         return group_finish(cs, n, nsteps, want_obs, width, obs_out);
