@@ -580,10 +580,10 @@ PRECISION_PHASES = 6
 
 
 def precision_leg(q, psi0, Et, At, Eo, Ao, nt, nphase=PRECISION_PHASES):
-    """The whole pulse (nt steps), untimed, in complex64 and in complex128
-    with the state rounded to complex64 at every store of a pass
-    (QND_ROUND_STATE: FP64 arithmetic, complex64 storage), each against
-    complex128.  complex128 stands for the FP64 oracle here: it matches the
+    """The whole pulse (nt steps), untimed, in complex64, in complex64
+    storage with FP64 pass arithmetic (c64f64), and in complex128 with the
+    state rounded to complex64 at every store of a pass (QND_ROUND_STATE: the
+    arithmetic of c64f64 in complex128 kernels), each against complex128.  complex128 stands for the FP64 oracle here: it matches the
     oracle to ~1e-12 (GPU tests and parity_vs_oracle_512), and a full-pulse
     oracle run takes minutes of CPU.
     The rounding errors of one run are one realization of a random process:
@@ -615,7 +615,8 @@ def precision_leg(q, psi0, Et, At, Eo, Ao, nt, nphase=PRECISION_PHASES):
            "realizations": f"{nphase} global phases of the initial state"}
     recs = {"c128": ref}
     finite = bool(np.isfinite(ref).all())
-    for label, prec, rnd in (("c64", "c64", False), ("c128_state_rounded_to_c64", "c128", True)):
+    for label, prec, rnd in (("c64", "c64", False), ("c64f64", "c64f64", False),
+                             ("c128_state_rounded_to_c64", "c128", True)):
         errs = []
         for k in range(nphase):
             rec = run(prec, rnd, psi0 * np.exp(2j * np.pi * k / nphase))
@@ -787,7 +788,7 @@ def main():
     Et, At, Eo, Ao = F.tables([mode], "BM4", DT, ntime, 2)
 
     res = {}
-    for prec in (["c64"] if args.skip_fp64 else ["c64", "c128"]):
+    for prec in (["c64"] if args.skip_fp64 else ["c64", "c128", "c64f64"]):
         log(f"real time {prec}: warmup {args.warmup}, timed {args.steps} ...")
         fixed = tuple(int(v) for v in args.tuning.split(",")) if args.tuning else None
         obs, wall, gpu_ms, prof, tuned = real_run(q, F, prec, psi0, args.warmup, args.steps, Et, At, Eo, Ao,
@@ -832,6 +833,14 @@ def main():
                        "step_roofline_frac": n_fft * 2 * 16 * v64 / (HBM_PEAK_GBS * 1e9),
                        "tuned_vpt_twc_tws": list(fp64["tuned"][0])}
         out["mixed_over_fp64"] = value / v64
+        xm = res["c64f64"]
+        vx = N * args.steps / xm["wall"]
+        out["c64f64"] = {"value": vx, "ms_per_step": xm["wall"] * 1e3 / args.steps,
+                         "gpu_ms_per_step": xm["gpu_ms"] / args.steps,
+                         "over_fp64": vx / v64, "over_c64": vx / value,
+                         "what": "complex64 state in memory, FP64 arithmetic in the passes (one rounding to "
+                                 "complex64 per value and pass, at its store); phases as for c64",
+                         "tuned_vpt_twc_tws": list(xm["tuned"][0])}
         a64 = mixed["obs"][:, 2]
         a128 = fp64["obs"][:, 2]
         S64, S128 = spectrum(a64, DT), spectrum(a128, DT)
@@ -840,6 +849,11 @@ def main():
             "steps": int(len(a64)),
             "accel_spectrum_rel_err": float(np.linalg.norm(S64 - S128) / np.linalg.norm(S128)),
             "dipole_rel_l2": float(np.linalg.norm(mixed["obs"][:, :2] - fp64["obs"][:, :2]) /
+                                   np.linalg.norm(fp64["obs"][:, :2]))}
+        Sx = spectrum(xm["obs"][:, 2], DT)
+        out["timed_steps_c64f64_vs_c128"] = {
+            "accel_spectrum_rel_err": float(np.linalg.norm(Sx - S128) / np.linalg.norm(S128)),
+            "dipole_rel_l2": float(np.linalg.norm(xm["obs"][:, :2] - fp64["obs"][:, :2]) /
                                    np.linalg.norm(fp64["obs"][:, :2]))}
     # roofline of the dominant kernel class (HIP events around its launches)
     cnt, ms = mixed["prof"]
@@ -886,10 +900,12 @@ def main():
         log(f"precision leg: the whole pulse ({ntime_full} steps) in c64, c128 and c128 with c64 storage ...")
         prec_res, prec_recs = precision_leg(q, psi0, Et, At, Eo, Ao, ntime_full)
         out["precision_full_pulse"] = prec_res
-        c64r, rr_ = prec_res["c64"], prec_res["c128_state_rounded_to_c64"]
+        c64r, rr_, xr_ = prec_res["c64"], prec_res["c128_state_rounded_to_c64"], prec_res["c64f64"]
         log(f"  c64 vs c128: dipole spectrum median {c64r['dipole_spectrum_rel_err_median']:.3e} "
             f"max {c64r['dipole_spectrum_rel_err_max']:.3e}, acceleration spectrum median "
-            f"{c64r['accel_spectrum_rel_err_median']:.3e}; c128 with c64 storage: dipole spectrum median "
+            f"{c64r['accel_spectrum_rel_err_median']:.3e}; c64f64: dipole spectrum median "
+            f"{xr_['dipole_spectrum_rel_err_median']:.3e} max {xr_['dipole_spectrum_rel_err_max']:.3e}; "
+            f"c128 with c64 storage: dipole spectrum median "
             f"{rr_['dipole_spectrum_rel_err_median']:.3e} max {rr_['dipole_spectrum_rel_err_max']:.3e}")
         if args.dump:
             np.savez(args.dump.replace(".npz", "") + "_full_pulse.npz", **prec_recs)

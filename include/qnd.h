@@ -35,9 +35,12 @@ extern "C" {
 typedef struct qnd_ctx qnd_ctx;
 
 /* precision of the propagated state */
-enum { QND_C128 = 0,   /* complex128 state, FP64 everywhere */
-       QND_C64 = 1 };  /* complex64 state, FP32 phases from FP64-reduced
-                          angles, FP64 reductions */
+enum { QND_C128 = 0,     /* complex128 state, FP64 everywhere */
+       QND_C64 = 1,      /* complex64 state, FP32 phases from FP64-reduced
+                            angles, FP64 reductions */
+       QND_C64F64 = 2 }; /* complex64 state in memory, FP64 arithmetic in
+                            the passes (each value rounded once per pass,
+                            at its store); phases as for QND_C64 */
 
 enum { QND_GAUGE_LENGTH = 0, QND_GAUGE_VELOCITY = 1 };
 

@@ -20,10 +20,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("QND_LIBRARY") or os.path.join(HERE, "libqnd.so")
 BIN_DIR = os.path.join(HERE, "bin")
 
-C128, C64 = 0, 1
+C128, C64, C64F64 = 0, 1, 2
 GAUGES = {"length": 0, "velocity": 1}
 OBS = {"none": 0, "Lz": 1, "energy": 2, "dipole_r": 3, "dipole_v": 4, "norm": 5, "dipole_a": 6}
-PRECISIONS = {"c128": C128, "complex128": C128, "fp64": C128, "c64": C64, "complex64": C64, "mixed": C64}
+PRECISIONS = {"c128": C128, "complex128": C128, "fp64": C128, "c64": C64, "complex64": C64, "mixed": C64,
+              "c64f64": C64F64}
 
 EXPORTS = ("qnd_create", "qnd_destroy", "qnd_last_error", "qnd_set_potential", "qnd_set_integrator",
            "qnd_set_mask", "qnd_upload_state", "qnd_download_state", "qnd_imag_steps",

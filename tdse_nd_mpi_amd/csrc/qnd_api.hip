@@ -53,7 +53,9 @@ struct qnd_ctx {
     double dr[QND_MAXD] = {0, 0, 0, 0};
     int64_t N = 0;
     int prec = QND_C128;
-    size_t esz = 16;
+    size_t esz = 16;                      // bytes per stored complex value
+    size_t lds_esz = 16;                  // bytes per complex value in the passes' registers and LDS
+    bool x64 = false;                     // QND_C64F64: complex64 storage, FP64 pass arithmetic
     int n_slots = 0;
     int device = 0;
     size_t lds_max = 65536;
@@ -296,8 +298,13 @@ int pow2floor(int64_t x)
 // threads and `lds` bytes of LDS, from HIP's occupancy calculator (VGPRs,
 // LDS and wavefront slots), cached per shape.  Axis 0 of a grid of two or
 // more dimensions is the potential pass; other strided axes are plain passes.
-template <typename R, int KIND, int DIR> const void *pass_fn(int vpt)
+template <typename R, int KIND, int DIR> const void *pass_fn(int vpt, bool x64 = false)
 {
+    if constexpr (std::is_same<R, float>::value) {
+        if (x64)
+            return vpt == 4 ? (const void *)k_pass_x<KIND, DIR, 4>
+                   : vpt == 12 ? (const void *)k_pass_x<KIND, DIR, 12> : (const void *)k_pass_x<KIND, DIR, 8>;
+    }
     return vpt == 4 ? (const void *)k_pass<R, KIND, DIR, 4>
            : vpt == 12 ? (const void *)k_pass<R, KIND, DIR, 12> : (const void *)k_pass<R, KIND, DIR, 8>;
 }
@@ -308,8 +315,8 @@ template <typename R> int blocks_per_cu(qnd_ctx *c, int kind, int vpt, int threa
     const uint64_t key = ((uint64_t)lds << 24) | ((uint64_t)threads << 8) | ((uint64_t)vpt << 2) | (uint64_t)kind;
     auto it = c->occ.find(key);
     if (it != c->occ.end()) return it->second;
-    const void *f = kind == PK_POT ? pass_fn<R, PK_POT, 1>(vpt)
-                    : kind == PK_KIN ? pass_fn<R, PK_KIN, -1>(vpt) : pass_fn<R, PK_PLAIN, 1>(vpt);
+    const void *f = kind == PK_POT ? pass_fn<R, PK_POT, 1>(vpt, c->x64)
+                    : kind == PK_KIN ? pass_fn<R, PK_KIN, -1>(vpt, c->x64) : pass_fn<R, PK_PLAIN, 1>(vpt, c->x64);
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, threads, lds) != hipSuccess) {
         (void)hipGetLastError();
@@ -350,7 +357,7 @@ Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1, bool tuned = true)
             g.vpt = 4;
     }
     g.T = g.n / g.vpt;
-    const size_t line_bytes = 2 * (size_t)g.n * c->esz;  // two exchange buffers per line
+    const size_t line_bytes = 2 * (size_t)g.n * c->lds_esz;  // two exchange buffers per line
     int twmax_lds = (int)(c->lds_max / line_bytes);
     if (twmax_lds < 1) twmax_lds = 1;
     int tw;
@@ -376,7 +383,7 @@ Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1, bool tuned = true)
     if (tw < 1) tw = 1;
     while (tw > 1 && lpi % tw != 0) tw /= 2;  // a workgroup never straddles two instances
     // one padding element per line spreads the strided tile rows over the banks
-    while (tw > 1 && 2 * (size_t)tw * (g.n + 1) * c->esz > c->lds_max) tw /= 2;
+    while (tw > 1 && 2 * (size_t)tw * (g.n + 1) * c->lds_esz > c->lds_max) tw /= 2;
     if (g.S > 1 && ov == 0) {
         // Strided lines: of tw, tw/2 and tw/4 take the one that keeps the most
         // wavefronts resident on the chip (workgroups per CU from the kernel's
@@ -389,7 +396,7 @@ Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1, bool tuned = true)
         int best = tw;
         for (int cand = tw; cand >= 1 && cand >= tw / 4; cand /= 2) {
             const int thr = g.T * cand;
-            const size_t lds = 2 * (size_t)cand * (g.n + 1) * c->esz;
+            const size_t lds = 2 * (size_t)cand * (g.n + 1) * c->lds_esz;
             const int bpc = c->esz == 8 ? blocks_per_cu<float>(c, kind, g.vpt, thr, lds)
                                         : blocks_per_cu<double>(c, kind, g.vpt, thr, lds);
             if (bpc <= 0) continue;
@@ -411,7 +418,7 @@ Geo geometry(qnd_ctx *c, int axis, int nb = 1, int kind = -1, bool tuned = true)
     g.threads = g.T * tw;
     g.grid = (int)(g.nlines / tw);
     g.ldsStride = g.n + (tw > 1 ? 1 : 0);
-    g.lds = 2 * (size_t)tw * g.ldsStride * c->esz;
+    g.lds = 2 * (size_t)tw * g.ldsStride * c->lds_esz;
     return g;
 }
 
@@ -455,6 +462,17 @@ void launch_vpt(qnd_ctx *c, const Geo &g, const PassArgs<R> &a, hipEvent_t e0, h
     const int extra = (KIND == PK_KIN && (a.flags & PF_FINALIZE)) ? a.B.nb : 0;
     const dim3 grid(g.grid + extra);
     const unsigned fl = a.flags | (c->round32 ? (unsigned)PF_ROUND32 : 0u);
+    if constexpr (std::is_same<R, float>::value) {
+        if (c->x64) {
+            if (e0)
+                hipExtLaunchKernelGGL((k_pass_x<KIND, DIR, VPT>), grid, dim3(g.threads), g.lds, c->stream, e0, e1, 0,
+                                      a.data, a.tw, a.S, geo, fl, (int)g.grid, a);
+            else
+                hipLaunchKernelGGL((k_pass_x<KIND, DIR, VPT>), grid, dim3(g.threads), g.lds, c->stream, a.data, a.tw,
+                                   a.S, geo, fl, (int)g.grid, a);
+            return;
+        }
+    }
     if (e0)
         hipExtLaunchKernelGGL((k_pass<R, KIND, DIR, VPT>), grid, dim3(g.threads), g.lds, c->stream, e0, e1, 0,
                               a.data, a.tw, a.S, geo, fl, (int)g.grid, a);
@@ -2075,7 +2093,7 @@ bool group_geometry(qnd_ctx *const *cs, int n, int axis, int kind, GroupGeo &G)
             const int tw = thr / g.T;
             const int64_t lpi = c->N / g.n;
             if ((tw & (tw - 1)) != 0 || lpi % tw != 0 || (int64_t)tw > g.nlines || (g.S > 1 && (g.S % tw) != 0) ||
-                2 * (size_t)tw * (g.n + (tw > 1 ? 1 : 0)) * c->esz > c->lds_max) {
+                2 * (size_t)tw * (g.n + (tw > 1 ? 1 : 0)) * c->lds_esz > c->lds_max) {
                 ok = false;
                 break;
             }
@@ -2084,7 +2102,7 @@ bool group_geometry(qnd_ctx *const *cs, int n, int axis, int kind, GroupGeo &G)
             g.threads = g.T * tw;
             g.grid = (int)(g.nlines / tw);
             g.ldsStride = g.n + (tw > 1 ? 1 : 0);
-            g.lds = 2 * (size_t)tw * g.ldsStride * c->esz;
+            g.lds = 2 * (size_t)tw * g.ldsStride * c->lds_esz;
         }
         if (ok) {
             G.g = out;
@@ -2129,7 +2147,7 @@ void record_group(const GroupGeo &G, const std::vector<PassArgs<R>> &as, std::ve
         for (int k = 0; k < n; ++k) {
             const Geo &g = G.g[k];
             BLaunch<R> b;
-            b.fn = pass_fn<R, KIND, DIR>(g.vpt);
+            b.fn = pass_fn<R, KIND, DIR>(g.vpt);  // (x64 members are refused by the group call)
             const int extra = (KIND == PK_KIN && (as[k].flags & PF_FINALIZE)) ? as[k].B.nb : 0;
             b.grid = g.grid + extra;
             b.threads = g.threads;
@@ -2755,7 +2773,7 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     if (!out) return fail(nullptr, QND_EINVAL, "null output pointer");
     *out = nullptr;
     if (dim < 1 || dim > QND_MAXD) return fail(nullptr, QND_EINVAL, "dimension %d not in [1,%d]", dim, QND_MAXD);
-    if (precision != QND_C128 && precision != QND_C64)
+    if (precision != QND_C128 && precision != QND_C64 && precision != QND_C64F64)
         return fail(nullptr, QND_EINVAL, "unknown precision %d", precision);
     if (n_slots < 1) return fail(nullptr, QND_EINVAL, "n_slots must be >= 1");
     for (int j = 0; j < dim; ++j) {
@@ -2769,8 +2787,12 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     }
     qnd_ctx *c = new qnd_ctx();
     c->dim = dim;
-    c->prec = precision;
-    c->esz = precision == QND_C64 ? 8 : 16;
+    // complex64 storage with FP64 pass arithmetic is a complex64 context
+    // (tables, buffers, host paths) whose passes are the k_pass_x kernels
+    c->x64 = precision == QND_C64F64;
+    c->prec = c->x64 ? QND_C64 : precision;
+    c->esz = c->prec == QND_C64 ? 8 : 16;
+    c->lds_esz = c->x64 ? 16 : c->esz;
     c->n_slots = n_slots;
     c->N = 1;
     for (int j = 0; j < dim; ++j) {
@@ -2795,7 +2817,7 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     }
     for (int j = 0; j < dim; ++j) {
         const int64_t tpl = (nr[j] % 3 == 0) ? nr[j] / 12 : nr[j] / 8;  // threads per line
-        if (2 * (size_t)nr[j] * c->esz > c->lds_max || tpl > QND_PASS_THREADS) {
+        if (2 * (size_t)nr[j] * c->lds_esz > c->lds_max || tpl > QND_PASS_THREADS) {
             delete c;
             return fail(nullptr, QND_EUNSUPPORTED, "NR[%d] = %lld exceeds the single-pass line limit", j,
                         (long long)nr[j]);
@@ -3448,6 +3470,8 @@ int qnd_group_real_steps(qnd_ctx *const *ctxs, int nctx, int gauge, int64_t step
             if (ctxs[j] == c) return fail(c0, QND_EINVAL, "context %d appears twice in the group", k);
         if (c->prec != c0->prec || c->dim != 2)
             return fail(c0, QND_EUNSUPPORTED, "group members must be 2-D grids of one precision");
+        if (c->x64)
+            return fail(c0, QND_EUNSUPPORTED, "group launches have no complex64-storage FP64-arithmetic passes");
         if (c->device != c0->device) return fail(c0, QND_EINVAL, "group members must share one device");
         if (c->s != c0->s || memcmp(c->as_scaled, c0->as_scaled, sizeof c->as_scaled) != 0)
             return fail(c0, QND_EINVAL, "group members must share the integrator and DT");

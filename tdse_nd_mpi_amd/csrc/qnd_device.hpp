@@ -185,6 +185,20 @@ template <int DIR> __device__ __forceinline__ C<float> twmul_f(C<float> x, float
 template <> __device__ __forceinline__ C<float> twmul<float, -1>(C<float> x, float4 w) { return twmul_f<-1>(x, w); }
 template <> __device__ __forceinline__ C<float> twmul<float, +1>(C<float> x, float4 w) { return twmul_f<+1>(x, w); }
 
+// A twiddle or factor table entry widened to the arithmetic type CR: the
+// passes that keep complex64 states in FP64 registers (the complex64 storage,
+// FP64 arithmetic form) read the complex64 tables, whose FP32 hi + lo pairs
+// hold the FP64 values to about 1e-15.
+template <typename CR, typename TE> struct TwCvt {
+    static __device__ __forceinline__ typename TwT<CR>::T f(const TE &e) { return e; }
+};
+template <> struct TwCvt<double, float4> {
+    static __device__ __forceinline__ double2 f(const float4 &e)
+    {
+        return make_double2((double)e.x + (double)e.z, (double)e.y + (double)e.w);
+    }
+};
+
 // Phase factor e^{arg}(cos ph - i sin ph) applied to a value: complex128
 // multiplies by the FP64 factor; complex64 carries the FP64 factor as an
 // FP32 hi+lo pair (compensated product), which keeps the propagation
@@ -298,9 +312,8 @@ template <typename R> struct Tw4 {
     typename TwT<R>::T w4[QND_TW4_STAGES][3];  // stage s: W^{r k n/(4 Ns)}, k = t mod Ns, r = 1..3
     typename TwT<R>::T w2[2];                   // last radix-2 stage (odd log2 n): W^{t + b T}
 };
-template <typename R>
-__device__ __forceinline__ void tw4_load(Tw4<R> &P, const typename TwT<R>::T *__restrict__ tw, int log2n, int T,
-                                         int t)
+template <typename R, typename TE>
+__device__ __forceinline__ void tw4_load(Tw4<R> &P, const TE *__restrict__ tw, int log2n, int T, int t)
 {
     const int ns4 = log2n >> 1;
 #pragma unroll
@@ -309,12 +322,12 @@ __device__ __forceinline__ void tw4_load(Tw4<R> &P, const typename TwT<R>::T *__
             const int k = t & ((1 << (2 * s)) - 1);
             const int stride = 1 << (log2n - 2 - 2 * s);
 #pragma unroll
-            for (int r = 1; r < 4; ++r) P.w4[s - 1][r - 1] = tw[r * k * stride];
+            for (int r = 1; r < 4; ++r) P.w4[s - 1][r - 1] = TwCvt<R, TE>::f(tw[r * k * stride]);
         }
     }
     if (log2n & 1) {
-        P.w2[0] = tw[t];
-        P.w2[1] = tw[t + T];
+        P.w2[0] = TwCvt<R, TE>::f(tw[t]);
+        P.w2[1] = TwCvt<R, TE>::f(tw[t + T]);
     }
 }
 
@@ -341,10 +354,10 @@ template <int VPT, int L> struct Sched {
 // through the two LDS buffers: stage S of a transform that starts at the
 // kernel's exchange count wc0 writes buffer (wc0 + S) mod 2.  Twiddles come
 // from the registers of Tw4 (VPT 4) or from the table tw.
-template <typename R, int DIR, int VPT, int L, bool WL, int S>
+template <typename R, int DIR, int VPT, int L, bool WL, int S, typename TE>
 struct FftStages {
     static __device__ __forceinline__ void run(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int wc0, int t,
-                                               const typename TwT<R>::T *__restrict__ tw, const Tw4<R> &P)
+                                               const TE *__restrict__ tw, const Tw4<R> &P)
     {
         using Sc = Sched<VPT, L>;
         if constexpr (S < Sc::NST) {
@@ -361,28 +374,27 @@ struct FftStages {
                 stage_core<R, RAD, DIR, VPT, IN, OUT>(x, lin, lout, n, T, t, Ns, twf);
             } else {
                 constexpr int stride = n / (Ns * RAD);
-                auto twf = [&](int b, int r) { return tw[r * ((t + b * T) & (Ns - 1)) * stride]; };
+                auto twf = [&](int b, int r) { return TwCvt<R, TE>::f(tw[r * ((t + b * T) & (Ns - 1)) * stride]); };
                 stage_core<R, RAD, DIR, VPT, IN, OUT>(x, lin, lout, n, T, t, Ns, twf);
             }
             if constexpr (OUT) {
                 if constexpr (WL) wave_fence();
                 else lds_barrier();
             }
-            FftStages<R, DIR, VPT, L, WL, S + 1>::run(x, bufA, bufB, wc0, t, tw, P);
+            FftStages<R, DIR, VPT, L, WL, S + 1, TE>::run(x, bufA, bufB, wc0, t, tw, P);
         }
     }
 };
-template <typename R, int DIR, int VPT, int L>
+template <typename R, int DIR, int VPT, int L, typename TE>
 __device__ __forceinline__ void fft_fixed(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int &wc, int t,
-                                          const typename TwT<R>::T *__restrict__ tw, const Tw4<R> &P,
-                                          bool wave_line)
+                                          const TE *__restrict__ tw, const Tw4<R> &P, bool wave_line)
 {
     // wavefront-local exchanges exist only for 64-thread lines
     if constexpr (Sched<VPT, L>::N / VPT == 64) {
-        if (wave_line) FftStages<R, DIR, VPT, L, true, 0>::run(x, bufA, bufB, wc, t, tw, P);
-        else FftStages<R, DIR, VPT, L, false, 0>::run(x, bufA, bufB, wc, t, tw, P);
+        if (wave_line) FftStages<R, DIR, VPT, L, true, 0, TE>::run(x, bufA, bufB, wc, t, tw, P);
+        else FftStages<R, DIR, VPT, L, false, 0, TE>::run(x, bufA, bufB, wc, t, tw, P);
     } else {
-        FftStages<R, DIR, VPT, L, false, 0>::run(x, bufA, bufB, wc, t, tw, P);
+        FftStages<R, DIR, VPT, L, false, 0, TE>::run(x, bufA, bufB, wc, t, tw, P);
     }
     wc += Sched<VPT, L>::NST - 1;  // exchanges of the transform
 }
@@ -390,32 +402,32 @@ __device__ __forceinline__ void fft_fixed(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB
 // Full transform of the line whose two LDS buffers are bufA / bufB: one
 // uniform switch on the line length picks its fixed schedule.  log2n is
 // log2 n (VPT 4, 8; n <= 2048 resp. 4096) or log2(n/3) (VPT 12, n <= 3072).
-// Every thread of the block calls it (barriers inside).
-template <typename R, int DIR, int VPT>
+// Every thread of the block calls it (barriers inside).  TE: the twiddle
+// table's entry type (a complex64 table for FP64 arithmetic is widened).
+template <typename R, int DIR, int VPT, typename TE>
 __device__ __forceinline__ void transform(C<R> (&x)[VPT], C<R> *bufA, C<R> *bufB, int &wc, int n, int log2n, int T,
-                                          int t, const typename TwT<R>::T *__restrict__ tw, const Tw4<R> &P,
-                                          bool wave_line)
+                                          int t, const TE *__restrict__ tw, const Tw4<R> &P, bool wave_line)
 {
 #define QND_FFT_CASE(Lc)                                                                                            \
-    case Lc: fft_fixed<R, DIR, VPT, Lc>(x, bufA, bufB, wc, t, tw, P, wave_line); break;
+    case Lc: fft_fixed<R, DIR, VPT, Lc, TE>(x, bufA, bufB, wc, t, tw, P, wave_line); break;
     if constexpr (VPT == 4) {
         switch (log2n) {
             QND_FFT_CASE(3) QND_FFT_CASE(4) QND_FFT_CASE(5) QND_FFT_CASE(6) QND_FFT_CASE(7) QND_FFT_CASE(8)
-            QND_FFT_CASE(9) QND_FFT_CASE(10) default: fft_fixed<R, DIR, VPT, 11>(x, bufA, bufB, wc, t, tw, P,
-                                                                                  wave_line);
+            QND_FFT_CASE(9) QND_FFT_CASE(10) default: fft_fixed<R, DIR, VPT, 11, TE>(x, bufA, bufB, wc, t, tw, P,
+                                                                                      wave_line);
         }
     } else if constexpr (VPT == 8) {
         switch (log2n) {
             QND_FFT_CASE(3) QND_FFT_CASE(4) QND_FFT_CASE(5) QND_FFT_CASE(6) QND_FFT_CASE(7) QND_FFT_CASE(8)
-            QND_FFT_CASE(9) QND_FFT_CASE(10) QND_FFT_CASE(11) default: fft_fixed<R, DIR, VPT, 12>(x, bufA, bufB, wc,
-                                                                                                   t, tw, P,
-                                                                                                   wave_line);
+            QND_FFT_CASE(9) QND_FFT_CASE(10) QND_FFT_CASE(11) default: fft_fixed<R, DIR, VPT, 12, TE>(x, bufA, bufB,
+                                                                                                       wc, t, tw, P,
+                                                                                                       wave_line);
         }
     } else {
         switch (log2n) {
             QND_FFT_CASE(2) QND_FFT_CASE(3) QND_FFT_CASE(4) QND_FFT_CASE(5) QND_FFT_CASE(6) QND_FFT_CASE(7)
-            QND_FFT_CASE(8) QND_FFT_CASE(9) default: fft_fixed<R, DIR, VPT, 10>(x, bufA, bufB, wc, t, tw, P,
-                                                                                 wave_line);
+            QND_FFT_CASE(8) QND_FFT_CASE(9) default: fft_fixed<R, DIR, VPT, 10, TE>(x, bufA, bufB, wc, t, tw, P,
+                                                                                     wave_line);
         }
     }
 #undef QND_FFT_CASE
@@ -704,6 +716,50 @@ template <> __device__ __forceinline__ Fac<float> tab_fac<float>(const float4 &v
     f.lo = mk<float>(rot ? -yl : xl, rot ? xl : yl);
     return f;
 }
+// Factors of the complex64 storage, FP64 arithmetic form: formed as for
+// complex64 (FP64 angle reduction, FP32 polynomials, modulus-corrected hi +
+// lo; or a complex64 table entry) and applied as one FP64 product.
+template <typename CR> struct FacCvt;
+template <> struct FacCvt<float> {
+    static __device__ __forceinline__ Fac<float> f(const Fac<float> &a) { return a; }
+};
+template <> struct FacCvt<double> {
+    static __device__ __forceinline__ Fac<double> f(const Fac<float> &a)
+    {
+        // (-i)^q (hi + lo): q = 0 (u, v), 1 (v, -u), 2 (-u, -v), 3 (-v, u)
+        const double hx = (double)a.hi.x + (double)a.lo.x, hy = (double)a.hi.y + (double)a.lo.y;
+        const bool odd = (a.q & 1) != 0;
+        const double u = odd ? hy : hx, v = odd ? hx : hy;
+        Fac<double> r;
+        r.v = mk<double>((a.q == 2 || a.q == 3) ? -u : u, (a.q == 1 || a.q == 2) ? -v : v);
+        return r;
+    }
+};
+// the phase factor e^{arg}(cos ph - i sin ph) of a pass whose state is stored
+// as C<R> and held as C<CR>
+template <typename R, typename CR> __device__ __forceinline__ Fac<CR> phase_fac(double arg, double ph)
+{
+    if constexpr (std::is_same<R, CR>::value) return phase_factor<R>(arg, ph);
+    else return FacCvt<CR>::f(phase_factor<R>(arg, ph));
+}
+// a table entry TE as a factor of arithmetic type CR
+template <typename CR, typename TE> __device__ __forceinline__ Fac<CR> tab_fac_x(const TE &v, bool conj)
+{
+    if constexpr (std::is_same<TE, typename TwT<CR>::T>::value) {
+        return tab_fac<CR>(v, conj);
+    } else {
+        const double x = (double)v.x + (double)v.z, y = (double)v.y + (double)v.w;
+        Fac<double> f;
+        f.v = mk<double>(x, conj ? -y : y);
+        return f;
+    }
+}
+// a stored value widened to the registers' type, and narrowed back for the store
+template <typename CR, typename V> __device__ __forceinline__ C<CR> cvtc(const V &v)
+{
+    return mk<CR>((CR)v.x, (CR)v.y);
+}
+
 // Store the unit factor (c, s) into a table entry (FP32 hi + lo, or FP64).
 __device__ __forceinline__ void tab_store(float4 *t, double c, double s)
 {
@@ -1108,10 +1164,13 @@ __host__ __device__ inline unsigned pass_geo(int log2n, int log2T, int log2TW, i
 // (and, for a finalizing kinetic pass, of the finalize workgroups after
 // them).  TOUCH: touch the argument block's lines at entry (the kernel's
 // PassArgs is its only argument block).
-template <typename R, int KIND, int DIR1, int VPT, bool TOUCH>
+template <typename R, int KIND, int DIR1, int VPT, bool TOUCH, bool X64 = false>
 __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *tw, int64_t S, unsigned geo,
                                           unsigned fl_arg, int nblk, const PassArgs<R> &a, int bid)
 {
+    // X64: the state is stored as C<R> (complex64) and held in FP64 registers
+    // (loads widen, the store rounds once); every pass operator runs in CR
+    using CR = typename std::conditional<X64, double, R>::type;
     const unsigned fl = KIND == PK_POTL   ? (unsigned)(PF_V1 | PF_SECOND)
                         : (KIND == PK_POTI || KIND == PK_POTV) ? (unsigned)(PF_V1 | PF_V2 | PF_SECOND)
                         : KIND == PK_P1DI ? (unsigned)(PF_V1 | PF_V2 | PF_SECOND | PF_KPOST)
@@ -1134,7 +1193,7 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
     const bool r32 = std::is_same<R, double>::value && (fl_arg & PF_ROUND32) != 0;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ double red[16 * QND_NOBS];
-    C<R> *lds = reinterpret_cast<C<R> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
+    C<CR> *lds = reinterpret_cast<C<CR> *>(smem_raw);  // [2 buffers][TW lines][ldsStride]
     constexpr bool POTK = (KIND == PK_POT || KIND == PK_POT1D || KIND == PK_POTL || KIND == PK_POTI ||
                            KIND == PK_POTX || KIND == PK_POTO || KIND == PK_POTS || KIND == PK_POTG ||
                            KIND == PK_POTV || KIND == PK_P1DI || KIND == PK_POTJ || KIND == PK_POTB);
@@ -1176,20 +1235,20 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
     if (contig) { lbase = L * n; estride = 1; }
     else if (pow2) { lbase = (((L >> log2S) << log2n) << log2S) + (L & (S - 1)); estride = S; }
     else { const int64_t o = L / S; lbase = o * n * S + (L - o * S); estride = S; }
-    Tw4<R> tw4;  // four values per thread: all twiddles requested before the state
-    if constexpr (VPT == 4) tw4_load<R>(tw4, tw, log2n, T, t);
+    Tw4<CR> tw4;  // four values per thread: all twiddles requested before the state
+    if constexpr (VPT == 4) tw4_load<CR>(tw4, tw, log2n, T, t);
 
     // ---- load x[m] = element t + m T (with the Gram-Schmidt update prologue).
     // Twiddles are read from the table in memory (L1/L2 hits); staging it in
     // LDS measured slower. ----
-    C<R> x[VPT];
+    C<CR> x[VPT];
 #pragma unroll
-    for (int m = 0; m < VPT; ++m) x[m] = data[lbase + (int64_t)(t + m * T) * estride];
+    for (int m = 0; m < VPT; ++m) x[m] = cvtc<CR>(data[lbase + (int64_t)(t + m * T) * estride]);
     // the argument block, now that the first loads are out
     if constexpr (TOUCH) kernarg_touch<sizeof(PassArgs<R>) + 40>();
     const int ldsS = a.ldsStride;
-    C<R> *bufA = lds + l * ldsS;
-    C<R> *bufB = lds + (TW + l) * ldsS;
+    C<CR> *bufA = lds + l * ldsS;
+    C<CR> *bufB = lds + (TW + l) * ldsS;
     int wc = 0;
 
     if (POTK && (fl & PF_GS_UPDATE)) {
@@ -1200,7 +1259,7 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
         // the previous states' values are requested QND_GS_G states at a
         // time, so that one memory latency serves several states (the
         // subtraction order stays j = 0, 1, ...)
-        constexpr int G = QND_GS_G(R);
+        constexpr int G = QND_GS_G(CR);
         for (int j0 = 0; j0 < a.nprev; j0 += G) {
             C<R> p[G][VPT];
 #pragma unroll
@@ -1224,7 +1283,7 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
             }
         }
 #pragma unroll
-        for (int m = 0; m < VPT; ++m) x[m] = mk<R>((R)(vr[m] * gsinv), (R)(vi[m] * gsinv));
+        for (int m = 0; m < VPT; ++m) x[m] = mk<CR>((CR)(vr[m] * gsinv), (CR)(vi[m] * gsinv));
     }
 
     const int ax = a.axis;
@@ -1233,11 +1292,11 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
     // need no workgroup barrier
     const bool wave_line = contig && T == 64 && a.wave_lines;
     if (KIND == PK_PLAIN) {
-        transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+        transform<CR, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
         if (a.scale1 != 1.0) {
-            const R s1 = (R)a.scale1;
+            const CR s1 = (CR)a.scale1;
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], s1);
+            for (int m = 0; m < VPT; ++m) x[m] = scl<CR>(x[m], s1);
         }
     } else {
         int idx[QND_MAXD];
@@ -1333,8 +1392,8 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
             for (int m = 0; m < VPT; ++m) {
                 const double p = coord_p(P, ax, t + m * T);
                 const double K = .5 * p * p;
-                if (kmode == KM_IMAG) x[m] = scl<R>(x[m], real_exp<R>(-a.ck_pre * K));
-                else x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_pre * K));
+                if (kmode == KM_IMAG) x[m] = scl<CR>(x[m], real_exp<CR>(-a.ck_pre * K));
+                else x[m] = fac_mul<CR>(x[m], phase_fac<R, CR>(0.0, cs * a.ck_pre * K));
             }
         }
 
@@ -1343,40 +1402,40 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
             // axis's part of K(a_0) conjugated, this axis's mask, then the
             // boundary's source term (axis-0 Fourier representation)
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], (R)a.scale2);
-            transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            for (int m = 0; m < VPT; ++m) x[m] = scl<CR>(x[m], (CR)a.scale2);
+            transform<CR, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(atab_at(m), cs < 0.0));
-            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<CR>(x[m], tab_fac_x<CR>(atab_at(m), cs < 0.0));
+            transform<CR, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
             const bool mk = (fl & PF_MASK) && P.mask_on;
 #pragma unroll
             for (int m = 0; m < VPT; ++m) {
-                const C<R> src = a.save[lbase + (int64_t)(t + m * T) * estride];
-                const C<R> v = mk ? scl<R>(x[m], (R)kmask_at(m)) : x[m];
-                x[m] = add<R>(v, src);
+                const C<CR> src = cvtc<CR>(a.save[lbase + (int64_t)(t + m * T) * estride]);
+                const C<CR> v = mk ? scl<CR>(x[m], (CR)kmask_at(m)) : x[m];
+                x[m] = add<CR>(v, src);
             }
         }
 
         // ---- first transform ----
-        transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+        transform<CR, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
 
         // adjoint source of a merged boundary: the record's state y from its
         // saved axis-0 Fourier representation (inverse transform, scale, mask)
-        C<R> ysv[VPT];
+        C<CR> ysv[VPT];
         if (KIND == PK_POT && (fl & PF_SRCM)) {
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) ysv[m] = a.save[lbase + (int64_t)(t + m * T) * estride];
-            transform<R, DIR1, VPT>(ysv, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            for (int m = 0; m < VPT; ++m) ysv[m] = cvtc<CR>(a.save[lbase + (int64_t)(t + m * T) * estride]);
+            transform<CR, DIR1, VPT>(ysv, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
             const bool mky = (fl & PF_MASK) && P.mask_on;
 #pragma unroll
             for (int m = 0; m < VPT; ++m) {
                 const double f = mky ? msk_at(m) * mko * a.scale1 : a.scale1;
-                ysv[m] = mk<R>((R)((double)ysv[m].x * f), (R)((double)ysv[m].y * f));
+                ysv[m] = mk<CR>((CR)((double)ysv[m].x * f), (CR)((double)ysv[m].y * f));
             }
         }
         if (KIND == PK_POT && (fl & PF_ZERO)) {
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) x[m] = mk<R>((R)0, (R)0);
+            for (int m = 0; m < VPT; ++m) x[m] = mk<CR>((CR)0, (CR)0);
         }
 
         if ((KIND == PK_POT || KIND == PK_POTX) && (fl & PF_PREKX)) {
@@ -1387,13 +1446,13 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
             const bool mk0 = (fl & PF_MASK) && P.mask_on;
 #pragma unroll
             for (int m = 0; m < VPT; ++m) {
-                const R f = mk0 ? (R)(msk_at(m) * a.scale1) : (R)a.scale1;
-                x[m] = scl<R>(x[m], f);
+                const CR f = mk0 ? (CR)(msk_at(m) * a.scale1) : (CR)a.scale1;
+                x[m] = scl<CR>(x[m], f);
             }
-            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            transform<CR, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(atab_at(m), cs < 0.0));
-            transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<CR>(x[m], tab_fac_x<CR>(atab_at(m), cs < 0.0));
+            transform<CR, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
         }
 
         if (KINK) {
@@ -1414,30 +1473,30 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
             // one loop per mode (mode tests outside the loop over the points)
             if (KINK && VPT == 4 && (fl & PF_KTAB)) {
 #pragma unroll
-                for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(ktv[m], cs < 0.0));
+                for (int m = 0; m < VPT; ++m) x[m] = fac_mul<CR>(x[m], tab_fac_x<CR>(ktv[m], cs < 0.0));
             } else if ((fl & PF_KAX) && kmode == KM_IMAG) {
                 // e^{-c K} = prod_j e^{-c K_j}: this axis's factor per element,
                 // times the folded axes' factors of the line
                 const typename TwT<R>::T *kx = a.ktab;
-                R fo = (R)1;
+                CR fo = (CR)1;
 #pragma unroll
                 for (int j = 0; j < QND_MAXD; ++j)
-                    if (j < P.dim && j != ax && ((a.kin_axes >> j) & 1u)) fo *= (R)kx[P.off[j] + idx[j]].x;
+                    if (j < P.dim && j != ax && ((a.kin_axes >> j) & 1u)) fo *= (CR)kx[P.off[j] + idx[j]].x;
 #pragma unroll
-                for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], (R)kx[P.off[ax] + t + m * T].x * fo);
+                for (int m = 0; m < VPT; ++m) x[m] = scl<CR>(x[m], (CR)kx[P.off[ax] + t + m * T].x * fo);
             } else if ((fl & PF_KAX) && kmode == KM_PHASE) {
                 // e^{-i c K} = prod_j e^{-i c K_j}, each factor a unit table
                 // entry applied by the compensated product (one rounding of
                 // the state per factor)
                 const typename TwT<R>::T *kx = a.ktab;
 #pragma unroll
-                for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(kx[P.off[ax] + t + m * T], cs < 0.0));
+                for (int m = 0; m < VPT; ++m) x[m] = fac_mul<CR>(x[m], tab_fac_x<CR>(kx[P.off[ax] + t + m * T], cs < 0.0));
 #pragma unroll
                 for (int j = 0; j < QND_MAXD; ++j) {
                     if (j < P.dim && j != ax && ((a.kin_axes >> j) & 1u)) {
-                        const Fac<R> fo = tab_fac<R>(kx[P.off[j] + idx[j]], cs < 0.0);
+                        const Fac<CR> fo = tab_fac_x<CR>(kx[P.off[j] + idx[j]], cs < 0.0);
 #pragma unroll
-                        for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], fo);
+                        for (int m = 0; m < VPT; ++m) x[m] = fac_mul<CR>(x[m], fo);
                     }
                 }
             } else if (kmode == KM_IMAG) {
@@ -1445,21 +1504,21 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
                 for (int m = 0; m < VPT; ++m) {
                     const double p = coord_p(P, ax, t + m * T);
                     const double K = Ko + .5 * p * p;
-                    x[m] = scl<R>(x[m], real_exp<R>(-a.ck_mid * K));
+                    x[m] = scl<CR>(x[m], real_exp<CR>(-a.ck_mid * K));
                 }
             } else if (kmode == KM_VELOCITY) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) {
                     const double p = coord_p(P, ax, t + m * T);
                     const double K = Ko + .5 * p * p;
-                    x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_mid * (K + (Wk + (p + .5 * Aax) * Aax))));
+                    x[m] = fac_mul<CR>(x[m], phase_fac<R, CR>(0.0, cs * a.ck_mid * (K + (Wk + (p + .5 * Aax) * Aax))));
                 }
             } else {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) {
                     const double p = coord_p(P, ax, t + m * T);
                     const double K = Ko + .5 * p * p;
-                    x[m] = fac_mul<R>(x[m], phase_factor<R>(0.0, cs * a.ck_mid * K));
+                    x[m] = fac_mul<CR>(x[m], phase_fac<R, CR>(0.0, cs * a.ck_mid * K));
                 }
             }
         } else {
@@ -1480,7 +1539,7 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
             // potential, its phase angle and this thread's observable sums are
             // FP64 for complex128 and FP32 for complex64 (a thread sums its VPT
             // points in obr, which then enter the FP64 workgroup reduction).
-            constexpr bool F64 = std::is_same<R, double>::value;
+            constexpr bool F64 = std::is_same<CR, double>::value;
             // precision of the potential: the state's, or FP64 in complex64
             // too (QND_POT_F64)
             using PR = typename std::conditional<F64 || QND_POT_F64, double, R>::type;
@@ -1488,23 +1547,24 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
 #pragma unroll
             for (int q = 0; q < QND_NOBS; ++q) {
                 obs[q] = 0.0;
-                obr[q] = (R)0;
+                obr[q] = (CR)0;
             }
-            auto add_obs = [&](int q, R val) {
+            auto add_obs = [&](int q, CR val) {
                 if constexpr (F64) obs[q] += val;
                 else obr[q] += val;
             };
-            const R cv1 = (R)a.cv1, cv2 = (R)a.cv2, csr = (R)cs, fva = (R)fv[ax], Wvr = (R)Wv;
+            const CR cv1 = (CR)a.cv1, cv2 = (CR)a.cv2, csr = (CR)cs, fva = (CR)fv[ax], Wvr = (CR)Wv;
             // potential factor of damping `damp` and angle c * arg: the angle
             // in FP64 (argd, cd) or, for the FP32-angle build of complex64, in
             // FP32 (argr, cr)
-            auto pf = [&](R damp, R cr, double cd, R argr, double argd) -> Fac<R> {
-                if constexpr (F64 || QND_POT_ANGLE_FP64) return pot_factor(damp, cs * cd * argd);
+            auto pf = [&](CR damp, CR cr, double cd, CR argr, double argd) -> Fac<CR> {
+                if constexpr (X64) return FacCvt<CR>::f(pot_factor((float)damp, cs * cd * argd));
+                else if constexpr (F64 || QND_POT_ANGLE_FP64) return pot_factor(damp, cs * cd * argd);
                 else return pot_factor(damp, csr * cr * argr);
             };
-            R xor_[QND_MAXD];
+            CR xor_[QND_MAXD];
 #pragma unroll
-            for (int j = 0; j < QND_MAXD; ++j) xor_[j] = (R)xo[j];
+            for (int j = 0; j < QND_MAXD; ++j) xor_[j] = (CR)xo[j];
             const bool need_v = (fl & (PF_V1 | PF_V2 | PF_OBS | PF_SRC)) != 0;
             // Groups of four points: each operator is one unrolled loop over the
             // group with its mode tests outside, so the points are independent
@@ -1515,20 +1575,20 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
             constexpr int G = VPT == 4 ? 4 : 2;  // points per group (registers vs interleaving)
 #pragma unroll
             for (int m0 = 0; m0 < VPT; m0 += G) {
-                C<R> v[G];
+                C<CR> v[G];
                 double xc[G], rr[G];
-                R xr[G], Vc[G];
+                CR xr[G], Vc[G];
                 PR V[G], dVdu[G];
 #pragma unroll
                 for (int i = 0; i < G; ++i) {
                     const int m = m0 + i;
-                    v[i] = scl<R>(x[m], (R)a.scale1);
+                    v[i] = scl<CR>(x[m], (CR)a.scale1);
                     xc[i] = coord_x(P, ax, t + m * T);
-                    xr[i] = (R)xc[i];
+                    xr[i] = (CR)xc[i];
                     rr[i] = xc[i] * xc[i] + rro;
                     V[i] = (PR)0;
                     dVdu[i] = (PR)0;
-                    Vc[i] = P.cap_on ? (R)(cap_at(m) + Vco) : (R)0;
+                    Vc[i] = P.cap_on ? (CR)(cap_at(m) + Vco) : (CR)0;
                 }
                 // V from the table where only V is needed; V and dV/du from
                 // the closed forms where the observables or the adjoint need
@@ -1545,8 +1605,8 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
                     for (int i = 0; i < G; ++i) {
                         const int m = m0 + i;
                         // adjoint: u = costate here; phi / y = forward state saved here
-                        const C<R> f = (KIND == PK_POT && (fl & PF_SRCM)) ? ysv[m]
-                                                                           : a.save[lbase + (int64_t)(t + m * T) * estride];
+                        const C<CR> f = (KIND == PK_POT && (fl & PF_SRCM)) ? ysv[m]
+                                                                           : cvtc<CR>(a.save[lbase + (int64_t)(t + m * T) * estride]);
                         double dV_da, dVdu_da;
                         softcore_derivs(P, rr[i], dV_da, dVdu_da);
                         if (fl & PF_GRAD) {
@@ -1566,8 +1626,8 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
                                     dwsum += srow[j] * (-2.0 * xj * dVdu_da);
                                 }
                             }
-                            v[i] = mk<R>((R)((double)v[i].x + wsum * (double)f.x),
-                                         (R)((double)v[i].y + wsum * (double)f.y));
+                            v[i] = mk<CR>((CR)((double)v[i].x + wsum * (double)f.x),
+                                         (CR)((double)v[i].y + wsum * (double)f.y));
                             obs[0] += dwsum * ((double)f.x * (double)f.x + (double)f.y * (double)f.y);
                         }
                     }
@@ -1576,30 +1636,30 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
                     if (vmode == VM_IMAG) {
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
-                            v[i] = scl<R>(v[i], real_exp<R>(-a.cv1 * V[i]));
+                            v[i] = scl<CR>(v[i], real_exp<CR>(-a.cv1 * V[i]));
                         }
                     } else if (vmode == VM_LENGTH) {
                         // without a CAP the damping argument is the constant 0 (no exponential)
                         if (P.cap_on) {
 #pragma unroll
                             for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i], pf(-cv1 * Vc[i], cv1, a.cv1, V[i] + (xr[i] * fva + Wvr),
+                                v[i] = fac_mul<CR>(v[i], pf(-cv1 * Vc[i], cv1, a.cv1, V[i] + (xr[i] * fva + Wvr),
                                                            (double)V[i] + (xc[i] * fv[ax] + Wv)));
                         } else {
 #pragma unroll
                             for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i], pf((R)0, cv1, a.cv1, V[i] + (xr[i] * fva + Wvr),
+                                v[i] = fac_mul<CR>(v[i], pf((CR)0, cv1, a.cv1, V[i] + (xr[i] * fva + Wvr),
                                                            (double)V[i] + (xc[i] * fv[ax] + Wv)));
                         }
                     } else {
                         if (P.cap_on) {
 #pragma unroll
                             for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i], pf(-cv1 * Vc[i], cv1, a.cv1, V[i], (double)V[i]));
+                                v[i] = fac_mul<CR>(v[i], pf(-cv1 * Vc[i], cv1, a.cv1, V[i], (double)V[i]));
                         } else {
 #pragma unroll
                             for (int i = 0; i < G; ++i)
-                                v[i] = fac_mul<R>(v[i], pf((R)0, cv1, a.cv1, V[i], (double)V[i]));
+                                v[i] = fac_mul<CR>(v[i], pf((CR)0, cv1, a.cv1, V[i], (double)V[i]));
                         }
                     }
                 }
@@ -1607,20 +1667,20 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
                         const int m = m0 + i;
-                        v[i] = scl<R>(v[i], (R)(msk_at(m) * mko));
+                        v[i] = scl<CR>(v[i], (CR)(msk_at(m) * mko));
                     }
                 }
                 if (fl & PF_OBS) {
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
-                        const R d = v[i].x * v[i].x + v[i].y * v[i].y;
+                        const CR d = v[i].x * v[i].x + v[i].y * v[i].y;
                         add_obs(0, d);
 #pragma unroll
                         for (int j = 0; j < QND_MAXD; ++j) {
                             if (j < P.dim) {
-                                const R xj = (j == ax) ? xr[i] : xor_[j];
+                                const CR xj = (j == ax) ? xr[i] : xor_[j];
                                 add_obs(1 + j, d * xj);
-                                add_obs(1 + QND_MAXD + j, d * dVdu[i] * (R)2 * xj);
+                                add_obs(1 + QND_MAXD + j, d * dVdu[i] * (CR)2 * xj);
                             }
                         }
                     }
@@ -1629,22 +1689,22 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
                     if (vmode == VM_IMAG) {
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
-                            v[i] = scl<R>(v[i], real_exp<R>(-a.cv2 * V[i]));
+                            v[i] = scl<CR>(v[i], real_exp<CR>(-a.cv2 * V[i]));
                         }
                     } else if (P.cap_on) {
 #pragma unroll
                         for (int i = 0; i < G; ++i)
-                            v[i] = fac_mul<R>(v[i], pf(-cv2 * Vc[i], cv2, a.cv2, V[i], (double)V[i]));
+                            v[i] = fac_mul<CR>(v[i], pf(-cv2 * Vc[i], cv2, a.cv2, V[i], (double)V[i]));
                     } else {
 #pragma unroll
-                        for (int i = 0; i < G; ++i) v[i] = fac_mul<R>(v[i], pf((R)0, cv2, a.cv2, V[i], (double)V[i]));
+                        for (int i = 0; i < G; ++i) v[i] = fac_mul<CR>(v[i], pf((CR)0, cv2, a.cv2, V[i], (double)V[i]));
                     }
                 }
                 if (fl & PF_SAVE) {
 #pragma unroll
                     for (int i = 0; i < G; ++i) {
                         const int m = m0 + i;
-                        a.save[lbase + (int64_t)(t + m * T) * estride] = v[i];
+                        a.save[lbase + (int64_t)(t + m * T) * estride] = cvtc<R>(v[i]);
                     }
                 }
 #pragma unroll
@@ -1663,37 +1723,37 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
 
         // ---- second transform ----
         if (KINK || (fl & PF_SECOND)) {
-            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
-            const R s2 = (R)a.scale2;
+            transform<CR, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            const CR s2 = (CR)a.scale2;
             if ((KIND == PK_POT1D || KIND == PK_P1DI) && (fl & PF_KPOST)) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m) {
                     const double p = coord_p(P, ax, t + m * T);
                     const double K = .5 * p * p;
-                    const C<R> v = scl<R>(x[m], s2);
-                    if (kmode == KM_IMAG) x[m] = scl<R>(v, real_exp<R>(-a.ck_post * K));
+                    const C<CR> v = scl<CR>(x[m], s2);
+                    if (kmode == KM_IMAG) x[m] = scl<CR>(v, real_exp<CR>(-a.ck_post * K));
                     else if (kmode == KM_VELOCITY)
-                        x[m] = fac_mul<R>(v, phase_factor<R>(0.0, cs * a.ck_post * (K + (p + .5 * fv[ax]) * fv[ax])));
-                    else x[m] = fac_mul<R>(v, phase_factor<R>(0.0, cs * a.ck_post * K));
+                        x[m] = fac_mul<CR>(v, phase_fac<R, CR>(0.0, cs * a.ck_post * (K + (p + .5 * fv[ax]) * fv[ax])));
+                    else x[m] = fac_mul<CR>(v, phase_fac<R, CR>(0.0, cs * a.ck_post * K));
                 }
-            } else if (s2 != (R)1) {
+            } else if (s2 != (CR)1) {
 #pragma unroll
-                for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], s2);
+                for (int m = 0; m < VPT; ++m) x[m] = scl<CR>(x[m], s2);
             }
         }
         if (KIND == PK_POT && (fl & PF_POSTKX)) {
             // adjoint of PF_PREKX: the conjugated axis-0 part of K(a_0), then
             // the axis-0 mask, on the state just returned to axis-0 Fourier space
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<R>(x[m], tab_fac<R>(atab_at(m), cs < 0.0));
-            transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            for (int m = 0; m < VPT; ++m) x[m] = fac_mul<CR>(x[m], tab_fac_x<CR>(atab_at(m), cs < 0.0));
+            transform<CR, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
             const bool mk0 = (fl & PF_MASK) && P.mask_on;
 #pragma unroll
             for (int m = 0; m < VPT; ++m) {
-                const R f = mk0 ? (R)(msk_at(m) * a.scale1) : (R)a.scale1;
-                x[m] = scl<R>(x[m], f);
+                const CR f = mk0 ? (CR)(msk_at(m) * a.scale1) : (CR)a.scale1;
+                x[m] = scl<CR>(x[m], f);
             }
-            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            transform<CR, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
         }
         if (KIND == PK_KINS) {
             // merged step boundary: the state after the step's last kinetic
@@ -1703,18 +1763,18 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
             if (fl & PF_SAVE) {
 #pragma unroll
                 for (int m = 0; m < VPT; ++m)
-                    a.save[lbase + (int64_t)(t + m * T) * estride] = r32 ? round_to_c64<R>(x[m]) : x[m];
+                    a.save[lbase + (int64_t)(t + m * T) * estride] = cvtc<R>(r32 ? round_to_c64<CR>(x[m]) : x[m]);
             }
             if ((fl & PF_MASK) && P.mask_on) {
 #pragma unroll
-                for (int m = 0; m < VPT; ++m) x[m] = scl<R>(x[m], (R)kmask_at(m));
+                for (int m = 0; m < VPT; ++m) x[m] = scl<CR>(x[m], (CR)kmask_at(m));
             }
-            transform<R, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
-            const R s2 = (R)a.scale2;
+            transform<CR, DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+            const CR s2 = (CR)a.scale2;
 #pragma unroll
             for (int m = 0; m < VPT; ++m)
-                x[m] = scl<R>(fac_mul<R>(x[m], tab_fac<R>(atab_at(m), cs < 0.0)), s2);
-            transform<R, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
+                x[m] = scl<CR>(fac_mul<CR>(x[m], tab_fac_x<CR>(atab_at(m), cs < 0.0)), s2);
+            transform<CR, -DIR1, VPT>(x, bufA, bufB, wc, n, log2n, T, t, tw, tw4, wave_line);
         }
     }
 
@@ -1722,10 +1782,15 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
     if (!(POTK && (fl & PF_NOSTORE))) {
         if (r32) {
 #pragma unroll
-            for (int m = 0; m < VPT; ++m) x[m] = round_to_c64<R>(x[m]);
+            for (int m = 0; m < VPT; ++m) x[m] = round_to_c64<CR>(x[m]);
         }
 #pragma unroll
-        for (int m = 0; m < VPT; ++m) data[lbase + (int64_t)(t + m * T) * estride] = x[m];
+        for (int m = 0; m < VPT; ++m) data[lbase + (int64_t)(t + m * T) * estride] = cvtc<R>(x[m]);
+        if constexpr (X64) {
+            // the dots below pair the state with stored values: use the stored one too
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) x[m] = cvtc<CR>(cvtc<R>(x[m]));
+        }
     }
 
     // ---- Gram-Schmidt dots on the stored representation: <prev_j|v> for
@@ -1735,7 +1800,7 @@ __device__ __forceinline__ void pass_body(C<R> *data, const typename TwT<R>::T *
     // sums over the wavefronts (in wavefront order: deterministic). ----
     if (POTK && (fl & PF_GS_DOTS)) {
         __shared__ double gsred[2 * QND_GS_CHUNK * 16];
-        constexpr int G = QND_GS_G(R);
+        constexpr int G = QND_GS_G(CR);
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         const int nw = (blockDim.x + 63) >> 6;
         const int nq = a.nprev + 1;
@@ -1794,6 +1859,20 @@ __global__ void __launch_bounds__(QND_PASS_THREADS)
            const PassArgs<R> a)
 {
     pass_body<R, KIND, DIR1, VPT, true>(data, tw, S, geo, fl_arg, nblk, a, (int)blockIdx.x);
+}
+
+// The same pass for a complex64 state held in FP64 registers (the complex64
+// storage, FP64 arithmetic precision): the arguments and tables of the
+// complex64 pass, each value widened at its load and rounded once at its
+// store.  On gfx950, with packed FP32 arithmetic off, an FP64 FMA issues at
+// the FP32 rate, so the transforms and factors cost no more VALU time and
+// need none of the compensated FP32 products.
+template <int KIND, int DIR1, int VPT>
+__global__ void __launch_bounds__(QND_PASS_THREADS)
+    k_pass_x(C<float> *data, const float4 *tw, int64_t S, unsigned geo, unsigned fl_arg, int nblk,
+             const PassArgs<float> a)
+{
+    pass_body<float, KIND, DIR1, VPT, true, true>(data, tw, S, geo, fl_arg, nblk, a, (int)blockIdx.x);
 }
 
 // One launch of a pass over several grids ("classes": e.g. the instances of
@@ -1862,6 +1941,15 @@ __global__ void __launch_bounds__(QND_PASS_THREADS) k_pass_group(const GroupArgs
     template __global__ void k_pass<R, K, D, 4>(QND_PASS_ARGS(R));                                  \
     template __global__ void k_pass<R, K, D, 8>(QND_PASS_ARGS(R));                                  \
     template __global__ void k_pass<R, K, D, 12>(QND_PASS_ARGS(R));
+#define QND_PASSX_ARGS C<float> *, const float4 *, int64_t, unsigned, unsigned, int, const PassArgs<float>
+#define QND_PASSX_EXTERN(R, K, D)                                                                    \
+    extern template __global__ void k_pass_x<K, D, 4>(QND_PASSX_ARGS);                              \
+    extern template __global__ void k_pass_x<K, D, 8>(QND_PASSX_ARGS);                              \
+    extern template __global__ void k_pass_x<K, D, 12>(QND_PASSX_ARGS);
+#define QND_PASSX_INSTANTIATE(R, K, D)                                                               \
+    template __global__ void k_pass_x<K, D, 4>(QND_PASSX_ARGS);                                     \
+    template __global__ void k_pass_x<K, D, 8>(QND_PASSX_ARGS);                                     \
+    template __global__ void k_pass_x<K, D, 12>(QND_PASSX_ARGS);
 #define QND_GROUP_EXTERN(R, K, D) extern template __global__ void k_pass_group<R, K, D>(const GroupArgs<R> *);
 #define QND_GROUP_INSTANTIATE(R, K, D) template __global__ void k_pass_group<R, K, D>(const GroupArgs<R> *);
 
@@ -1870,6 +1958,8 @@ QND_PASS_SET_0(QND_PASS_EXTERN, float) QND_PASS_SET_0(QND_PASS_EXTERN, double)
 QND_PASS_SET_1(QND_PASS_EXTERN, float) QND_PASS_SET_1(QND_PASS_EXTERN, double)
 QND_PASS_SET_2(QND_PASS_EXTERN, float) QND_PASS_SET_2(QND_PASS_EXTERN, double)
 QND_GROUP_SET(QND_GROUP_EXTERN, float) QND_GROUP_SET(QND_GROUP_EXTERN, double)
+QND_PASS_SET_0(QND_PASSX_EXTERN, float) QND_PASS_SET_1(QND_PASSX_EXTERN, float)
+QND_PASS_SET_2(QND_PASSX_EXTERN, float)
 #endif
 
 // advances the device step counter of a replayed step sequence

@@ -27,7 +27,7 @@ def candidates(nr, precision):
     pow2 = all(n & (n - 1) == 0 for n in nr)
     vpts = [4, 8] if pow2 else [0]
     tw_c = [0, 1, 2, 4]
-    tw_s = [0, 1, 2, 4, 8, 16] if precision in ("c64", 1) else [0, 1, 2, 4, 8]
+    tw_s = [0, 1, 2, 4, 8, 16] if precision in ("c64", "c64f64", 1, 2) else [0, 1, 2, 4, 8]
     if len(nr) == 1:
         tw_s = [0]
     return list(itertools.product(vpts, tw_c, tw_s))
@@ -50,7 +50,7 @@ def device_key():
 
 
 def cache_key(grid, nb, gauge, cands):
-    prec = {0: "c128", 1: "c64"}.get(grid.precision, str(grid.precision))
+    prec = {0: "c128", 1: "c64", 2: "c64f64"}.get(grid.precision, str(grid.precision))
     return json.dumps({"device": device_key(), "nr": list(grid.nr), "dr": [float(d) for d in grid.dr],
                        "precision": prec, "nb": int(nb), "gauge": gauge,
                        "candidates": [list(c) for c in cands]}, sort_keys=True)
