@@ -385,10 +385,10 @@ def eig1d_run(q):
 # W/cm^2 at 20 degrees, seeded relative phase, with dipole, acceleration and
 # the cos^(1/8) mask every step.  Few steps (the per-step cost is stationary).
 # ---------------------------------------------------------------------
-def big3d_run(q, F, imag_steps, real_steps, fixed=None):
+def big3d_run(q, F, imag_steps, real_steps, fixed=None, prec="c64"):
     nr, dr = (512, 512, 512), 0.2
     N = 512 ** 3
-    g = q.Grid(nr, dr, "c64", n_slots=8)
+    g = q.Grid(nr, dr, prec, n_slots=8)
     g.set_potential("H_3D", softcore=0.05)
     g.set_integrator("BM6", 0.05)
     x = -0.5 * 512 * dr + np.arange(512) * dr
@@ -441,7 +441,7 @@ def big3d_run(q, F, imag_steps, real_steps, fixed=None):
     g.synchronize()
     t_real = (time.perf_counter() - t0) / real_steps
     g.close()
-    return {"grid": list(nr), "softcore_a": 0.05, "imag_BM6_ms_per_step_state7": t_imag * 1e3,
+    return {"grid": list(nr), "precision": prec, "softcore_a": 0.05, "imag_BM6_ms_per_step_state7": t_imag * 1e3,
             "imag_grid_point_steps_per_s": N / t_imag,
             "real_BM4_ms_per_step": t_real * 1e3, "real_grid_point_steps_per_s": N / t_real,
             "real_step_roofline_frac": 14 * 2 * 8 * (N / t_real) / (HBM_PEAK_GBS * 1e9),
@@ -731,6 +731,8 @@ def main():
     ap.add_argument("--fd-samples", type=int, default=16)
     ap.add_argument("--skip-1d", action="store_true", help="skip the 1-D eigenstate workload")
     ap.add_argument("--big3d-steps", type=int, default=6, help="timed steps of the 512^3 workload (0 skips it)")
+    ap.add_argument("--big3d-prec", choices=["c64", "c64f64"], default="c64",
+                    help="precision of the 512^3 workload")
     ap.add_argument("--tuning3d", default="",
                     help="vpt,tw_contig,tw_strided of the 512^3 real-time step instead of the autotuner")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -759,7 +761,8 @@ def main():
     fixed3d = tuple(int(v) for v in args.tuning3d.split(",")) if args.tuning3d else None
     if args.only:
         runs = {"eig1d": lambda: eig1d_run(q),
-                "big3d": lambda: big3d_run(q, F, max(args.big3d_steps, 1), max(args.big3d_steps, 1), fixed3d),
+                "big3d": lambda: big3d_run(q, F, max(args.big3d_steps, 1), max(args.big3d_steps, 1), fixed3d,
+                                           args.big3d_prec),
                 "adjoint": lambda: adjoint_run(q, args.adjoint_steps, args.fd_samples),
                 "sweep": lambda: sweep_run(q, F, args.sweep_steps, 10, compare_steps=200)}
         print(json.dumps({"workload": args.only, "result": runs[args.only]()}))
@@ -917,7 +920,7 @@ def main():
         log(f"  max |E_gpu - E_oracle| = {out['eig1d_4096']['seeded_gaussian']['max_abs_diff_Ha']:.2e} Ha")
     if args.big3d_steps > 0:
         log(f"3-D 512^3: {args.big3d_steps} imaginary and real steps ...")
-        out["big3d_512"] = big3d_run(q, F, args.big3d_steps, args.big3d_steps, fixed3d)
+        out["big3d_512"] = big3d_run(q, F, args.big3d_steps, args.big3d_steps, fixed3d, args.big3d_prec)
         out["big3d_512"]["tile_probes"] = tile_probes()
         log(f"  real {out['big3d_512']['real_BM4_ms_per_step']:.2f} ms/step")
     if args.adjoint_steps > 0:

@@ -123,7 +123,8 @@ int qnd_observables(qnd_ctx *ctx, int slot, const int *codes, int ncodes,
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * Computes the fused position-space codes QND_OBS_NORM, QND_OBS_DIPOLE_R and
  * QND_OBS_DIPOLE_A (others: QND_EUNSUPPORTED) of the state psi_dev, a device
- * array of N elements of the context's precision (float2 / double2,
+ * array of N elements of the context's precision (float2 for QND_C64 and
+ * QND_C64F64, double2 for QND_C128,
  * row-major, position space), into out_dev, a device array of the codes'
  * widths (as qnd_observables).  Efield (host, dim values or NULL) is E(t) of
  * dipole_a.  It calls no allocating routine: its reduction partials and sums
