@@ -48,7 +48,7 @@ typedef struct {
     int guess_mode;       /* GUESS reference|gaussian */
     uint64_t seed;        /* SEED */
     double softcore;      /* SOFTCORE a: V=-../sqrt(r^2+a^2); <0 = reference */
-    int precision;        /* PRECISION c128|c64 */
+    int precision;        /* PRECISION c128|c64|c64f64 (QND_C128, QND_C64, QND_C64F64) */
     double mask_frac;     /* MASK f: cos^(1/8) ramp on the outer f of each half-width */
     int chunk;            /* STEPS_PER_CALL: real-time steps per library call */
 } Parameters;

@@ -6,8 +6,8 @@
  *
  * Extension keys (not in the reference; files that do not use them parse
  * exactly as in the reference):
- *   params.imag: GUESS reference|gaussian, SEED n, SOFTCORE a, PRECISION c128|c64
- *   params.real: MASK f, PRECISION c128|c64, STEPS_PER_CALL n
+ *   params.imag: GUESS reference|gaussian, SEED n, SOFTCORE a, PRECISION c128|c64|c64f64
+ *   params.real: MASK f, PRECISION c128|c64|c64f64, STEPS_PER_CALL n
  */
 #include "qnd_host.h"
 
@@ -157,7 +157,8 @@ static int parse_precision(const char *s)
 {
     if (strcmp(s, "c128") == 0) return 0;
     if (strcmp(s, "c64") == 0) return 1;
-    printf("\nUnknown precision: %s (c128 or c64)\n", s);
+    if (strcmp(s, "c64f64") == 0) return 2;
+    printf("\nUnknown precision: %s (c128, c64 or c64f64)\n", s);
     qnd_abort();
     return 0;
 }

@@ -197,13 +197,24 @@ def embed(states, nr_real, coeffs):
     return out
 
 
-@pytest.mark.parametrize("gauge", ["length", "velocity"])
-def test_imag_then_real_executables_2d(tmp_path, gauge):
+# real-time precision of the chain and its tolerances against the FP64
+# oracle: observables (of the largest value), final state (rel. L2), PED
+# (relative), PMD phase (radians, where the density is above `big` of its
+# maximum: a complex64 amplitude carries an absolute error, so its phase is
+# compared only where the amplitude is large)
+PREC_TOL = {"c128": dict(obs=5e-12, wf=1e-12, ped=1e-10, phase=1e-8, big=1e-10),
+            "c64f64": dict(obs=2e-5, wf=1e-5, ped=1e-4, phase=1e-3, big=1e-4)}
+
+
+@pytest.mark.parametrize("gauge,prec", [("length", "c128"), ("velocity", "c128"), ("length", "c64f64")])
+def test_imag_then_real_executables_2d(tmp_path, gauge, prec):
     """A 2-D chain: QuantumND_imag (32 x 16, two seeded states, soft-core
     extension in its side file) then QuantumND_real on a larger 64 x 32 grid
     (centred embedding, a complex superposition), CAP, a two-mode field with
     two envelopes, every observable kind recorded every step, then PED, PMD
-    and the final wavefunction.  Every file against the oracle."""
+    and the final wavefunction.  Every file against the oracle.  The real-time
+    leg runs in complex128 or, through the PRECISION key, in c64f64."""
+    tol = PREC_TOL[prec]
     (tmp_path / "params.imag").write_text(IMAG_2D)
     run(IMAG, ["params.imag"], tmp_path)
     hdr, data = read_output_imag(tmp_path / "output.imag")
@@ -217,7 +228,7 @@ def test_imag_then_real_executables_2d(tmp_path, gauge):
         assert same_up_to_phase(states[j], psi_i[j], 1e-12), (j, rel(states[j], psi_i[j]))
     assert np.allclose([float(v) for v in hdr["E"]], E_i[:, 0], rtol=0, atol=1e-10)
 
-    (tmp_path / "params.real").write_text(REAL_2D.format(gauge=gauge))
+    (tmp_path / "params.real").write_text(REAL_2D.format(gauge=gauge) + f"PRECISION\t{prec}\n")
     out = run(REAL, ["params.real"], tmp_path)
     nr = (64, 32)
     dr = 0.4
@@ -228,7 +239,8 @@ def test_imag_then_real_executables_2d(tmp_path, gauge):
     E2 = o.energies(psi0, 2)
     m = re.search(r"norm\t+([-0-9.e]+)\n\s*E\t+([-0-9.e]+)", out)
     assert m, out[-2000:]
-    assert abs(float(m.group(1)) - nrm) < 1e-12 and abs(float(m.group(2)) - E2[0]) < 1e-12
+    etol = 1e-12 if prec == "c128" else 1e-6
+    assert abs(float(m.group(1)) - nrm) < etol and abs(float(m.group(2)) - E2[0]) < etol
     # field tables at the stage times (src/integrators.c:74-129), field at t_n
     dt, T = 0.05, 3.0
     nt = int(T / dt)
@@ -248,26 +260,26 @@ def test_imag_then_real_executables_2d(tmp_path, gauge):
     obs = np.loadtxt(tmp_path / "observables.real", skiprows=1)
     assert obs.shape == (nt, 1 + rec.shape[1]), obs.shape
     assert np.allclose(obs[:, 0], time, atol=1e-15)
-    assert np.allclose(obs[:, 1:], rec, rtol=0, atol=5e-12 * max(1.0, np.abs(rec).max())), \
+    assert np.allclose(obs[:, 1:], rec, rtol=0, atol=tol["obs"] * max(1.0, np.abs(rec).max())), \
         np.abs(obs[:, 1:] - rec).max()
     # wavefunction.real: the final state before the PMD mask
     whdr, wv = read_binary_after(tmp_path / "wavefunction.real", b"PSI_BINARY_START\n")
     assert whdr["DIMENSION"] == ["2"] and whdr["NR"] == ["64", "32"]
     wf = wv.view(np.complex128).reshape(nr)
-    assert rel(wf, psi_T) < 1e-12, rel(wf, psi_T)
+    assert rel(wf, psi_T) < tol["wf"], rel(wf, psi_T)
     # PED.real and PMD.real
     Rm, pm, Ne, de = (3.0, 6.0), (2.0, 1.5), 60, 0.02
     psi_p, ped, pmd, npr = o.final_observables(psi_T, Rm, pm, Ne, de)
     P = np.loadtxt(tmp_path / "PED.real")
     assert P.shape == (Ne, 2) and np.allclose(P[:, 0], np.arange(Ne) * de, atol=1e-15)
-    assert np.allclose(P[:, 1], ped, rtol=1e-10, atol=1e-14 * max(ped.max(), 1e-300))
+    assert np.allclose(P[:, 1], ped, rtol=tol["ped"], atol=1e-4 * tol["ped"] * max(ped.max(), 1e-300))
     phdr, pv = read_binary_after(tmp_path / "PMD.real", b"DATA_START\n")
     assert phdr["DIMENSION"] == ["2"] and [int(v) for v in phdr["NPR"]] == list(npr)
     assert np.allclose([float(v) for v in phdr["DP"]], [2 * np.pi / (dr * n) for n in nr], rtol=1e-15)
     got = pv.reshape(-1, 2)
     assert got.shape == pmd.shape
     dens = pmd[:, 0]
-    assert np.allclose(got[:, 0], dens, rtol=1e-10, atol=1e-14 * dens.max())
-    big = dens > 1e-10 * dens.max()
+    assert np.allclose(got[:, 0], dens, rtol=tol["ped"], atol=1e-4 * tol["ped"] * dens.max())
+    big = dens > tol["big"] * dens.max()
     dphi = np.angle(np.exp(1j * (got[big, 1] - pmd[big, 1])))
-    assert np.abs(dphi).max() < 1e-8, np.abs(dphi).max()
+    assert np.abs(dphi).max() < tol["phase"], np.abs(dphi).max()
