@@ -39,7 +39,7 @@ def oracle_field(o, nt, dim, modes):
 
 
 # ---------------------------------------------------------------- named potentials
-@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("prec", ["c128", "c64", "c64f64"])
 @pytest.mark.parametrize("pot", ["Ne_short_2D", "Ar_2D"])
 def test_named_potential_imag_and_real(gpu, pot, prec):
     """Imaginary time with Gram-Schmidt (two states), then length-gauge real
@@ -91,12 +91,12 @@ def test_named_potential_imag_and_real(gpu, pot, prec):
 
 
 # ---------------------------------------------------------------- the 3-D configuration's imaginary leg
-@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("prec", ["c128", "c64", "c64f64"])
 def test_3d_bm6_eight_states(gpu, prec):
     """H_3D with soft-core length 0.05, BM6, eight states relaxed one after
     another with Gram-Schmidt after every step: the eighth state's projection
     uses the one-pass Gram matrix of seven previous states (k_gram<R, 7>),
-    in complex64 the FP32-state path with FP64 reductions."""
+    in complex64 (and c64f64) the FP32-state path with FP64 reductions."""
     nr, dr, dt, nsteps = (32, 32, 32), 0.25, 0.05, 6
     print(f"case 3-D BM6 eight states {prec}", flush=True)
     o = O.Oracle(nr, dr, "H_3D", "BM6", dt, softcore=0.05)
@@ -125,7 +125,7 @@ def test_3d_bm6_eight_states(gpu, prec):
     g.close()
 
 
-@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("prec", ["c128", "c64", "c64f64"])
 @pytest.mark.parametrize("gauge", ["length", "velocity"])
 def test_3d_angled_two_colour_field(gpu, gauge, prec):
     """The 3-D configuration's field: 800 nm at 1e14 W/cm^2 plus its second
