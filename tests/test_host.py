@@ -110,6 +110,7 @@ TINTEGRATION \t2.
     (IMAG_OK.replace("BM4", "BM5"), [], "Unknown integrator: BM5"),
     (IMAG_OK.replace("H_1D", "He_1D"), [], "Unknown potential: He_1D"),
     (IMAG_OK.replace("N_STATES \t2", "N_STATES \t0"), [], "Could not read or interpret"),
+    (IMAG_OK + "PRECISION\tc32\n", [], "Unknown precision: c32 (c128, c64 or c64f64)"),
 ])
 def test_imag_error_paths(tmp_path, text, args, msg):
     (tmp_path / "params.imag").write_text(text)
@@ -124,8 +125,9 @@ def test_imag_usage(tmp_path):
 
 
 @pytest.mark.skipif(have_gpu(), reason="checks the no-GPU failure path")
-def test_imag_parses_then_needs_gpu(tmp_path):
-    (tmp_path / "params.imag").write_text(IMAG_OK)
+@pytest.mark.parametrize("prec", ["", "PRECISION\tc64\n", "PRECISION\tc64f64\n"])
+def test_imag_parses_then_needs_gpu(tmp_path, prec):
+    (tmp_path / "params.imag").write_text(IMAG_OK + prec)
     rc, out = run(IMAG, ["params.imag", "DT", "0.05", "OUTPUT_IMAG", "x.imag"], tmp_path)
     assert rc == 1
     assert "DT\t\t\t0.050000000000000" in out and "OUTPUT_IMAG\t\tx.imag" in out
