@@ -149,8 +149,8 @@ def sweep_run(q, F, steps, warmup, compare_steps=1000):
         size classes (qnd_group_real_steps), the steps after the first a
         replayed HIP graph (also timed without the graph over compare_steps);
       graph_branches: qnd_group_real_steps in its branch form: each size
-        class's own kernels on its own stream, forked and joined every step,
-        eight steps of all 96 instances per launch of a captured HIP graph;
+        class's own kernels on its own stream, forked and joined once per
+        chunk, 32 steps of all 96 instances per launch of a captured HIP graph;
       three_contexts: one batched context per size class (one launch per
         pass per class), the three on three host threads.
     `value` is the fastest form."""
@@ -233,8 +233,8 @@ def sweep_run(q, F, steps, warmup, compare_steps=1000):
                                     "replayed per step",
                           "without_graph": {"steps": compare_steps, "value": pts * compare_steps / wall_ng}},
            "graph_branches": {"value": vb, "wall_s": wallb,
-                              "launch": "each size class's own kernels on its own stream, forked and joined every "
-                                        "step (qnd_group_real_steps, branch form); eight steps of all three classes "
+                              "launch": "each size class's own kernels on its own stream, forked and joined once per "
+                                        "chunk (qnd_group_real_steps, branch form); 32 steps of all three classes "
                                         "per launch of a captured HIP graph"},
            "three_contexts": {"value": v3, "wall_s": wall3,
                               "launch": "one batched context per size (one launch per pass per size), three "
