@@ -30,7 +30,8 @@ def test_transform(gpu, shape):
 
 
 @pytest.mark.parametrize("nr,dr,pot,gauge", [((64, 64), 0.4, "Ne_2D", "length"), ((64, 64), 0.4, "H_2D", "velocity"),
-                                           ((16, 32, 16), 0.5, "H_3D", "length"), ((256,), 0.2, "H_1D", "length")])
+                                           ((16, 32, 16), 0.5, "H_3D", "length"), ((256,), 0.2, "H_1D", "length"),
+                                           ((48, 96), 0.4, "Ne_2D", "length"), ((96,), 0.3, "H_1D", "velocity")])
 def test_steps_match_oracle(gpu, nr, dr, pot, gauge):
     """Imaginary time with Gram-Schmidt, then real time with CAP, mask and a
     field, against the FP64 oracle; the complex64 path's error on the same
@@ -72,10 +73,12 @@ def test_steps_match_oracle(gpu, nr, dr, pot, gauge):
     assert e_psi <= errs["c64"][1] and e_obs <= errs["c64"][2], errs
 
 
-def test_batched_and_merged_paths(gpu):
-    """The batched step (one launch over instances, per-instance fields) and
-    the merged step boundary of long 2-D calls, against complex128."""
-    print("case c64f64 batched + merged boundary", flush=True)
+@pytest.mark.parametrize("gauge", ["length", "velocity"])
+def test_batched_and_merged_paths(gpu, gauge):
+    """The batched step (one launch over instances, per-instance fields) and,
+    in the length gauge, the merged step boundary of long 2-D calls, against
+    complex128."""
+    print(f"case c64f64 batched + merged boundary {gauge}", flush=True)
     nr, dr, dt, nt, nb = (48, 64), 0.4, 0.05, 150, 3
     o = O.Oracle(nr, dr, "H_2D", "BM4", dt, cap=(0.05, 1.0))
     psi = smooth_state(o, 9)[0]
@@ -92,11 +95,11 @@ def test_batched_and_merged_paths(gpu):
                           np.stack([t[2] for t in tabs]))
         for b in range(nb):
             g.upload(b, psi)
-        ob = g.real_steps_batch(0, nb, "length", 0, nt, names)
+        ob = g.real_steps_batch(0, nb, gauge, 0, nt, names)
         # one instance alone through the single-instance path (merged boundary)
         g.set_field(*tabs[1])
         g.upload(0, psi)
-        o1 = g.real_steps(0, "length", 0, nt, names)
+        o1 = g.real_steps(0, gauge, 0, nt, names)
         out[prec] = (ob, o1, g.download(0))
         g.close()
     (ob, o1, p1), (rb, r1, rp) = out["c64f64"], out["c128"]
