@@ -61,6 +61,25 @@ def test_mode_struct_matches_header(tmp_path):
     assert q.ENVELOPES == {"constant": 0, "trapz": 1, "cos2": 2, "cos4": 3}
 
 
+def test_constants_match_header(tmp_path):
+    """The Python binding's precision, gauge and observable codes are the
+    enum values of include/qnd.h (a C program compiled against it prints
+    them)."""
+    src = tmp_path / "c.c"
+    src.write_text('#include <stdio.h>\n#include "qnd.h"\n'
+                   'int main(void){printf("%d %d %d %d %d %d %d %d %d %d\\n", QND_C128, QND_C64, QND_C64F64,'
+                   ' QND_GAUGE_LENGTH, QND_GAUGE_VELOCITY, QND_OBS_DIPOLE_R, QND_OBS_DIPOLE_V, QND_OBS_NORM,'
+                   ' QND_OBS_DIPOLE_A, QND_OBS_LZ); return 0;}\n')
+    exe = tmp_path / "c"
+    r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    assert got == [q.PRECISIONS["c128"], q.PRECISIONS["c64"], q.PRECISIONS["c64f64"],
+                   q.GAUGES["length"], q.GAUGES["velocity"], q.OBS["dipole_r"], q.OBS["dipole_v"], q.OBS["norm"],
+                   q.OBS["dipole_a"], q.OBS["Lz"]]
+
+
 @pytest.mark.skipif(have_gpu(), reason="checks the no-GPU failure path")
 def test_create_fails_loudly_without_gpu():
     with pytest.raises(q.QNDError, match="no HIP device"):
