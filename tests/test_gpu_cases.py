@@ -39,8 +39,13 @@ def oracle_field(o, nt, dim, modes):
 
 
 # ---------------------------------------------------------------- named potentials
+# complex64-storage bounds of the acceleration records per named potential
+# (see the test below)
+ACCEL_TOL = {"Ne_short_2D": 1e-4, "Ar_2D": 1e-4, "Kr_2D": 2e-3, "Xe_2D": 2e-2}
+
+
 @pytest.mark.parametrize("prec", ["c128", "c64", "c64f64"])
-@pytest.mark.parametrize("pot", ["Ne_short_2D", "Ar_2D"])
+@pytest.mark.parametrize("pot", ["Ne_short_2D", "Ar_2D", "Kr_2D", "Xe_2D"])
 def test_named_potential_imag_and_real(gpu, pot, prec):
     """Imaginary time with Gram-Schmidt (two states), then length-gauge real
     time with CAP, mask and field, on the potential's own device branch."""
@@ -87,7 +92,17 @@ def test_named_potential_imag_and_real(gpu, pot, prec):
     else:
         assert rel(got, ref_psi) < 1e-4, rel(got, ref_psi)
         scale = np.abs(ref_obs).max(axis=0) + 1e-3
-        assert (np.abs(got_obs - ref_obs) / scale).max() < 1e-4
+        err = (np.abs(got_obs - ref_obs) / scale).max(axis=0)
+        print(f"named potential {pot} {prec}: state {rel(got, ref_psi):.2e}, records {err}", flush=True)
+        # columns: dipole_r (2), dipole_a (2), norm (1), each against its largest value plus
+        # 1e-3.  dipole_a = -<grad V> - E<psi|psi> is a small difference of large terms for
+        # the deep potentials (|V(0)| = 19, 41 and 74 Ha for Ar, Kr and Xe), which magnifies
+        # the complex64 rounding of the stored state: measured 1.6e-5, 6.3e-4 and 7.7e-3 in
+        # complex64, 1.6e-6, 2.9e-4 and 5.7e-3 with FP64 pass arithmetic (c64f64), whose
+        # state error is ten times smaller; complex128 matches to 1e-11 (above).  The bounds
+        # catch a wrong potential branch or constant, which moves the records by O(1).
+        assert err[[0, 1, 4]].max() < 3e-4, err
+        assert err[[2, 3]].max() < ACCEL_TOL[pot], (err, ACCEL_TOL[pot])
 
 
 # ---------------------------------------------------------------- the 3-D configuration's imaginary leg
