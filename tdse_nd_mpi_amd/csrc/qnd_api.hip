@@ -124,6 +124,7 @@ struct qnd_ctx {
     int64_t kax_gen = -1;
     int kax_prec = -1;
     bool use_kax = true;                  // QND_KAX=0: kinetic factors formed per element instead
+    bool imag1d = true;                   // QND_IMAG1D=0: 1-D imaginary time in one launch per pass instead
     int gs_split = -1;                    // QND_GS_SPLIT=0/1: Gram-Schmidt fused / split (-1: by size)
     // descriptors of the group launches this context leads (qnd_group_real_steps)
     void *d_gdesc = nullptr;
@@ -1122,6 +1123,19 @@ template <typename R> void gs_split(qnd_ctx *c, int nprev, bool dots, C<R> *psi,
     }
 }
 
+// k_imag1d for a 1-D line of 2^log2n points in the automatic geometry's
+// values per thread: 256 to 2048 points (four) and 4096 (eight); nullptr for
+// the other lines (shorter, 3 * 2^k, or another geometry), which keep one
+// launch per pass
+template <typename R> const void *imag1d_fn(int vpt, int log2n)
+{
+#define QND_I1D(V, Lc)                                                                                              \
+    if (vpt == V && log2n == Lc) return reinterpret_cast<const void *>(&k_imag1d<R, V, Lc>);
+    QND_I1D(4, 8) QND_I1D(4, 9) QND_I1D(4, 10) QND_I1D(4, 11) QND_I1D(8, 12)
+#undef QND_I1D
+    return nullptr;
+}
+
 template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int gs)
 {
     const int d = c->dim, s = c->s;
@@ -1161,7 +1175,32 @@ template <typename R> int imag_steps(qnd_ctx *c, int slot, int64_t nsteps, int g
                     slot, (long long)h[2], h[1]);
     };
     int64_t chunks_out = 0;  // flag copies enqueued
-    for (int64_t it = 0; it < nsteps; ++it) {
+    // 1-D: the whole call in one launch of one workgroup (k_imag1d), for the
+    // line lengths it is compiled for
+    const void *k1d = d == 1 && c->imag1d && !c->x64 && !c->prof && s <= 10 && nsteps > 0 &&
+                              nprev <= QND_GRAM_MAX && g0.nlines == 1 && g0.TW == 1 && g0.T <= 512
+                          ? imag1d_fn<R>(g0.vpt, g0.log2n)
+                          : nullptr;
+    const bool one_launch = k1d != nullptr;
+    if (one_launch) {
+        Imag1D A;
+        memset(&A, 0, sizeof A);
+        for (int k = 0; k < 2 * s; ++k) A.as[k] = c->as_scaled[k];
+        A.s = s;
+        A.nprev = gs ? nprev : -1;
+        A.ldsStride = g0.ldsStride;
+        A.inv_n = inv_n0;
+        A.gs_scale = c->P.dNr * inv_n0;
+        const auto *tw = reinterpret_cast<const typename TwT<R>::T *>(c->plan[0].tw);
+        const size_t lds = 2 * (size_t)g0.ldsStride * sizeof(C<R>);
+        const C<R> *prev = state_ptr<R>(c, 0);
+        const double *gram = c->d_small + SMALL_GRAM;
+        int64_t slot_stride = c->N;
+        Phys P = c->P;
+        void *args[] = {&psi, &prev, &slot_stride, &tw, &P, &A, &gram, &coef, &nsteps};
+        HIPCK(c, hipLaunchKernel(k1d, dim3(1), dim3(g0.T), args, lds, c->stream));
+    }
+    for (int64_t it = 0; it < (one_launch ? 0 : nsteps); ++it) {
         for (int ia = 0; ia < s; ++ia) {
             PassArgs<R> a = base_args<R>(c, psi, g0);
             a.scale1 = inv_n0;
@@ -2852,6 +2891,7 @@ int qnd_create(qnd_ctx **out, int dim, const int64_t *nr, const double *dr, int 
     if (const char *e = getenv("QND_WAVE_LINES")) c->wave_lines = atoi(e) != 0;
     if (const char *e = getenv("QND_ROUND_STATE")) c->round32 = precision == QND_C128 && atoi(e) != 0;
     if (const char *e = getenv("QND_KAX")) c->use_kax = atoi(e) != 0;
+    if (const char *e = getenv("QND_IMAG1D")) c->imag1d = atoi(e) != 0;
     if (const char *e = getenv("QND_GS_SPLIT")) c->gs_split = atoi(e) != 0 ? 1 : 0;
     if (const char *e = getenv("QND_VPT")) {  // tuning knob: values per thread
         const int v = atoi(e);

@@ -2004,13 +2004,6 @@ static __global__ void __launch_bounds__(1024) k_obs_finalize(const double *__re
     obs_finalize_instance(part, nwg, dim, dNr, spec, B, blockIdx.x, rec, red);
 }
 
-// Modified Gram-Schmidt coefficients from partial dots in the stored
-// representation (scale = dNr / n0 for the axis-0 Fourier representation):
-// b_j = <prev_j|v>, nv = <v|v>; c_j = b_j - sum_{i<j} c_i G_{j i};
-// |v'|^2 = nv - 2 Re sum_j c_j conj(b_j) + sum_{ij} conj(c_i) c_j G_{ij}.
-// coef = [c_0 .. c_{N-1} (re,im), invnorm, flag, norm, step of the first collapse]
-// `step` is the caller's index of the step whose dots these are: the first
-// collapse (norm below 1e-12) records it beside the sticky flag.
 // First-level sums of a [nrows][npart] partials array: block b sums the
 // rows of its fixed slice [b * per, (b + 1) * per), per = ceil(nrows /
 // gridDim.x), into out[b][0 .. npart).  The slices and the order inside one
@@ -2031,23 +2024,19 @@ static __global__ void __launch_bounds__(256) k_row_blocks(const double *__restr
     }
 }
 
-static __global__ void __launch_bounds__(1024) k_gs_coeffs(const double *__restrict__ part, int nwg, int npart,
-                                                    int nprev, double scale, const double *__restrict__ gram,
-                                                    double *coef, double step)
+// Modified Gram-Schmidt coefficients (one thread) from the scaled dots in
+// the stored representation (scale = dNr / n0 for the axis-0 Fourier
+// representation): b_j = <prev_j|v>, nv = <v|v>;
+// c_j = b_j - sum_{i<j} c_i G_{j i};
+// |v'|^2 = nv - 2 Re sum_j c_j conj(b_j) + sum_{ij} conj(c_i) c_j G_{ij}.
+// coef = [c_0 .. c_{N-1} (re,im), invnorm, flag, norm, step of the first collapse]
+// `step` is the caller's index of the step whose dots these are: the first
+// collapse (norm below 1e-12) records it beside the sticky flag, and the
+// return value says whether this step collapsed.
+__device__ __forceinline__ bool gs_coefficients(const double *b, int nprev, const double *__restrict__ gram,
+                                                double *cre, double *cim, double *coef, double step)
 {
-    __shared__ double b[2 * 65 + 8];
-    __shared__ double cre[64], cim[64];
-    __shared__ double red[16 * 8];
-    // column sums of the dot partials, eight columns at a time over the whole block
-    const int ncol = 2 * (nprev + 1);
-    for (int q0 = 0; q0 < ncol; q0 += 8) {
-        double tot[8];
-        block_column_sums<8>(part, nwg, npart, q0, tot, red);
-        if (threadIdx.x == 0)
-            for (int q = 0; q < 8 && q0 + q < ncol; ++q) b[q0 + q] = tot[q] * scale;
-    }
-    lds_barrier();
-    if (threadIdx.x == 0) {
+    {
         const double nv = b[2 * nprev];
         for (int j = 0; j < nprev; ++j) {
             double r = b[2 * j], im = b[2 * j + 1];
@@ -2079,11 +2068,31 @@ static __global__ void __launch_bounds__(1024) k_gs_coeffs(const double *__restr
             }
             coef[2 * nprev] = 0.0;
             coef[2 * nprev + 1] = 1.0;
-        } else {
-            coef[2 * nprev] = 1.0 / sqrt(n2);
-            if (coef[2 * nprev + 1] == 0.0) coef[2 * nprev + 2] = n2;
+            return true;
         }
+        coef[2 * nprev] = 1.0 / sqrt(n2);
+        if (coef[2 * nprev + 1] == 0.0) coef[2 * nprev + 2] = n2;
+        return false;
     }
+}
+
+static __global__ void __launch_bounds__(1024) k_gs_coeffs(const double *__restrict__ part, int nwg, int npart,
+                                                    int nprev, double scale, const double *__restrict__ gram,
+                                                    double *coef, double step)
+{
+    __shared__ double b[2 * 65 + 8];
+    __shared__ double cre[64], cim[64];
+    __shared__ double red[16 * 8];
+    // column sums of the dot partials, eight columns at a time over the whole block
+    const int ncol = 2 * (nprev + 1);
+    for (int q0 = 0; q0 < ncol; q0 += 8) {
+        double tot[8];
+        block_column_sums<8>(part, nwg, npart, q0, tot, red);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 8 && q0 + q < ncol; ++q) b[q0 + q] = tot[q] * scale;
+    }
+    lds_barrier();
+    if (threadIdx.x == 0) (void)gs_coefficients(b, nprev, gram, cre, cim, coef, step);
 }
 
 // ---------------------------------------------------------------------
@@ -2460,6 +2469,163 @@ __global__ void __launch_bounds__(256) k_ped(const C<R> *psi, int64_t N, Phys P,
         }
         partials[(int64_t)blockIdx.x * Ne + j] = s;
     }
+}
+
+// ---------------------------------------------------------------------
+// 1-D imaginary time in one launch (src/integrators.c:135-163 and
+// src/QuantumND_imag.c:182-185, 265-311).  A 1-D line fits one workgroup, so
+// every pass of the multi-launch form would be a single workgroup waiting on
+// its own launch, load and store.  Here one workgroup keeps the line in
+// registers for all nsteps steps: per stage the inverse transform, the
+// potential factors, the forward transform and the kinetic factor, in the
+// operator order and arithmetic of the 1-D stage pass (PK_P1DI); after the
+// last potential half-step the Gram-Schmidt dots against the previous states
+// (read from memory), the coefficients of gs_coefficients and the update.
+// The state is read at entry and stored at exit, in the axis-0 Fourier
+// representation of the slots.  A collapse (norm below 1e-12) stops the loop
+// at that step, with the flag, norm and step in coef as from k_gs_coeffs.
+// ---------------------------------------------------------------------
+struct Imag1D {
+    double as[20];        // as_scaled: a_k h, k = 0 .. 2s-1 (s <= 10)
+    int s;
+    int nprev;            // previous states for Gram-Schmidt; -1: none
+    int ldsStride;
+    double inv_n;         // 1 / n (the inverse transform's scale)
+    double gs_scale;      // dNr / n (dots in the Fourier representation)
+};
+
+template <typename R, int VPT, int L>
+static __global__ void __launch_bounds__(512) k_imag1d(C<R> *psi, const C<R> *__restrict__ prev,
+                                                       int64_t slot_stride, const typename TwT<R>::T *tw,
+                                                       const Phys P, const Imag1D A,
+                                                       const double *__restrict__ gram, double *coef,
+                                                       int64_t nsteps)
+{
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    constexpr int NQ = 2 * (QND_GRAM_MAX + 1);
+    __shared__ double red[NQ * 16];
+    __shared__ double bq[NQ], cre[QND_GRAM_MAX], cim[QND_GRAM_MAX];
+    __shared__ double upd[2 * QND_GRAM_MAX + 2];  // coefficients, invnorm, stop
+    const int t = threadIdx.x, T = blockDim.x;
+    C<R> *bufA = reinterpret_cast<C<R> *>(smem_raw);
+    C<R> *bufB = bufA + A.ldsStride;
+    int wc = 0;
+    Tw4<R> tw4;
+    if constexpr (VPT == 4) tw4_load<R>(tw4, tw, L, T, t);
+    using TE = typename TwT<R>::T;
+
+    C<R> x[VPT];
+#pragma unroll
+    for (int m = 0; m < VPT; ++m) x[m] = psi[t + m * T];
+
+    // V (FP64, as in the stage pass: closed forms, FP64 in both precisions)
+    // at this thread's positions
+    double V[VPT];
+    {
+        double rr[VPT], dV[VPT];
+#pragma unroll
+        for (int m = 0; m < VPT; ++m) {
+            const double xc = coord_x(P, 0, t + m * T);
+            rr[m] = xc * xc;
+        }
+        potential_pts<double, VPT, !std::is_same<R, double>::value && QND_POT_F64>(P, rr, V, dV);
+    }
+    const R sc = (R)A.inv_n;
+    const int np = A.nprev;
+    const int lane = t & 63, wv = t >> 6, nw = (T + 63) >> 6;
+
+    for (int64_t it = 0; it < nsteps; ++it) {
+        for (int ia = 0; ia < A.s; ++ia) {
+            fft_fixed<R, 1, VPT, L, TE>(x, bufA, bufB, wc, t, tw, tw4, false);
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                C<R> v = scl<R>(x[m], sc);
+                if (ia > 0) v = scl<R>(v, real_exp<R>(-A.as[2 * ia - 1] * V[m]));
+                x[m] = scl<R>(v, real_exp<R>(-A.as[2 * ia] * V[m]));
+            }
+            fft_fixed<R, -1, VPT, L, TE>(x, bufA, bufB, wc, t, tw, tw4, false);
+            const double ck = A.as[2 * ia] + A.as[2 * ia + 1];
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                const double p = coord_p(P, 0, t + m * T);
+                x[m] = scl<R>(x[m], real_exp<R>(-ck * (.5 * p * p)));
+            }
+        }
+        // the last potential half-step
+        fft_fixed<R, 1, VPT, L, TE>(x, bufA, bufB, wc, t, tw, tw4, false);
+#pragma unroll
+        for (int m = 0; m < VPT; ++m)
+            x[m] = scl<R>(scl<R>(x[m], sc), real_exp<R>(-A.as[2 * A.s - 1] * V[m]));
+        fft_fixed<R, -1, VPT, L, TE>(x, bufA, bufB, wc, t, tw, tw4, false);
+        if (np < 0) continue;
+
+        // Gram-Schmidt: dots <prev_j|v> for j < np, then <v|v> (FP64; each
+        // summed over the wavefront and parked in LDS, then over the
+        // wavefronts in order)
+#pragma unroll 1
+        for (int j = 0; j <= np; ++j) {
+            double sr = 0.0, si = 0.0;
+            if (j < np) {
+                const C<R> *pj = prev + (int64_t)j * slot_stride;
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) {
+                    const C<R> p = pj[t + m * T];
+                    sr += (double)p.x * (double)x[m].x + (double)p.y * (double)x[m].y;
+                    si += (double)p.x * (double)x[m].y - (double)p.y * (double)x[m].x;
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < VPT; ++m) sr += (double)x[m].x * (double)x[m].x + (double)x[m].y * (double)x[m].y;
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            if (lane == 0) {
+                red[(2 * j) * 16 + wv] = sr;
+                red[(2 * j + 1) * 16 + wv] = si;
+            }
+        }
+        lds_barrier();
+        if (t == 0) {
+            for (int q = 0; q <= 2 * np + 1; ++q) {
+                double sum = 0.0;
+                for (int i = 0; i < nw; ++i) sum += red[q * 16 + i];
+                bq[q] = sum * A.gs_scale;
+            }
+            const bool collapsed = gs_coefficients(bq, np, gram, cre, cim, coef, (double)it);
+            for (int j = 0; j < np; ++j) {
+                upd[2 * j] = cre[j];
+                upd[2 * j + 1] = cim[j];
+            }
+            upd[2 * QND_GRAM_MAX] = coef[2 * np];
+            upd[2 * QND_GRAM_MAX + 1] = collapsed ? 1.0 : 0.0;
+        }
+        lds_barrier();
+        if (upd[2 * QND_GRAM_MAX + 1] != 0.0) break;  // uniform: the collapse step ends the call
+        // v' = (v - sum_j c_j prev_j) * invnorm, subtracting in the order j = 0, 1, ...
+        double vr[VPT], vi[VPT];
+#pragma unroll
+        for (int m = 0; m < VPT; ++m) {
+            vr[m] = (double)x[m].x;
+            vi[m] = (double)x[m].y;
+        }
+#pragma unroll 1
+        for (int j = 0; j < np; ++j) {
+            const C<R> *pj = prev + (int64_t)j * slot_stride;
+            const double cr = upd[2 * j], ci = upd[2 * j + 1];
+#pragma unroll
+            for (int m = 0; m < VPT; ++m) {
+                const C<R> p = pj[t + m * T];
+                vr[m] -= cr * (double)p.x - ci * (double)p.y;
+                vi[m] -= cr * (double)p.y + ci * (double)p.x;
+            }
+        }
+        const double gsinv = upd[2 * QND_GRAM_MAX];
+#pragma unroll
+        for (int m = 0; m < VPT; ++m) x[m] = mk<R>((R)(vr[m] * gsinv), (R)(vi[m] * gsinv));
+        lds_barrier();  // red and upd are rewritten in the next step
+    }
+#pragma unroll
+    for (int m = 0; m < VPT; ++m) psi[t + m * T] = x[m];
 }
 
 }  // namespace qnd
