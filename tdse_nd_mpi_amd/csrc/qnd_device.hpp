@@ -2475,8 +2475,9 @@ __global__ void __launch_bounds__(256) k_ped(const C<R> *psi, int64_t N, Phys P,
 // 1-D imaginary time in one launch (src/integrators.c:135-163 and
 // src/QuantumND_imag.c:182-185, 265-311).  A 1-D line fits one workgroup, so
 // every pass of the multi-launch form would be a single workgroup waiting on
-// its own launch, load and store.  Here one workgroup keeps the line in
-// registers for all nsteps steps: per stage the inverse transform, the
+// its own launch, load and store.  Here one workgroup of 2^L / VPT threads
+// keeps the 2^L-point line in registers for all nsteps steps: per stage the
+// inverse transform, the
 // potential factors, the forward transform and the kinetic factor, in the
 // operator order and arithmetic of the 1-D stage pass (PK_P1DI); after the
 // last potential half-step the Gram-Schmidt dots against the previous states
